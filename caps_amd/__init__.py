@@ -1,0 +1,462 @@
+"""caps_amd — the MI355X force-and-step path of CAPS, reached from Python through the capsgpu C ABI.
+
+The Python layer mirrors the reference's own interfaces for this path:
+  * Evaluator(ff, options).compute(x, cell) -> (EnergyTerms, forces)   core/include/caps/field.hpp:327-352
+  * run_dynamics(system, options) -> DynamicsReport                    core/include/caps/dynamics.hpp:180
+with the same meanings, units and error behaviour (FieldError for force-field failures, ValueError for bad
+arguments). Everything is computed by libcapsgpu.so (HIP kernels for gfx950); there is no CPU fallback: the module
+raises when the library is missing or no GPU is visible.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcapsgpu.so")
+
+NTERMS = 15
+TERM_NAMES = ("bond", "angle", "dihedral", "improper", "vdw", "coulomb", "virial", "wxx", "wyy", "wzz", "wxy", "wxz", "wyz",
+              "tail_energy", "tail_virial")
+
+
+class FieldError(RuntimeError):
+    """caps::FieldError (core/include/caps/field.hpp:17-19)"""
+
+
+class DeviceError(RuntimeError):
+    """a HIP runtime failure or no GPU"""
+
+
+class DynamicsCancelled(RuntimeError):
+    """caps::DynamicsCancelled (core/include/caps/dynamics.hpp:168)"""
+
+
+_lib = None
+
+
+class FF(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("n_types", C.c_int32), ("type", C.c_void_p), ("charge", C.c_void_p), ("mass", C.c_void_p),
+                ("eps", C.c_void_p), ("sigma", C.c_void_p), ("lj96", C.c_int32), ("excl_offset", C.c_void_p),
+                ("excl_index", C.c_void_p), ("p14_offset", C.c_void_p), ("p14_index", C.c_void_p), ("coul14", C.c_double),
+                ("n_bonds", C.c_int32), ("bond_atoms", C.c_void_p), ("bond_param", C.c_void_p), ("n_angles", C.c_int32),
+                ("angle_atoms", C.c_void_p), ("angle_param", C.c_void_p), ("n_fene", C.c_int32), ("fene_atoms", C.c_void_p),
+                ("fene_param", C.c_void_p)]
+
+
+class EOpts(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("skin", C.c_double), ("coulomb", C.c_int32), ("electrostatics", C.c_int32),
+                ("dsf_alpha", C.c_double), ("ewald_rtol", C.c_double), ("pme_spacing", C.c_double), ("pme_order", C.c_int32),
+                ("tail", C.c_int32), ("parts", C.c_int32), ("no_reciprocal", C.c_int32)]
+
+
+class DOpts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("steps", C.c_int64), ("temperature", C.c_double), ("thermostat", C.c_int32),
+                ("tau_t", C.c_double), ("seed", C.c_uint64), ("thermo_every", C.c_int32), ("new_velocities", C.c_int32),
+                ("respa", C.c_int32), ("step_offset", C.c_int64), ("learned_k", C.c_int32), ("learned_tol", C.c_double)]
+
+
+class Thermo(C.Structure):
+    _fields_ = [("step", C.c_int64), ("time_ps", C.c_double), ("temperature", C.c_double), ("kinetic", C.c_double),
+                ("potential", C.c_double), ("total", C.c_double), ("pressure", C.c_double), ("conserved", C.c_double),
+                ("terms", C.c_double * NTERMS), ("learned", C.c_int32), ("learned_rmse", C.c_double)]
+
+
+class DReport(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("seconds", C.c_double), ("ns_per_day", C.c_double), ("list_builds", C.c_int32),
+                ("prunes", C.c_int32), ("learned_steps", C.c_int32), ("full_steps", C.c_int32), ("fallbacks", C.c_int32)]
+
+
+class Sys(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("n_types", C.c_int32), ("cell", C.c_double * 12), ("x", C.POINTER(C.c_double)),
+                ("type", C.POINTER(C.c_int32)), ("charge", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
+                ("eps", C.POINTER(C.c_double)), ("sigma", C.POINTER(C.c_double)), ("n_bonds", C.c_int32),
+                ("bond_atoms", C.POINTER(C.c_int32)), ("bond_param", C.POINTER(C.c_double)), ("n_angles", C.c_int32),
+                ("angle_atoms", C.POINTER(C.c_int32)), ("angle_param", C.POINTER(C.c_double)), ("n_fene", C.c_int32),
+                ("fene_atoms", C.POINTER(C.c_int32)), ("fene_param", C.POINTER(C.c_double)), ("excl_offset", C.POINTER(C.c_int32)),
+                ("excl_index", C.POINTER(C.c_int32)), ("opaque", C.c_void_p)]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(Thermo), C.c_void_p)
+
+
+def lib():
+    """the loaded libcapsgpu.so; raises when it is not built"""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libcapsgpu.so is not built: run `python -m caps_amd.build` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        L.capsgpu_last_error.restype = C.c_char_p
+        L.capsgpu_build_error.restype = C.c_char_p
+        L.capsgpu_create.argtypes = [C.POINTER(FF), C.POINTER(EOpts), C.POINTER(C.c_void_p)]
+        L.capsgpu_destroy.argtypes = [C.c_void_p]
+        L.capsgpu_compute.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+        L.capsgpu_list_builds.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+        L.capsgpu_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.capsgpu_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.capsgpu_get_forces.argtypes = [C.c_void_p, C.c_void_p]
+        L.capsgpu_run_dynamics.argtypes = [C.c_void_p, C.POINTER(DOpts), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                           C.POINTER(C.c_int32), C.POINTER(DReport)]
+        L.capsgpu_list_sizes.argtypes = [C.c_void_p, C.c_void_p]
+        L.capsgpu_list_get.argtypes = [C.c_void_p] + [C.c_void_p] * 13
+        L.capsgpu_list_pruned.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.capsgpu_stage_times.argtypes = [C.c_void_p, C.c_void_p]
+        L.capsgpu_build_melt.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.POINTER(Sys))]
+        L.capsgpu_build_kg.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(C.POINTER(Sys))]
+        L.capsgpu_system_free.argtypes = [C.POINTER(Sys)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc == 0:
+        return
+    msg = lib().capsgpu_last_error().decode()
+    if rc == -2:
+        raise FieldError(msg)
+    if rc == -3:
+        raise DeviceError(msg)
+    if rc == -4:
+        raise DynamicsCancelled(msg)
+    raise ValueError(msg)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---- data model (the parts of caps::ForceField / caps::System this path reads) ----------------------------------------
+@dataclass
+class ForceField:
+    """the flattened force field: per atom type, charge, mass; per type pair mixed ε and σ; bonded terms; exclusions"""
+    type: np.ndarray
+    charge: np.ndarray
+    mass: np.ndarray
+    eps: np.ndarray            # nt x nt
+    sigma: np.ndarray          # nt x nt
+    excl_offset: np.ndarray
+    excl_index: np.ndarray
+    bond_atoms: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))
+    bond_param: np.ndarray = field(default_factory=lambda: np.zeros((0, 2)))
+    angle_atoms: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int32))
+    angle_param: np.ndarray = field(default_factory=lambda: np.zeros((0, 2)))
+    fene_atoms: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))
+    fene_param: np.ndarray = field(default_factory=lambda: np.zeros((0, 4)))
+    p14_offset: np.ndarray = None
+    p14_index: np.ndarray = None
+    coul14: float = 1.0 / 1.2
+    lj96: bool = False
+    lj_shift: bool = False     # ForceField::lj_shift: LJ shifted to zero at the cut-off, no tail correction
+    cutoff: float = 0.0        # ForceField::cutoff: the model's own cut-off (0: the options')
+
+    @property
+    def n(self):
+        return int(self.type.size)
+
+    @property
+    def ntypes(self):
+        return int(self.eps.shape[0])
+
+
+@dataclass
+class System:
+    x: np.ndarray              # (N, 3) Å
+    cell: np.ndarray           # (12,) origin, a, b, c
+    ff: ForceField
+    v: np.ndarray = None       # (N, 3) Å/fs
+
+
+@dataclass
+class EnergyOptions:
+    """caps::EnergyOptions (core/include/caps/field.hpp:274-308)"""
+    cutoff: float = 10.0
+    skin: float = 2.0
+    coulomb: bool = True
+    electrostatics: str = "dsf"   # "dsf" or "pme"
+    dsf_alpha: float = 0.2
+    ewald_rtol: float = 1e-5
+    pme_spacing: float = 1.0
+    pme_order: int = 5
+    tail: bool = True
+    parts: int = 3
+    no_reciprocal: bool = False   # checks only: the real-space part of PME alone
+
+
+@dataclass
+class EnergyTerms:
+    """caps::EnergyTerms (core/include/caps/field.hpp:310-320)"""
+    bond: float = 0.0
+    angle: float = 0.0
+    dihedral: float = 0.0
+    improper: float = 0.0
+    vdw: float = 0.0
+    coulomb: float = 0.0
+    virial: float = 0.0
+    w: tuple = (0.0,) * 6
+    tail_energy: float = 0.0
+    tail_virial: float = 0.0
+
+    def total(self):
+        return self.bond + self.angle + self.dihedral + self.improper + self.vdw + self.coulomb
+
+    @staticmethod
+    def from_array(t):
+        t = [float(v) for v in t]
+        return EnergyTerms(t[0], t[1], t[2], t[3], t[4], t[5], t[6], tuple(t[7:13]), t[13], t[14])
+
+
+class Evaluator:
+    """caps::Evaluator on one GPU (core/include/caps/field.hpp:327-352): holds the device copy of the force field and
+    of the pair list between calls; compute() returns the energy terms and the forces."""
+
+    def __init__(self, ff, options=None):
+        self.ff = ff
+        o = options or EnergyOptions()
+        self.options = o
+        L = lib()
+        f = FF()
+        self._keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        f.n_atoms, f.n_types = ff.n, ff.ntypes
+        f.type, f.charge, f.mass = arr(ff.type, np.int32), arr(ff.charge, np.float64), arr(ff.mass, np.float64)
+        f.eps, f.sigma = arr(ff.eps, np.float64), arr(ff.sigma, np.float64)
+        f.lj96 = int(ff.lj96)
+        f.excl_offset, f.excl_index = arr(ff.excl_offset, np.int32), arr(ff.excl_index if ff.excl_index.size else [0], np.int32)
+        if ff.p14_offset is not None:
+            f.p14_offset, f.p14_index = arr(ff.p14_offset, np.int32), arr(ff.p14_index if ff.p14_index.size else [0], np.int32)
+        f.coul14 = ff.coul14
+        f.n_bonds = len(ff.bond_atoms)
+        f.bond_atoms, f.bond_param = arr(ff.bond_atoms, np.int32), arr(ff.bond_param, np.float64)
+        f.n_angles = len(ff.angle_atoms)
+        f.angle_atoms, f.angle_param = arr(ff.angle_atoms, np.int32), arr(ff.angle_param, np.float64)
+        f.n_fene = len(ff.fene_atoms)
+        f.fene_atoms, f.fene_param = arr(ff.fene_atoms, np.int32), arr(ff.fene_param, np.float64)
+        e = EOpts()
+        e.cutoff = ff.cutoff if ff.cutoff > 0 else o.cutoff
+        e.skin, e.coulomb = o.skin, int(o.coulomb)
+        e.electrostatics = 1 if o.electrostatics.lower() == "pme" else 0
+        e.dsf_alpha, e.ewald_rtol, e.pme_spacing, e.pme_order = o.dsf_alpha, o.ewald_rtol, o.pme_spacing, o.pme_order
+        e.tail = int(o.tail and not ff.lj_shift)   # field.cpp:442: lj_shift turns the tail correction off
+        e.parts = o.parts
+        e.no_reciprocal = int(o.no_reciprocal)
+        self.cutoff = e.cutoff
+        self._h = C.c_void_p()
+        _check(L.capsgpu_create(C.byref(f), C.byref(e), C.byref(self._h)))
+        self._energies = True
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().capsgpu_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def energies(self, on):
+        """Evaluator::energies(bool): skip pair energies and the virial on steps that report none"""
+        self._energies = bool(on)
+
+    def compute(self, x, cell):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        cell = np.ascontiguousarray(cell, np.float64)
+        f = np.zeros_like(x)
+        t = np.zeros(NTERMS)
+        _check(lib().capsgpu_compute(self._h, _p(x), _p(cell), _p(f), _p(t), int(self._energies)))
+        return EnergyTerms.from_array(t), f.reshape(-1, 3)
+
+    def list_builds(self):
+        b = C.c_int32()
+        _check(lib().capsgpu_list_builds(self._h, C.byref(b)))
+        return b.value
+
+    def stage_times(self):
+        t = np.zeros(6)
+        _check(lib().capsgpu_stage_times(self._h, _p(t)))
+        return dict(zip(("list_build", "prune", "pairs", "pme", "bonded", "rest"), t))
+
+    # inspection of the cluster list (parity checks)
+    def list_arrays(self):
+        L = lib()
+        s = np.zeros(8, np.int64)
+        _check(L.capsgpu_list_sizes(self._h, _p(s)))
+        n, ncl, ns, ne, nc = (int(v) for v in s[:5])
+        out = dict(perm=np.zeros(ns, np.uint32), iperm=np.zeros(n, np.uint32), first=np.zeros(ncl + 1, np.int64),
+                   cj=np.zeros(ne, np.uint32), code=np.zeros(ne, np.uint16), mask=np.zeros(ne, np.uint16), ex=np.zeros(ne, np.uint16),
+                   ex14=np.zeros(ne, np.uint16), org=np.zeros(3 * ncl), off=np.zeros(3 * ns), ioff=np.zeros(3 * ns, np.int32),
+                   qint=np.zeros(3 * nc, np.int32), qvec=np.zeros(3 * nc))
+        keys = ("perm", "iperm", "first", "cj", "code", "mask", "ex", "ex14", "org", "off", "ioff", "qint", "qvec")
+        _check(L.capsgpu_list_get(self._h, *[_p(out[k]) for k in keys]))
+        out["nu"], out["nv"], out["npruned"] = int(s[5]), int(s[6]), int(s[7])
+        return out
+
+    def list_pruned(self):
+        L = lib()
+        s = np.zeros(8, np.int64)
+        _check(L.capsgpu_list_sizes(self._h, _p(s)))
+        ns, ne = int(s[2]), int(s[3])
+        X, Y, Z = (np.zeros(ns, np.float32) for _ in range(3))
+        pm, pe, p14 = (np.zeros(ne, np.uint16) for _ in range(3))
+        _check(L.capsgpu_list_pruned(self._h, _p(X), _p(Y), _p(Z), _p(pm), _p(pe), _p(p14)))
+        return X, Y, Z, pm, pe, p14
+
+    # dynamics state on the device
+    def set_state(self, x, v=None, cell=None):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        v = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+        cell = None if cell is None else np.ascontiguousarray(cell, np.float64)
+        _check(lib().capsgpu_set_state(self._h, _p(x), _p(v), _p(cell)))
+
+    def get_state(self):
+        n = self.ff.n
+        x, v, cell = np.zeros(3 * n), np.zeros(3 * n), np.zeros(12)
+        _check(lib().capsgpu_get_state(self._h, _p(x), _p(v), _p(cell)))
+        return x.reshape(-1, 3), v.reshape(-1, 3), cell
+
+
+@dataclass
+class DynamicsOptions:
+    """caps::DynamicsOptions (core/include/caps/dynamics.hpp:60-154), the fields this path reads"""
+    dt: float = 1.0
+    steps: int = 10000
+    temperature: float = 300.0
+    thermostat: str = "bussi"        # none, bussi, langevin
+    tau_t: float = 100.0
+    seed: int = 1
+    thermo_every: int = 100
+    new_velocities: bool = False
+    respa: int = 1
+    step_offset: int = 0
+    energy: EnergyOptions = field(default_factory=EnergyOptions)
+    progress: object = None          # callable(row) -> bool
+    learned_k: int = 0
+    learned_tol: float = 1e-3
+
+
+@dataclass
+class DynamicsReport:
+    thermo: list
+    steps: int
+    seconds: float
+    ns_per_day: float
+    list_builds: int
+    prunes: int
+    learned_steps: int = 0
+    full_steps: int = 0
+    fallbacks: int = 0
+
+
+_THERMOSTATS = {"none": 0, "nve": 0, "bussi": 1, "csvr": 1, "v-rescale": 1, "langevin": 2, "baoab": 2}
+
+
+def run_dynamics(system, o, evaluator=None):
+    """caps::run_dynamics (core/src/dynamics.cpp:71-815) on the GPU: updates system.x, system.v in place and returns a
+    DynamicsReport with the thermo rows (dicts)."""
+    if o.thermostat.lower() not in _THERMOSTATS:
+        raise ValueError("unknown thermostat '%s' (none, bussi, langevin)" % o.thermostat)
+    ev = evaluator or Evaluator(system.ff, o.energy)
+    has_v = system.v is not None and not o.new_velocities
+    ev.set_state(system.x, system.v if has_v else None, system.cell)
+    d = DOpts()
+    d.dt, d.steps, d.temperature = o.dt, o.steps, o.temperature
+    d.thermostat, d.tau_t, d.seed = _THERMOSTATS[o.thermostat.lower()], o.tau_t, o.seed
+    d.thermo_every, d.new_velocities, d.respa, d.step_offset = o.thermo_every, int(not has_v), o.respa, o.step_offset
+    d.learned_k, d.learned_tol = o.learned_k, o.learned_tol
+    cap = int(o.steps // max(1, o.thermo_every) + 2)
+    rows = (Thermo * cap)()
+    nr = C.c_int32()
+    rep = DReport()
+    cb = None
+    if o.progress is not None:
+        def _cb(row, user):
+            return 1 if o.progress(_row(row.contents)) else 0
+        cb = PROGRESS_FN(_cb)
+    _check(lib().capsgpu_run_dynamics(ev._h, C.byref(d), C.cast(cb, C.c_void_p) if cb else None, None, C.cast(rows, C.c_void_p), cap,
+                                      C.byref(nr), C.byref(rep)))
+    x, v, _ = ev.get_state()
+    system.x, system.v = x, v
+    return DynamicsReport([_row(rows[k]) for k in range(min(nr.value, cap))], rep.steps, rep.seconds, rep.ns_per_day,
+                          rep.list_builds, rep.prunes, rep.learned_steps, rep.full_steps, rep.fallbacks)
+
+
+def _row(r):
+    return dict(step=r.step, time_ps=r.time_ps, temperature=r.temperature, kinetic=r.kinetic, potential=r.potential, total=r.total,
+                pressure=r.pressure, conserved=r.conserved, terms=EnergyTerms.from_array(r.terms), learned=r.learned,
+                learned_rmse=r.learned_rmse)
+
+
+# ---- synthetic systems ------------------------------------------------------------------------------------------------
+def _system_from(ptr):
+    s = ptr.contents
+    n, nt = s.n_atoms, s.n_types
+
+    def a(p, m, dt):
+        return np.ctypeslib.as_array(p, shape=(m,)).astype(dt).copy() if m else np.zeros(0, dt)
+
+    x = a(s.x, 3 * n, np.float64).reshape(-1, 3)
+    off = a(s.excl_offset, n + 1, np.int32)
+    ff = ForceField(type=a(s.type, n, np.int32), charge=a(s.charge, n, np.float64), mass=a(s.mass, n, np.float64),
+                    eps=a(s.eps, nt * nt, np.float64).reshape(nt, nt), sigma=a(s.sigma, nt * nt, np.float64).reshape(nt, nt),
+                    excl_offset=off, excl_index=a(s.excl_index, int(off[-1]), np.int32),
+                    bond_atoms=a(s.bond_atoms, 2 * s.n_bonds, np.int32).reshape(-1, 2),
+                    bond_param=a(s.bond_param, 2 * s.n_bonds, np.float64).reshape(-1, 2),
+                    angle_atoms=a(s.angle_atoms, 3 * s.n_angles, np.int32).reshape(-1, 3),
+                    angle_param=a(s.angle_param, 2 * s.n_angles, np.float64).reshape(-1, 2),
+                    fene_atoms=a(s.fene_atoms, 2 * s.n_fene, np.int32).reshape(-1, 2),
+                    fene_param=a(s.fene_param, 4 * s.n_fene, np.float64).reshape(-1, 4))
+    cell = np.array(list(s.cell), np.float64)
+    lib().capsgpu_system_free(ptr)
+    return System(x=x, cell=cell, ff=ff)
+
+
+def build_melt(chains, seed=1, density=1.0):
+    """an all-atom polyethylene-like melt of 128-atom chains (the 32,768- and 262,144-atom cases)"""
+    p = C.POINTER(Sys)()
+    if lib().capsgpu_build_melt(chains, seed, density, C.byref(p)) != 0:
+        raise ValueError(lib().capsgpu_build_error().decode())
+    return _system_from(p)
+
+
+def kremer_grest(chains=200, beads=50, density=0.85, k_theta=0.0, bond=0.97, seed=1):
+    """kremer_grest() + kremer_grest_forcefield() (core/src/kremer_grest.cpp:15-124): FENE + WCA, r_c = 2^(1/6)"""
+    p = C.POINTER(Sys)()
+    if lib().capsgpu_build_kg(chains, beads, density, k_theta, bond, seed, C.byref(p)) != 0:
+        raise ValueError(lib().capsgpu_build_error().decode())
+    s = _system_from(p)
+    s.ff.lj_shift = True
+    s.ff.cutoff = 2.0 ** (1.0 / 6.0)
+    s.ff.coul14 = 1.0
+    return s
+
+
+def minimise(ev, system, steps=200, max_move=0.1, ftol=0.0, verbose=False):
+    """steepest descent with the largest atomic move capped (the push-off a grown melt needs before dynamics);
+    returns the final energy terms"""
+    x = np.array(system.x, np.float64)
+    step = max_move
+    t, f = ev.compute(x, system.cell)
+    e = t.total()
+    for it in range(steps):
+        fmax = float(np.sqrt((f * f).sum(1)).max())
+        if fmax <= ftol:
+            break
+        xn = x + f * (step / fmax)
+        tn, fn = ev.compute(xn, system.cell)
+        if tn.total() < e:
+            x, f, e, t = xn, fn, tn.total(), tn
+            step = min(max_move, step * 1.2)
+        else:
+            step *= 0.5
+        if verbose and it % 20 == 0:
+            print("minimise", it, e, fmax, step, flush=True)
+    system.x = x
+    return t

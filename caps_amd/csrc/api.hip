@@ -1,0 +1,604 @@
+// The C boundary of capsgpu (include/capsgpu.h): exceptions become return codes and a thread-local message, as capi's
+// guard() does (capi/src/caps_c.cpp:353, 373-383). Host orchestration of one force evaluation (the order of
+// core/src/field.cpp:1279-2539's cluster path) and of the dynamics loop (core/src/dynamics.cpp:71-815).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "context.hpp"
+#include "host_rng.hpp"
+#include "mlp.hpp"
+#include "step.hpp"
+
+using namespace capsgpu;
+
+namespace {
+thread_local std::string g_error;
+
+template <class F>
+int32_t guard(F&& f) {
+  try {
+    f();
+    return CAPSGPU_OK;
+  } catch (const Error& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return CAPSGPU_EINVAL;
+  }
+}
+
+double erfc_as(double x) {   // erfc_exp (field.cpp:585-590): A&S 7.1.26
+  const double P = 0.3275911, A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027, A5 = 1.061405429;
+  const double t = 1.0 / (1.0 + P * x);
+  return t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * std::exp(-x * x);
+}
+
+// Ewald β with erfc(β r_c) = rtol by bisection (core/src/kspace.cpp:35-43)
+double ewald_beta(double cutoff, double rtol) {
+  double lo = 0, hi = 20.0 / cutoff;
+  for (int it = 0; it < 100; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    (std::erfc(mid * cutoff) > rtol ? lo : hi) = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+double cell_volume(const double* c) {
+  const double *a = c + 3, *b = c + 6, *cc = c + 9;
+  return std::fabs(a[0] * (b[1] * cc[2] - b[2] * cc[1]) + a[1] * (b[2] * cc[0] - b[0] * cc[2]) + a[2] * (b[0] * cc[1] - b[1] * cc[0]));
+}
+
+bool pme_on(const capsgpu_ctx& c) { return c.opt.coulomb && c.opt.electrostatics == 1 && !c.opt.no_reciprocal; }
+bool ewald_on(const capsgpu_ctx& c) { return c.opt.coulomb && c.opt.electrostatics == 1; }
+
+// the energy terms from the reduced partial sums of one evaluation (field.cpp:2376-2537)
+struct Sums {
+  double pair[11], slot[9], rec[7], bond[9];
+};
+
+void assemble(const capsgpu_ctx& c, const Sums& S, bool nonb, bool bonded, double* t) {
+  for (int k = 0; k < CAPSGPU_NTERMS; ++k) t[k] = 0;
+  double w[6] = {0, 0, 0, 0, 0, 0}, vir = 0;
+  if (bonded) {
+    t[CAPSGPU_T_BOND] = S.bond[0];
+    t[CAPSGPU_T_ANGLE] = S.bond[1];
+    vir += S.bond[2];
+    for (int q = 0; q < 6; ++q) w[q] += S.bond[3 + q];
+  }
+  if (nonb) {
+    // the full list counts every pair twice
+    t[CAPSGPU_T_VDW] = 0.5 * S.pair[0];
+    t[CAPSGPU_T_COULOMB] = 0.5 * S.pair[1];
+    double W[9];
+    for (int q = 0; q < 9; ++q) W[q] = S.slot[q];
+    const double* A[3] = {c.cell + 3, c.cell + 6, c.cell + 9};
+    for (int L = 0; L < 3; ++L)
+      for (int u = 0; u < 3; ++u)
+        for (int v = 0; v < 3; ++v) W[3 * u + v] += 0.5 * A[L][u] * S.pair[2 + 3 * L + v];
+    vir += W[0] + W[4] + W[8];
+    w[0] += W[0], w[1] += W[4], w[2] += W[8];
+    w[3] += 0.5 * (W[1] + W[3]), w[4] += 0.5 * (W[2] + W[6]), w[5] += 0.5 * (W[5] + W[7]);
+    double q2 = 0, qt = 0;
+    for (double q : c.charge) q2 += q * q, qt += q;
+    const double rc = c.opt.cutoff, V = cell_volume(c.cell);
+    if (c.opt.coulomb && ewald_on(c)) {
+      if (pme_on(c)) {
+        t[CAPSGPU_T_COULOMB] += S.rec[0];
+        vir += S.rec[1] + S.rec[2] + S.rec[3];
+        for (int q = 0; q < 6; ++q) w[q] += S.rec[1 + q];
+        const double beta = c.pme.beta;
+        t[CAPSGPU_T_COULOMB] -= kCoulomb * beta / std::sqrt(kPi) * q2;
+        if (std::fabs(qt) > 1e-10) {
+          const double eb = -kCoulomb * kPi * qt * qt / (2 * V * beta * beta);
+          t[CAPSGPU_T_COULOMB] += eb;
+          vir += 3 * eb;
+          for (int q = 0; q < 3; ++q) w[q] += eb;
+        }
+      }
+    } else if (c.opt.coulomb) {
+      {
+        const double a = c.opt.dsf_alpha, erfc_rc = erfc_as(a * rc);
+        const double f0 = erfc_rc / (rc * rc) + 2 * a / std::sqrt(kPi) * std::exp(-a * a * rc * rc) / rc;
+        t[CAPSGPU_T_COULOMB] -= (erfc_rc / (2 * rc) + f0 * rc / 2 + a / std::sqrt(kPi)) * q2 * kCoulomb;
+      }
+    }
+    if (c.opt.tail) {
+      const double rc2 = rc * rc, rc3 = rc2 * rc, rc9 = rc3 * rc3 * rc3;
+      double et = 0, wt = 0;
+      for (int a = 0; a < c.nt; ++a)
+        for (int b = 0; b < c.nt; ++b) {
+          const size_t tp = size_t(a) * c.nt + b;
+          const double eps = c.eps[tp], s6 = c.sig6[tp], s12 = s6 * s6, nn = c.type_count[size_t(a)] * c.type_count[size_t(b)];
+          if (c.lj96) {
+            const double s3 = std::sqrt(s6), rc6 = rc3 * rc3;
+            et += nn * 2 * kPi * eps * s6 * (s3 - 3 * rc3) / (3 * rc6);
+            wt += nn * 3 * 2 * kPi * eps * s6 * (s3 - 2 * rc3) / rc6;
+          } else {
+            et += nn * 2 * kPi * 4 * eps * (s12 / (9 * rc9) - s6 / (3 * rc3));
+            wt += nn * 2 * kPi * 4 * eps * (4 * s12 / (3 * rc9) - 2 * s6 / rc3);
+          }
+        }
+      t[CAPSGPU_T_VDW] += et / V;
+      vir += wt / V;
+      t[CAPSGPU_T_TAIL_ENERGY] = et / V, t[CAPSGPU_T_TAIL_VIRIAL] = wt / V;
+      for (int q = 0; q < 3; ++q) w[q] += wt / V / 3;
+    }
+  }
+  t[CAPSGPU_T_VIRIAL] = vir;
+  for (int q = 0; q < 6; ++q) t[CAPSGPU_T_WXX + q] = w[q];
+}
+
+void read_flags(capsgpu_ctx& c) {
+  HIPCHECK(hipMemcpyAsync(c.h_flags, c.flags.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
+}
+
+void ev_record(capsgpu_ctx& c, int k) {
+  if (c.timing) HIPCHECK(hipEventRecord(c.ev[k], c.stream));
+}
+
+// forces at the device positions c.x into c.f; the energy terms into t when want. The list follows the reference's
+// rules (field.cpp:1286-1312): rebuilt when an atom moved more than skin/2 since the build (flag 0), pruned at
+// r_c + b (b = min(0.8, skin)) when one moved more than b/2 since the prune (flag 1) and after every build.
+void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known) {
+  const bool nonb = c.opt.parts & 2, bonded = c.opt.parts & 1;
+  const double b = std::min(0.8, c.opt.skin);
+  ev_record(c, 0);
+  if (nonb) {
+    if (!flags_known && c.L.built) {
+      c.flags.zero(c.stream);
+      check_moves(c, 0.25 * c.opt.skin * c.opt.skin, 0.25 * b * b);
+      read_flags(c);
+    }
+    if (c.h_flags[3]) throw Error(CAPSGPU_EFIELD, "positions are not finite numbers");
+    const bool rebuild = !c.L.built || c.h_flags[0];
+    if (rebuild) {
+      list_build(c);
+      read_flags(c);
+      if (c.h_flags[3]) throw Error(CAPSGPU_EFIELD, "positions are not finite numbers");
+      HIPCHECK(hipMemcpyAsync(c.x0.p, c.x.p, 3 * size_t(c.n) * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      ++c.builds;
+      c.pruned_ok = false;
+    }
+    ev_record(c, 1);
+    list_positions(c);
+    if (!c.pruned_ok || c.h_flags[1]) {
+      list_prune(c, c.opt.cutoff + b);
+      HIPCHECK(hipMemcpyAsync(c.x0p.p, c.x.p, 3 * size_t(c.n) * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      ++c.prunes;
+      c.pruned_ok = true;
+    }
+    ev_record(c, 2);
+    pair_forces(c, want);
+    ev_record(c, 3);
+    if (pme_on(c)) pme_forces(c, want);
+    ev_record(c, 4);
+  } else {
+    for (int k = 1; k <= 4; ++k) ev_record(c, k);
+  }
+  if (bonded) bonded_forces(c, want);
+  ev_record(c, 5);
+  sum_forces(c, nonb, nonb && pme_on(c), bonded, c.f.p);
+  if (want) {
+    Sums S{};
+    double* out = c.terms.p;
+    if (nonb) {
+      colsum(c, c.L.epart.p, c.L.g.nclusters, 11, out);
+      c.partials.resize(9 * size_t(blocks_for(c.L.g.ns, 256)));
+      slot_virial(c, c.partials.p);
+      colsum(c, c.partials.p, blocks_for(c.L.g.ns, 256), 9, out + 11);
+      if (pme_on(c)) {
+        const int mblocks = (c.pme.h1 + 15) / 16;
+        colsum(c, c.pme.epart.p, int64_t(mblocks) * c.pme.k[1], 7, out + 20);
+      }
+    }
+    if (bonded) colsum(c, c.bonded.epart.p, blocks_for(c.n, 256), 9, out + 27);
+    HIPCHECK(hipMemcpyAsync(c.h_terms, out, 36 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    std::memcpy(S.pair, c.h_terms, sizeof S.pair);
+    std::memcpy(S.slot, c.h_terms + 11, sizeof S.slot);
+    if (nonb && pme_on(c)) std::memcpy(S.rec, c.h_terms + 20, sizeof S.rec);
+    if (bonded) std::memcpy(S.bond, c.h_terms + 27, sizeof S.bond);
+    assemble(c, S, nonb, bonded, t);
+  }
+  ev_record(c, 6);
+}
+
+void upload_cell(capsgpu_ctx& c, const double* cell) {
+  bool same = true;
+  for (int k = 0; k < 12; ++k) same = same && c.cell[k] == cell[k];
+  if (!same) {
+    for (int k = 0; k < 12; ++k) c.cell[k] = cell[k];
+    c.L.built = false;   // a new cell: the list is built again (the reference may rescale instead, field.cpp:1298-1302)
+  }
+  if (!(cell_volume(cell) > 1e-9)) throw Error(CAPSGPU_EINVAL, "the cell must have a volume (periodic along a, b and c)");
+}
+
+}  // namespace
+
+capsgpu_ctx::~capsgpu_ctx() {
+  if (h_flags) (void)hipHostFree(h_flags);
+  if (h_terms) (void)hipHostFree(h_terms);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+const char* capsgpu_last_error(void) { return g_error.c_str(); }
+
+int32_t capsgpu_create(const capsgpu_ff* ff, const capsgpu_energy_opts* o, capsgpu_ctx** out) {
+  return guard([&] {
+    if (!ff || !o || !out) throw Error(CAPSGPU_EINVAL, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(CAPSGPU_EDEVICE, "no GPU visible to HIP");
+    const int n = ff->n_atoms, nt = ff->n_types;
+    if (n < 1 || nt < 1) throw Error(CAPSGPU_EINVAL, "the force field needs atoms and types");
+    if (!ff->type || !ff->charge || !ff->mass || !ff->eps || !ff->sigma || !ff->excl_offset)
+      throw Error(CAPSGPU_EINVAL, "the force field lacks types, charges, masses, LJ tables or exclusions");
+    if (o->cutoff <= 0 || o->skin < 0) throw Error(CAPSGPU_EINVAL, "the cut-off must be positive and the skin not negative");
+    if (o->parts < 1 || o->parts > 3) throw Error(CAPSGPU_EINVAL, "parts must be 1, 2 or 3");
+    auto c = std::make_unique<capsgpu_ctx>();
+    c->n = n, c->nt = nt, c->opt = *o;
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flags), 4 * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_terms), 64 * sizeof(double), hipHostMallocDefault));
+    std::memset(c->h_flags, 0, 4 * sizeof(uint32_t));
+    c->type.assign(ff->type, ff->type + n);
+    for (int32_t t : c->type)
+      if (t < 0 || t >= nt) throw Error(CAPSGPU_EINVAL, "an atom type is out of range");
+    c->charge.assign(ff->charge, ff->charge + n);
+    c->mass.assign(ff->mass, ff->mass + n);
+    for (double m : c->mass)
+      if (!(m > 0)) throw Error(CAPSGPU_EINVAL, "every atom needs a positive mass");
+    c->lj96 = ff->lj96 != 0;
+    c->eps.resize(size_t(nt) * nt);
+    c->sig6.resize(size_t(nt) * nt);
+    std::vector<double> c12(size_t(nt) * nt), c6(size_t(nt) * nt);
+    for (size_t tp = 0; tp < c12.size(); ++tp) {
+      c->eps[tp] = ff->eps[tp];
+      c->sig6[tp] = std::pow(ff->sigma[tp], 6);
+      // Evaluator::cluster_coefficients (field.cpp:853-890): 12-6 4εσ¹², 4εσ⁶; 9-6 2εσ⁹, 3εσ⁶; factors when products
+      const double eps = c->eps[tp] > 0 ? c->eps[tp] : 0.0, s6 = c->sig6[tp];
+      c12[tp] = c->lj96 ? 2 * eps * s6 * std::sqrt(s6) : 4 * eps * s6 * s6;
+      c6[tp] = c->lj96 ? 3 * eps * s6 : 4 * eps * s6;
+    }
+    c->f12.assign(size_t(nt), 0.0);
+    c->f6.assign(size_t(nt), 0.0);
+    bool comb = true;
+    for (int t = 0; t < nt; ++t) c->f12[size_t(t)] = std::sqrt(c12[size_t(t) * nt + t]), c->f6[size_t(t)] = std::sqrt(c6[size_t(t) * nt + t]);
+    for (int a = 0; a < nt && comb; ++a)
+      for (int b = 0; b < nt && comb; ++b) {
+        const size_t tp = size_t(a) * nt + b;
+        const double p12 = c->f12[size_t(a)] * c->f12[size_t(b)], p6 = c->f6[size_t(a)] * c->f6[size_t(b)];
+        comb = std::fabs(p12 - c12[tp]) <= 1e-7 * std::max(std::fabs(c12[tp]), 1e-300) &&
+               std::fabs(p6 - c6[tp]) <= 1e-7 * std::max(std::fabs(c6[tp]), 1e-300);
+      }
+    c->comb = comb;
+    c->excl_off.assign(ff->excl_offset, ff->excl_offset + n + 1);
+    c->excl_idx.assign(ff->excl_index, ff->excl_index + c->excl_off[size_t(n)]);
+    for (int i = 0; i < n; ++i)
+      for (int32_t e = c->excl_off[size_t(i)]; e < c->excl_off[size_t(i) + 1]; ++e) {
+        if (c->excl_idx[size_t(e)] < 0 || c->excl_idx[size_t(e)] >= n) throw Error(CAPSGPU_EINVAL, "an excluded partner is out of range");
+        if (e > c->excl_off[size_t(i)] && c->excl_idx[size_t(e)] <= c->excl_idx[size_t(e) - 1])
+          throw Error(CAPSGPU_EINVAL, "the excluded partners of each atom must be sorted and distinct");
+      }
+    if (ff->p14_offset) {
+      c->p14_off.assign(ff->p14_offset, ff->p14_offset + n + 1);
+      c->p14_idx.assign(ff->p14_index, ff->p14_index + c->p14_off[size_t(n)]);
+      c->f14 = ff->coul14;
+    }
+    c->type_count.assign(size_t(nt), 0.0);
+    for (int32_t t : c->type) c->type_count[size_t(t)] += 1;
+    hipStream_t s = c->stream;
+    c->d_type.upload(c->type, s);
+    c->d_charge.upload(c->charge, s);
+    std::vector<double> minv(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) minv[size_t(i)] = 1.0 / c->mass[size_t(i)];
+    c->d_minv.upload(minv, s);
+    std::vector<float> f12f(c->f12.begin(), c->f12.end()), f6f(c->f6.begin(), c->f6.end());
+    c->d_f12.upload(f12f, s);
+    c->d_f6.upload(f6f, s);
+    c->d_excl_off.upload(c->excl_off, s);
+    if (c->excl_idx.empty()) c->d_excl_idx.resize(1);
+    else c->d_excl_idx.upload(c->excl_idx, s);
+    if (!c->p14_off.empty()) {
+      c->d_p14_off.upload(c->p14_off, s);
+      if (c->p14_idx.empty()) c->d_p14_idx.resize(1);
+      else c->d_p14_idx.upload(c->p14_idx, s);
+    }
+    bonded_setup(*c, *ff);
+    c->x.resize(3 * size_t(n));
+    c->v.resize(3 * size_t(n));
+    c->f.resize(3 * size_t(n));
+    c->x0.resize(3 * size_t(n));
+    c->x0p.resize(3 * size_t(n));
+    c->v.zero(s);
+    c->flags.resize(4);
+    c->flags.zero(s);
+    c->terms.resize(64);
+    if (ewald_on(*c)) {
+      if (o->pme_order < 3 || o->pme_order > 8) throw Error(CAPSGPU_EINVAL, "PME order must be 3 … 8");
+      c->pme.beta = ewald_beta(o->cutoff, o->ewald_rtol);
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    *out = c.release();
+  });
+}
+
+int32_t capsgpu_destroy(capsgpu_ctx* ctx) {
+  return guard([&] { delete ctx; });
+}
+
+int32_t capsgpu_set_state(capsgpu_ctx* c, const double* x, const double* v, const double cell[12]) {
+  return guard([&] {
+    if (!c) throw Error(CAPSGPU_EINVAL, "null context");
+    if (cell) upload_cell(*c, cell);
+    if (x) {
+      c->x.upload(x, 3 * size_t(c->n), c->stream);
+      c->state_set = true;
+    }
+    if (v) c->v.upload(v, 3 * size_t(c->n), c->stream);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int32_t capsgpu_get_state(capsgpu_ctx* c, double* x, double* v, double* cell) {
+  return guard([&] {
+    if (!c) throw Error(CAPSGPU_EINVAL, "null context");
+    if (x) c->x.download(x, 3 * size_t(c->n), c->stream);
+    if (v) c->v.download(v, 3 * size_t(c->n), c->stream);
+    if (cell) std::memcpy(cell, c->cell, 12 * sizeof(double));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int32_t capsgpu_get_forces(capsgpu_ctx* c, double* f) {
+  return guard([&] {
+    if (!c || !f) throw Error(CAPSGPU_EINVAL, "null argument");
+    c->f.download(f, 3 * size_t(c->n), c->stream);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int32_t capsgpu_compute(capsgpu_ctx* c, const double* x, const double cell[12], double* f, double* terms, int32_t want) {
+  return guard([&] {
+    if (!c || !x || !cell) throw Error(CAPSGPU_EINVAL, "null argument");
+    upload_cell(*c, cell);
+    for (size_t k = 0; k < 3 * size_t(c->n); ++k)
+      if (!std::isfinite(x[k])) throw Error(CAPSGPU_EFIELD, "positions are not finite numbers");
+    c->x.upload(x, 3 * size_t(c->n), c->stream);
+    c->state_set = true;
+    c->timing = true;
+    c->flags.zero(c->stream);
+    std::memset(c->h_flags, 0, 4 * sizeof(uint32_t));
+    double t[CAPSGPU_NTERMS];
+    evaluate(*c, want != 0 || terms != nullptr, t, false);
+    read_flags(*c);
+    if (c->h_flags[2]) {
+      c->flags.zero(c->stream);
+      throw Error(CAPSGPU_EFIELD, "a FENE bond is stretched to R0 or past; lower the time step or relax the structure first");
+    }
+    if (f) c->f.download(f, 3 * size_t(c->n), c->stream);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float ms[6];
+    for (int k = 0; k < 6; ++k) HIPCHECK(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
+    for (int k = 0; k < 6; ++k) c->stage_ms[k] = ms[k];
+    c->timing = false;
+    if (terms) std::memcpy(terms, t, sizeof t);
+  });
+}
+
+int32_t capsgpu_list_builds(capsgpu_ctx* c, int32_t* builds) {
+  return guard([&] {
+    if (!c || !builds) throw Error(CAPSGPU_EINVAL, "null argument");
+    *builds = c->builds;
+  });
+}
+
+int32_t capsgpu_stage_times(capsgpu_ctx* c, double out[6]) {
+  return guard([&] {
+    if (!c || !out) throw Error(CAPSGPU_EINVAL, "null argument");
+    for (int k = 0; k < 6; ++k) out[k] = c->stage_ms[k];
+  });
+}
+
+int32_t capsgpu_list_sizes(capsgpu_ctx* c, int64_t out[8]) {
+  return guard([&] {
+    if (!c || !out) throw Error(CAPSGPU_EINVAL, "null argument");
+    const ListGeom& g = c->L.g;
+    out[0] = c->n, out[1] = g.nclusters, out[2] = g.ns, out[3] = g.nentries, out[4] = g.ncodes, out[5] = g.nu, out[6] = g.nv;
+    out[7] = c->L.npruned;
+  });
+}
+
+int32_t capsgpu_list_get(capsgpu_ctx* c, uint32_t* perm, uint32_t* iperm, int64_t* first, uint32_t* cj, uint16_t* code, uint16_t* mask,
+                         uint16_t* ex, uint16_t* ex14, double* org, double* off, int32_t* ioff, int32_t* qint, double* qvec) {
+  return guard([&] {
+    if (!c || !c->L.built) throw Error(CAPSGPU_EINVAL, "no list has been built");
+    DevList& L = c->L;
+    hipStream_t s = c->stream;
+    const size_t ns = size_t(L.g.ns), ne = size_t(L.g.nentries), ncl = size_t(L.g.nclusters), nc = size_t(L.g.ncodes);
+    if (perm) L.perm.download(perm, ns, s);
+    if (iperm) L.iperm.download(iperm, size_t(c->n), s);
+    if (first) L.first.download(first, ncl + 1, s);
+    if (cj) L.e_cj.download(cj, ne, s);
+    if (code) L.e_code.download(code, ne, s);
+    if (mask) L.e_mask.download(mask, ne, s);
+    if (ex) L.e_ex.download(ex, ne, s);
+    if (ex14) L.e_ex14.download(ex14, ne, s);
+    if (org) L.org.download(org, 3 * ncl, s);
+    if (off) L.off.download(off, 3 * ns, s);
+    if (ioff) L.ioff.download(ioff, 3 * ns, s);
+    if (qint) L.qint.download(qint, 3 * nc, s);
+    if (qvec) L.qvec.download(qvec, 3 * nc, s);
+    HIPCHECK(hipStreamSynchronize(s));
+  });
+}
+
+int32_t capsgpu_list_pruned(capsgpu_ctx* c, float* X, float* Y, float* Z, uint16_t* pmask, uint16_t* pex, uint16_t* pex14) {
+  return guard([&] {
+    if (!c || !c->L.built) throw Error(CAPSGPU_EINVAL, "no list has been built");
+    DevList& L = c->L;
+    hipStream_t s = c->stream;
+    const size_t ns = size_t(L.g.ns), ne = size_t(L.g.nentries);
+    if (X || Y || Z) {
+      std::vector<float4> xq = L.xq.to_host(s);
+      for (size_t k = 0; k < ns; ++k) {
+        if (X) X[k] = xq[k].x;
+        if (Y) Y[k] = xq[k].y;
+        if (Z) Z[k] = xq[k].z;
+      }
+    }
+    if (pmask) L.pmask.download(pmask, ne, s);
+    if (pex) L.pex.download(pex, ne, s);
+    if (pex14) L.pex14.download(pex14, ne, s);
+    HIPCHECK(hipStreamSynchronize(s));
+  });
+}
+
+int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_progress_fn progress, void* user, capsgpu_thermo* rows,
+                             int32_t cap, int32_t* nrows, capsgpu_dyn_report* rep) {
+  return guard([&] {
+    if (!c || !o) throw Error(CAPSGPU_EINVAL, "null argument");
+    if (!c->state_set) throw Error(CAPSGPU_EINVAL, "set the positions first (capsgpu_set_state)");
+    if (o->dt <= 0 || o->dt > 5) throw Error(CAPSGPU_EINVAL, "time step must be in (0, 5] fs");
+    if (o->steps < 0) throw Error(CAPSGPU_EINVAL, "the number of steps must be 0 or more");
+    if (o->respa < 1 || o->respa > 16) throw Error(CAPSGPU_EINVAL, "r-RESPA inner steps must be 1 … 16");
+    if (o->respa > 1 && o->thermostat == 2) throw Error(CAPSGPU_EINVAL, "r-RESPA runs with the Bussi thermostat or none");
+    if (o->thermostat < 0 || o->thermostat > 2) throw Error(CAPSGPU_EINVAL, "thermostat must be 0 (none), 1 (Bussi) or 2 (Langevin)");
+    if (o->learned_k > 1 && (o->respa > 1)) throw Error(CAPSGPU_EINVAL, "learned r-RESPA replaces the bonded/non-bonded split");
+    if (o->learned_k > 1 && !pme_on(*c)) throw Error(CAPSGPU_EINVAL, "learned r-RESPA needs PME");
+    if (nrows) *nrows = 0;
+    const int n = c->n;
+    hipStream_t s = c->stream;
+    const double ndof = 3.0 * n - 3.0;
+    if (ndof < 1) throw Error(CAPSGPU_EINVAL, "dynamics needs at least two atoms");
+    HostRng rng(o->seed);
+    if (o->new_velocities) {
+      // Maxwell–Boltzmann from the run's own mt19937_64 and normal generator (dynamics.cpp:236-256)
+      std::vector<double> v(3 * size_t(n));
+      initial_velocities(rng, c->mass, o->temperature, v);
+      c->v.upload(v, s);
+    }
+    const double dt = o->dt, hk = 0.5 * dt * kAcc, bskin = std::min(0.8, c->opt.skin);
+    const double lim_b = 0.25 * c->opt.skin * c->opt.skin, lim_p = 0.25 * bskin * bskin;
+    const double c1 = std::exp(-dt / std::max(o->tau_t, 1e-9)), c2 = std::sqrt(1 - c1 * c1);
+    const double kt_target = 0.5 * ndof * kBoltz * o->temperature;
+    DevBuf<double> sdv, kpart, bpart, scal;   // scal: [0] α, [1] bath
+    if (o->thermostat == 2) {
+      std::vector<double> sd(static_cast<size_t>(n));
+      for (int i = 0; i < n; ++i) sd[size_t(i)] = std::sqrt(kBoltz * o->temperature * kAcc / c->mass[size_t(i)]);
+      sdv.upload(sd, s);
+    }
+    const unsigned nb = blocks_for(n, 256);
+    kpart.resize(7 * size_t(nb));
+    bpart.resize(size_t(nb));
+    scal.resize(2);
+    scal.zero(s);
+    std::vector<double> one = {1.0, 0.0};
+    scal.upload(one, s);
+    const int every = std::max(1, o->thermo_every);
+    int32_t nr = 0;
+    int64_t learned_steps = 0, full_steps = 0, fallbacks = 0;
+    double last_rmse = 0;
+    bool last_learned = false;
+    Learned* LR = nullptr;
+    if (o->learned_k > 1) LR = learned_begin(*c, *o);
+    auto emit = [&](int64_t step, const double* t) {
+      // the kinetic energy and its tensor after the closing kick (and Bussi's scaling)
+      colsum(*c, kpart.p, nb, 7, c->terms.p + 40);
+      HIPCHECK(hipMemcpyAsync(c->h_terms + 40, c->terms.p + 40, 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+      double hb[2];
+      HIPCHECK(hipMemcpyAsync(hb, scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      capsgpu_thermo r{};
+      r.step = step + o->step_offset;
+      r.time_ps = double(step + o->step_offset) * dt * 1e-3;
+      r.kinetic = c->h_terms[40];
+      r.temperature = 2 * r.kinetic / (ndof * kBoltz);
+      for (int k = 0; k < CAPSGPU_NTERMS; ++k) r.terms[k] = t[k];
+      r.potential = t[CAPSGPU_T_BOND] + t[CAPSGPU_T_ANGLE] + t[CAPSGPU_T_DIHEDRAL] + t[CAPSGPU_T_IMPROPER] + t[CAPSGPU_T_VDW] + t[CAPSGPU_T_COULOMB];
+      r.total = r.potential + r.kinetic;
+      const double V = cell_volume(c->cell);
+      r.pressure = (2 * r.kinetic + t[CAPSGPU_T_VIRIAL]) / (3 * V) * kAtm;
+      r.conserved = r.total + hb[1];
+      r.learned = last_learned ? 1 : 0;
+      r.learned_rmse = last_rmse;
+      if (rows && nr < cap) rows[nr] = r;
+      ++nr;
+      if (progress && !progress(&r, user)) throw Error(CAPSGPU_ECANCEL, "dynamics cancelled");
+    };
+    double t[CAPSGPU_NTERMS];
+    c->h_flags[0] = c->h_flags[1] = 0;
+    evaluate(*c, true, t, false);
+    // step 0: the kinetic energy of the starting velocities
+    kick(*c, 0.0, true, kpart.p);
+    emit(0, t);
+    HIPCHECK(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    bool alpha_pending = false;
+    for (int64_t step = 1; step <= o->steps; ++step) {
+      const bool thermo = step % every == 0 || step == o->steps;
+      c->flags.zero(s);
+      if (o->thermostat == 2) {
+        baoab_open(*c, hk, 0.5 * dt, c1, c2, sdv.p, o->seed, uint64_t(step), lim_b, lim_p, bpart.p);
+        bath_add(*c, bpart.p, scal.p + 1);
+      } else {
+        vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p);
+        alpha_pending = false;
+      }
+      read_flags(*c);
+      if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable at step " + std::to_string(step));
+      if (LR) {
+        const int r = learned_step(*c, *LR, step, thermo, t);
+        last_learned = r == 1;
+        if (r == 1) ++learned_steps;
+        else ++full_steps;
+        if (r == 2) ++fallbacks;
+        last_rmse = learned_rmse(*LR);
+      } else {
+        evaluate(*c, thermo, t, true);
+      }
+      kick(*c, hk, thermo || o->thermostat == 1, kpart.p);
+      if (o->thermostat == 1) {
+        const double r1 = rng.normal();
+        const double sum = ndof > 1 ? rng.gamma((ndof - 1) / 2, 2.0) : 0.0;
+        bussi(*c, kpart.p, kt_target, ndof, std::exp(-dt / o->tau_t), r1, sum, scal.p, scal.p + 1);
+        if (thermo) scale_v(*c, scal.p);
+        else alpha_pending = true;
+      }
+      if (thermo) {
+        if (o->thermostat == 1) kick(*c, 0.0, true, kpart.p);
+        emit(step, t);
+      }
+    }
+    if (alpha_pending) scale_v(*c, scal.p);
+    read_flags(*c);
+    if (c->h_flags[2]) throw Error(CAPSGPU_EFIELD, "a FENE bond is stretched to R0 or past; lower the time step or relax the structure first");
+    HIPCHECK(hipStreamSynchronize(s));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (LR) learned_end(*c, LR);
+    if (nrows) *nrows = nr;
+    if (rep) {
+      rep->steps = o->steps;
+      rep->seconds = secs;
+      rep->ns_per_day = secs > 0 ? double(o->steps) * dt * 1e-6 / secs * 86400 : 0;
+      rep->list_builds = c->builds;
+      rep->prunes = c->prunes;
+      rep->learned_steps = int32_t(learned_steps);
+      rep->full_steps = int32_t(full_steps);
+      rep->fallbacks = int32_t(fallbacks);
+    }
+  });
+}
+
+}  // extern "C"

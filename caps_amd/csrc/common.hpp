@@ -1,0 +1,102 @@
+// Internal: HIP error handling, device buffers and the context of the capsgpu library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace capsgpu {
+
+// errors thrown inside the library and turned into return codes at the C boundary (as capi's guard(),
+// capi/src/caps_c.cpp:373-383)
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess)
+    throw Error(-3, std::string("HIP error in ") + what + " (" + file + ":" + std::to_string(line) + "): " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) ::capsgpu::hip_check((x), #x, __FILE__, __LINE__)
+#ifdef CAPSGPU_DEBUG_GUARD
+// debug builds only: guarded allocations and a check of every guard after each launch
+void* guard_alloc(size_t bytes);
+void guard_free(void* p);
+void guard_check(const char* file, int line);
+#define LAUNCHCHECK() (::capsgpu::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__), ::capsgpu::guard_check(__FILE__, __LINE__))
+#else
+#define LAUNCHCHECK() ::capsgpu::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
+#endif
+
+// a device array that owns its memory and grows (never shrinks) on demand
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0, cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+#ifdef CAPSGPU_DEBUG_GUARD
+    if (p) guard_free(p);
+#else
+    if (p) (void)hipFree(p);
+#endif
+    p = nullptr, n = cap = 0;
+  }
+  // size n; contents are kept only when no reallocation happens
+  void resize(size_t m) {
+    if (m > cap) {
+      release();
+#ifdef CAPSGPU_DEBUG_GUARD
+      const size_t c = m ? m : 1;
+#else
+      const size_t c = m + m / 8 + 64;
+#endif
+#ifdef CAPSGPU_DEBUG_GUARD
+      p = static_cast<T*>(guard_alloc(c * sizeof(T)));
+#else
+      HIPCHECK(hipMalloc(&p, c * sizeof(T)));
+#endif
+      cap = c;
+    }
+    n = m;
+  }
+  // ordered on stream s and complete on return (the host array may be a temporary)
+  void upload(const T* h, size_t m, hipStream_t s) {
+    resize(m);
+    if (m) {
+      HIPCHECK(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipStreamSynchronize(s));
+    }
+  }
+  void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+  void download(T* h, size_t m, hipStream_t s) const {
+    if (m) HIPCHECK(hipMemcpyAsync(h, p, m * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+  std::vector<T> to_host(hipStream_t s) const {
+    std::vector<T> h(n);
+    download(h.data(), n, s);
+    HIPCHECK(hipStreamSynchronize(s));
+    return h;
+  }
+  void zero(hipStream_t s) {
+    if (n) HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+constexpr uint32_t kPad = 0xffffffffu;
+constexpr double kCoulomb = 332.06371;    // kcal·Å/(mol·e²) (core/include/caps/kspace.hpp:12)
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kBoltz = 0.0019872041;   // kcal/(mol·K) (core/src/dynamics.cpp:22)
+constexpr double kAcc = 4.184e-4;         // (kcal/mol/Å)/(g/mol) → Å/fs² (core/src/dynamics.cpp:23)
+constexpr double kAtm = 68568.415;        // kcal/(mol·Å³) → atm (core/src/dynamics.cpp:24)
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
+
+}  // namespace capsgpu

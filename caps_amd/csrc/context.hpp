@@ -1,0 +1,169 @@
+// Internal: the device state of one capsgpu context and the host launchers of each kernel family.
+#pragma once
+
+#include <array>
+#include <memory>
+#include <vector>
+
+#include "../../include/capsgpu.h"
+#include "common.hpp"
+#include "scan.hpp"
+
+namespace capsgpu {
+
+// Geometry of one cluster-list build, fixed on the host the way core/src/nbnxm.cpp:40-47 and 146-164 fix it
+struct ListGeom {
+  int n = 0, nu = 1, nv = 1, ncols = 1;
+  int ru = 0, rv = 0, rq = 0, nwin = 0;
+  int qlo[3] = {0, 0, 0}, span[3] = {1, 1, 1};
+  int ncodes = 0, zero_code = 0;
+  double rl = 0, rl2 = 0, dw = 0;
+  float rl2f = 0;
+  double cell[12] = {};       // origin, a, b, c
+  double bc[3] = {}, ca[3] = {}, ab[3] = {}, vsigned = 0;   // cross products and a·(b×c) for to_fractional
+  int64_t nclusters = 0, ns = 0, nentries = 0;
+};
+
+// The cluster list on the device: the half list exactly as nbnxm::List holds it, plus the derived full list the pair
+// kernel walks (each cluster pair from both sides, so no kernel writes another cluster's forces)
+struct DevList {
+  ListGeom g;
+  bool built = false;
+  // binning
+  DevBuf<double> fr;                    // 3 per atom: wrapped fractional coordinates
+  DevBuf<uint32_t> col, bucket, rank;
+  DevBuf<int32_t> colcnt, bfill;
+  DevBuf<int64_t> cstart, ccl;
+  DevBuf<uint32_t> ccls;                // clusters per column
+  DevBuf<uint32_t> perm, iperm, ccol;
+  DevBuf<double> cw, frs;               // 3 per slot
+  DevBuf<double> blo, bhi, org;         // 3 per cluster
+  DevBuf<double> wlo, whi;
+  DevBuf<float> bx, by, bz;
+  DevBuf<uint8_t> real;
+  // excluded pairs by cluster pair
+  DevBuf<uint32_t> xcount;
+  DevBuf<int64_t> xoff;
+  DevBuf<int32_t> xcnt;
+  DevBuf<uint32_t> xcj;
+  DevBuf<uint16_t> xbits, x14;
+  // the half list
+  DevBuf<uint32_t> wcount;              // entries per (i cluster, column window)
+  DevBuf<int64_t> woff;
+  DevBuf<int64_t> first;
+  DevBuf<uint32_t> e_cj, e_ci;
+  DevBuf<uint16_t> e_code, e_mask, e_ex, e_ex14;
+  DevBuf<double> off;                   // 3 per slot
+  DevBuf<int32_t> ioff;
+  DevBuf<int32_t> qint;
+  DevBuf<double> qvec;
+  DevBuf<int32_t> mirror;               // per image code: the code of the opposite image
+  // the full outer list: per cluster its own entries (half-list order), then the mirrored ones (by source entry)
+  DevBuf<uint32_t> own, mir, mfill;
+  DevBuf<uint32_t> fcount;
+  DevBuf<int64_t> fstart;
+  DevBuf<uint32_t> fsrc;                // source half entry, bit 31 set for a mirrored one
+  // pruning
+  DevBuf<uint16_t> pmask, pex, pex14;   // per half entry
+  DevBuf<uint32_t> pflag;               // per full entry
+  DevBuf<int64_t> ppos;
+  DevBuf<uint2> pent;                   // pruned full entries: {cj | code << 24, mask | ex << 16}
+  DevBuf<uint16_t> pent14;
+  DevBuf<int64_t> pfirst;
+  int64_t npruned = 0;
+  // per slot, every step / every build
+  DevBuf<float4> xq;                    // X, Y, Z relative to the cluster origin, charge
+  DevBuf<float2> pp;                    // LJ factors P12, P6
+  DevBuf<float4> fnb;                   // per slot: pair forces
+  DevBuf<double> epart;                 // per cluster: vdw, coulomb, Σ S⊗f by lattice coefficient (9)
+  DevBuf<int8_t> qint8;                 // 4 per image code: its lattice coefficients
+};
+
+struct Pme {
+  int k[3] = {0, 0, 0}, order = 5, h1 = 0;
+  double beta = 0, cell[12] = {}, hi[3][3] = {}, vol = 0;
+  bool green_ok = false;
+  DevBuf<float> grid;                   // real K1·K2·K3
+  DevBuf<float2> half;                  // (K1/2 + 1)·K2·K3
+  DevBuf<float> gtab;                   // influence function on the half spectrum
+  DevBuf<double> bmod;                  // |b(m)|² per axis, K1 + K2 + K3
+  DevBuf<float> theta;                  // per atom: p weights and p derivatives per axis (6p floats)
+  DevBuf<int32_t> base;                 // per atom: the first grid index per axis (3 ints)
+  DevBuf<uint32_t> tile, tcount, tfill, tatoms;
+  DevBuf<int64_t> tstart;
+  DevBuf<float> fpme;                   // 3 per atom
+  DevBuf<float2> tw;                    // twiddles of the three axes
+  DevBuf<double> epart;                 // energy and virial partials
+};
+
+struct Bonded {
+  int nb = 0, na = 0, nf = 0;
+  DevBuf<int32_t> bij, aijk, fij;
+  DevBuf<double> bkr, akt, fp;
+  DevBuf<int32_t> aoff;                 // per atom: its terms (CSR)
+  DevBuf<int32_t> aterm;                // term id | kind << 28 (0 bond, 1 angle, 2 fene)
+  DevBuf<double> f;                     // 3 per atom
+  DevBuf<double> epart;                 // per block: bond, angle, virial, w[6]
+};
+
+
+}  // namespace capsgpu
+
+struct capsgpu_ctx {
+  int n = 0, nt = 0;
+  capsgpu_energy_opts opt{};
+  hipStream_t stream = nullptr;
+  // force field
+  std::vector<int32_t> type;
+  std::vector<double> charge, mass;
+  std::vector<double> eps, sig6;        // per type pair (mixed)
+  std::vector<double> f12, f6;          // per type, when the pair coefficients factorise (geometric mixing)
+  bool comb = false, lj96 = false;
+  std::vector<int32_t> excl_off, excl_idx, p14_off, p14_idx;
+  double f14 = 0;
+  std::vector<double> type_count;
+  capsgpu::DevBuf<int32_t> d_type, d_excl_off, d_excl_idx, d_p14_off, d_p14_idx;
+  capsgpu::DevBuf<double> d_charge, d_minv;
+  capsgpu::DevBuf<float> d_f12, d_f6, d_c12, d_c6;
+  // state
+  double cell[12] = {};
+  capsgpu::DevBuf<double> x, v, f;      // 3 per atom, atom order
+  capsgpu::DevBuf<double> x0, x0p;      // positions at the last build / prune
+  capsgpu::DevBuf<double> terms;        // 2 × 16 partial sums reduced on the device
+  capsgpu::DevBuf<double> partials;
+  capsgpu::DevBuf<uint32_t> flags;      // [0] rebuild, [1] prune, [2] error (FENE past R0)
+  uint32_t* h_flags = nullptr;          // pinned
+  double* h_terms = nullptr;            // pinned
+  capsgpu::DevList L;
+  capsgpu::Pme pme;
+  capsgpu::Bonded bonded;
+  capsgpu::Scanner scan;
+  int builds = 0, prunes = 0;
+  bool state_set = false, pruned_ok = false;
+  double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  ~capsgpu_ctx();
+};
+
+namespace capsgpu {
+// nbnxm_build.hip
+void list_build(capsgpu_ctx& c);
+// nbnxm_prune.hip
+void list_positions(capsgpu_ctx& c);
+void list_prune(capsgpu_ctx& c, double radius);
+// nbnxm_pairs.hip: pair forces into L.fnb by slot; energies/virial partials into c.partials[0..] when want
+void pair_forces(capsgpu_ctx& c, bool want);
+// pme.hip: reciprocal forces into pme.fpme (atom order); energy and virial partials when want
+void pme_setup(capsgpu_ctx& c);
+void pme_forces(capsgpu_ctx& c, bool want);
+// bonded.hip
+void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff);
+void bonded_forces(capsgpu_ctx& c, bool want);
+// integrate.hip
+void sum_forces(capsgpu_ctx& c, bool nb, bool rec, bool bond, double* fdst);
+void check_moves(capsgpu_ctx& c, double lim_build, double lim_prune);
+// partial sums: slot k of c.partials; reduce_terms folds them (fixed order) into c.terms
+enum Part { P_VDW = 0, P_COUL, P_W0, P_BOND = 8, P_ANGLE, P_BVIR, P_BW0, P_RECE = 16, P_RW0, NPART = 24 };
+void zero_partials(capsgpu_ctx& c);
+}  // namespace capsgpu

@@ -1,0 +1,646 @@
+// Smooth PME reciprocal space on gfx950 in FP32 (the mesh), with FP64 energy and virial sums — the pipeline of
+// core/src/kspace.cpp:80-175 and core/fortran/caps_kspace.f90 (splines :776-796, spread :800-822, r2c/c2r lines
+// :392-461, FFT lines :348-387, influence function :518-564 / :880-928, gather :931-961).
+//
+//  k_splines      per atom: fractional grid coordinate, first grid point and B-spline offset w; its home tile
+//  k_tile_*       atoms binned by the 4x4x4 grid tile of their first point (sorted by atom in each tile)
+//  k_spread       one wavefront per tile, one grid point per lane: each point sums q·θa·θb·θc over the atoms whose
+//                 support covers it (every point written once, no atomics, a fixed order)
+//  k_fft_x_r2c    two real lines per complex transform along a (as the reference), the half spectrum written
+//  k_fft_y        complex lines along b, 16 lines of consecutive a-frequencies per workgroup (coalesced)
+//  k_fft_z_conv   lines along c: forward transform, × influence function (energy and virial on energy steps),
+//                 inverse transform — the convolution without a round trip through HBM
+//  k_fft_x_c2r    back to the real grid, two lines per complex transform
+//  k_gather       per atom: F = −q Σ_d K_d h⁻¹_d ∂_d over its p³ points
+// The FFT is a mixed-radix (2, 3, 4, 5, 7, 8) Stockham transform in LDS, X(m) = Σ x(k) exp(sign 2πi mk/n),
+// forward sign +1 as the reference; twiddles from a table computed in double on the host.
+#include <algorithm>
+#include <cmath>
+
+#include "context.hpp"
+
+namespace capsgpu {
+namespace {
+
+constexpr int kMaxOrder = 8;
+constexpr int kMaxFft = 512;
+
+__device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+// B-spline weights θ[j] = M_p(w + p − 1 − j) … in the reference's order (caps_kspace.f90:568-598), FP32
+__device__ inline void bspline(float w, int p, float* th, float* dth) {
+  for (int j = 0; j < kMaxOrder; ++j) th[j] = 0.0f;
+  th[1] = w;
+  th[0] = 1.0f - w;
+  for (int k = 3; k <= p - 1; ++k) {
+    const float div = 1.0f / float(k - 1);
+    th[k - 1] = div * w * th[k - 2];
+    for (int j = 1; j <= k - 2; ++j) th[k - j - 1] = div * ((w + float(j)) * th[k - j - 2] + (float(k - j) - w) * th[k - j - 1]);
+    th[0] = div * (1.0f - w) * th[0];
+  }
+  dth[0] = -th[0];
+  for (int j = 2; j <= p; ++j) dth[j - 1] = th[j - 2] - th[j - 1];
+  const float div = 1.0f / float(p - 1);
+  th[p - 1] = div * w * th[p - 2];
+  for (int j = 1; j <= p - 2; ++j) th[p - j - 1] = div * ((w + float(j)) * th[p - j - 2] + (float(p - j) - w) * th[p - j - 1]);
+  th[0] = div * (1.0f - w) * th[0];
+}
+
+struct GridArgs {
+  int k[3], nt[3], p, h1;
+  double hi[3][3];
+};
+
+// fractional coordinate s = h⁻¹ x, u = (s − ⌊s⌋) K, the first point ⌊u⌋ − p + 1 (mod K) and w = u − ⌊u⌋
+__global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
+                          int4* __restrict__ base, uint32_t* __restrict__ tcount) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float w[3];
+  int b[3];
+  for (int d = 0; d < 3; ++d) {
+    const double s = G.hi[d][0] * x[3 * i] + G.hi[d][1] * x[3 * i + 1] + G.hi[d][2] * x[3 * i + 2];
+    const double u = (s - floor(s)) * G.k[d];
+    const double fl = floor(u);
+    w[d] = float(u - fl);
+    int t = (int(fl) - G.p + 1) % G.k[d];
+    if (t < 0) t += G.k[d];
+    b[d] = t;
+  }
+  const int tile = ((b[2] >> 2) * G.nt[1] + (b[1] >> 2)) * G.nt[0] + (b[0] >> 2);
+  wq[i] = make_float4(w[0], w[1], w[2], float(q[i]));
+  base[i] = make_int4(b[0], b[1], b[2], tile);
+  atomicAdd(&tcount[tile], 1u);
+}
+
+__global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
+                            uint32_t* __restrict__ tatoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = base[i].w;
+  tatoms[tstart[t] + atomicAdd(&tfill[t], 1u)] = uint32_t(i);
+}
+
+__global__ void k_tile_sort(int ntiles, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tatoms) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int64_t b = tstart[t], e = tstart[t + 1];
+  for (int64_t k = b + 1; k < e; ++k) {
+    const uint32_t v = tatoms[k];
+    int64_t j = k;
+    while (j > b && tatoms[j - 1] > v) tatoms[j] = tatoms[j - 1], --j;
+    tatoms[j] = v;
+  }
+}
+
+__device__ inline float pick(const float* th, int j, int p) {
+  float r = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kMaxOrder; ++k) r = (k == j && k < p) ? th[k] : r;
+  return r;
+}
+
+// distinct home tiles (along one axis) of the first points that reach points [4T, 4T + 3]
+__device__ inline int home_tiles(int T, int K, int p, int* out) {
+  int m = 0;
+  for (int bb = 4 * T - p + 1; bb <= 4 * T + 3; ++bb) {
+    int b = bb % K;
+    if (b < 0) b += K;
+    const int h = b >> 2;
+    bool seen = false;
+    for (int q = 0; q < m; ++q) seen |= out[q] == h;
+    if (!seen && m < 8) out[m++] = h;
+  }
+  return m;
+}
+
+__global__ void __launch_bounds__(256) k_spread(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
+                                                const float4* __restrict__ wq, const int4* __restrict__ base, float* __restrict__ grid) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= ntiles) return;
+  const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
+  const int gx = 4 * tx + (lane & 3), gy = 4 * ty + ((lane >> 2) & 3), gz = 4 * tz + (lane >> 4);
+  int hx[8], hy[8], hz[8];
+  const int mx = home_tiles(tx, G.k[0], G.p, hx), my = home_tiles(ty, G.k[1], G.p, hy), mz = home_tiles(tz, G.k[2], G.p, hz);
+  float acc = 0.0f;
+  for (int iz = 0; iz < mz; ++iz)
+    for (int iy = 0; iy < my; ++iy)
+      for (int ix = 0; ix < mx; ++ix) {
+        const int h = (hz[iz] * G.nt[1] + hy[iy]) * G.nt[0] + hx[ix];
+        for (int64_t k = tstart[h]; k < tstart[h + 1]; ++k) {
+          const uint32_t a = tatoms[k];
+          const float4 w = wq[a];
+          const int4 b = base[a];
+          int dx = gx - b.x, dy = gy - b.y, dz = gz - b.z;
+          dx += dx < 0 ? G.k[0] : 0;
+          dy += dy < 0 ? G.k[1] : 0;
+          dz += dz < 0 ? G.k[2] : 0;
+          if (dx < G.p && dy < G.p && dz < G.p) {
+            float th[kMaxOrder], dth[kMaxOrder];
+            bspline(w.x, G.p, th, dth);
+            const float ta = pick(th, dx, G.p);
+            bspline(w.y, G.p, th, dth);
+            const float tb = pick(th, dy, G.p);
+            bspline(w.z, G.p, th, dth);
+            const float tc = pick(th, dz, G.p);
+            acc += w.w * tb * tc * ta;
+          }
+        }
+      }
+  if (gx < G.k[0] && gy < G.k[1] && gz < G.k[2]) grid[(size_t(gz) * G.k[1] + gy) * G.k[0] + gx] = acc;
+}
+
+// ---- FFT ------------------------------------------------------------------------------------------------------------
+struct Fft1 {
+  int n, nr;
+  int rad[12];
+  int tw;   // offset of this axis's twiddles in the table
+};
+struct FftPlan {
+  Fft1 ax[3];
+};
+
+Fft1 make_fft(int n, int twoff) {
+  Fft1 f{};
+  f.n = n, f.tw = twoff;
+  int m = n;
+  for (const int r : {8, 4, 2, 3, 5, 7}) {
+    while (m % r == 0) {
+      if (r == 8 && m % 8 != 0) break;
+      f.rad[f.nr++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) throw Error(-1, "PME grid sizes must be products of 2, 3, 5 and 7");
+  return f;
+}
+
+// nl lines of length n in A (line-major), transformed by the whole workgroup; returns the buffer with the result
+__device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const float2* __restrict__ tw, int sign) {
+  float2* src = A;
+  float2* dst = B;
+  const int n = F.n;
+  int p = 1;
+  for (int s = 0; s < F.nr; ++s) {
+    const int R = F.rad[s], nb = n / R, step = n / (p * R), rstep = n / R;
+    for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
+      const int line = t / nb, i = t - line * nb;
+      const int k = i % p;
+      float2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (q < R) {
+          float2 x = src[line * n + i + q * nb];
+          const int idx = (q * k * step) % n;
+          float2 w = tw[F.tw + idx];
+          if (sign < 0) w.y = -w.y;
+          v[q] = cmul(x, w);
+        }
+      }
+      float2 o[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (u < R) {
+          float2 acc = v[0];
+#pragma unroll
+          for (int q = 1; q < 8; ++q) {
+            if (q < R) {
+              const int idx = ((q * u) % R) * rstep;
+              float2 w = tw[F.tw + idx];
+              if (sign < 0) w.y = -w.y;
+              const float2 c = cmul(v[q], w);
+              acc.x += c.x, acc.y += c.y;
+            }
+          }
+          o[u] = acc;
+        }
+      }
+      const int j = (i - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < R) dst[line * n + j + q * p] = o[q];
+    }
+    __syncthreads();
+    float2* tmp = src;
+    src = dst, dst = tmp;
+    p *= R;
+  }
+  return src;
+}
+
+// real lines along a (index j + K2·k, pairs 2l, 2l+1) -> the half spectrum, as caps_f_r2c_lines (f90:392-423)
+__global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
+                                                   float2* __restrict__ half) {
+  extern __shared__ float2 lds[];
+  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2;
+  const int LP = 256 / 64;   // line pairs per workgroup
+  float2* A = lds;
+  float2* B = lds + LP * K1;
+  const int pb = blockIdx.x * LP;
+  for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
+    const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (pr < npairs) {
+      const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
+      z.x = grid[size_t(la) * K1 + m];
+      z.y = lb != la ? grid[size_t(lb) * K1 + m] : 0.0f;
+    }
+    A[pl * K1 + m] = z;
+  }
+  __syncthreads();
+  const float2* Z = fft_lines(A, B, LP, FP.ax[0], tw, +1);
+  for (int t = threadIdx.x; t < LP * h1; t += blockDim.x) {
+    const int pl = t / h1, m = t - pl * h1, pr = pb + pl;
+    if (pr >= npairs) continue;
+    const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
+    const float2 zm = Z[pl * K1 + m], zc0 = Z[pl * K1 + (K1 - m) % K1];
+    const float2 zc = make_float2(zc0.x, -zc0.y);
+    half[size_t(la) * h1 + m] = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
+    if (lb != la) {
+      const float2 d = make_float2(zm.x - zc.x, zm.y - zc.y);
+      half[size_t(lb) * h1 + m] = make_float2(0.5f * d.y, -0.5f * d.x);
+    }
+  }
+}
+
+// the inverse: half spectra of two real lines rebuilt whole, Z = A + iB, one inverse transform (f90:427-461)
+__global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
+                                                   float* __restrict__ grid) {
+  extern __shared__ float2 lds[];
+  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2;
+  const int LP = 256 / 64;
+  float2* A = lds;
+  float2* B = lds + LP * K1;
+  const int pb = blockIdx.x * LP;
+  for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
+    const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (pr < npairs) {
+      const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
+      const bool up = m > K1 / 2;
+      const int mh = up ? K1 - m : m;
+      float2 a = half[size_t(la) * h1 + mh];
+      float2 b = lb != la ? half[size_t(lb) * h1 + mh] : make_float2(0.0f, 0.0f);
+      if (up) a.y = -a.y, b.y = -b.y;
+      z = make_float2(a.x - b.y, a.y + b.x);
+    }
+    A[pl * K1 + m] = z;
+  }
+  __syncthreads();
+  const float2* Z = fft_lines(A, B, LP, FP.ax[0], tw, -1);
+  for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
+    const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
+    if (pr >= npairs) continue;
+    const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
+    grid[size_t(la) * K1 + m] = Z[pl * K1 + m].x;
+    if (lb != la) grid[size_t(lb) * K1 + m] = Z[pl * K1 + m].y;
+  }
+}
+
+constexpr int kLinesYZ = 16;   // lines (consecutive a-frequencies) per workgroup along b and c
+
+// lines along b of the half spectrum: element (m, j, k) at m + h1 (j + K2 k)
+__global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half, int sign) {
+  extern __shared__ float2 lds[];
+  const int h1 = G.h1, K2 = G.k[1];
+  const int mblocks = (h1 + kLinesYZ - 1) / kLinesYZ;
+  const int kz = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) * kLinesYZ;
+  const int nl = min(kLinesYZ, h1 - m0);
+  float2* A = lds;
+  float2* B = lds + kLinesYZ * K2;
+  for (int t = threadIdx.x; t < kLinesYZ * K2; t += blockDim.x) {
+    const int j = t / kLinesYZ, li = t % kLinesYZ;
+    if (li < nl) A[li * K2 + j] = half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)];
+  }
+  __syncthreads();
+  const float2* Z = fft_lines(A, B, nl, FP.ax[1], tw, sign);
+  for (int t = threadIdx.x; t < kLinesYZ * K2; t += blockDim.x) {
+    const int j = t / kLinesYZ, li = t % kLinesYZ;
+    if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * K2 + j];
+  }
+}
+
+// the influence function and its virial weights at half-spectrum index (a, b, c), in double (f90:518-564)
+__device__ inline void green_at(const GridArgs& G, int a, int b, int c, double fac, double kcoul, double vol, const double* bm,
+                                double* g, double* w) {
+  const int idx[3] = {a, b, c};
+  int m[3], nyq[3], nn = 0;
+  for (int d = 0; d < 3; ++d) {
+    m[d] = idx[d];
+    if (m[d] >= (G.k[d] + 1) / 2) m[d] -= G.k[d];
+    nyq[d] = (G.k[d] % 2 == 0) && 2 * idx[d] == G.k[d];
+    nn += nyq[d];
+  }
+  const double bmod = bm[a] * bm[G.k[0] + b] * bm[G.k[0] + G.k[1] + c];
+  const int ns = 1 << nn;
+  double gs = 0, ws[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = 0; s < ns; ++s) {
+    int sel = 0;
+    double mv[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+      double sg = 1;
+      if (nyq[d]) {
+        if ((s >> sel) & 1) sg = -1;
+        ++sel;
+      }
+      for (int q = 0; q < 3; ++q) mv[q] += sg * m[d] * G.hi[d][q];
+    }
+    const double m2 = mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2];
+    const double e = kcoul * exp(-fac * m2) / (kPi * vol * m2 * bmod);
+    const double ss = 2 * (1 + fac * m2) / m2;
+    gs += e;
+    ws[0] += e * (1 - ss * mv[0] * mv[0]);
+    ws[1] += e * (1 - ss * mv[1] * mv[1]);
+    ws[2] += e * (1 - ss * mv[2] * mv[2]);
+    ws[3] -= e * ss * mv[0] * mv[1];
+    ws[4] -= e * ss * mv[0] * mv[2];
+    ws[5] -= e * ss * mv[1] * mv[2];
+  }
+  *g = gs / ns;
+  if (w)
+    for (int q = 0; q < 6; ++q) w[q] = ws[q] / ns;
+}
+
+__global__ void k_green(GridArgs G, double fac, double kcoul, double vol, const double* __restrict__ bm, float* __restrict__ gtab) {
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t total = size_t(G.h1) * G.k[1] * G.k[2];
+  if (t >= total) return;
+  const int a = int(t % G.h1), b = int((t / G.h1) % G.k[1]), c = int(t / (size_t(G.h1) * G.k[1]));
+  double g = 0;
+  if (a || b || c) green_at(G, a, b, c, fac, kcoul, vol, bm, &g, nullptr);
+  gtab[t] = float(g);
+}
+
+// lines along c: forward transform, × G (energy and virial when kE), inverse transform
+template <bool kE>
+__global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half,
+                                                    const float* __restrict__ gtab, double fac, double kcoul, double vol,
+                                                    const double* __restrict__ bm, double* __restrict__ epart) {
+  extern __shared__ float2 lds[];
+  const int h1 = G.h1, K2 = G.k[1], K3 = G.k[2];
+  const int mblocks = (h1 + kLinesYZ - 1) / kLinesYZ;
+  const int jy = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) * kLinesYZ;
+  const int nl = min(kLinesYZ, h1 - m0);
+  float2* A = lds;
+  float2* B = lds + kLinesYZ * K3;
+  const size_t zs = size_t(h1) * K2;
+  for (int t = threadIdx.x; t < kLinesYZ * K3; t += blockDim.x) {
+    const int k = t / kLinesYZ, li = t % kLinesYZ;
+    if (li < nl) A[li * K3 + k] = half[size_t(m0 + li) + size_t(h1) * jy + zs * k];
+  }
+  __syncthreads();
+  float2* Z = fft_lines(A, B, nl, FP.ax[2], tw, +1);
+  double e = 0, w[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
+    const int li = t / K3, k = t - li * K3, a = m0 + li;
+    const float g = gtab[size_t(a) + size_t(h1) * jy + zs * k];
+    float2 v = Z[li * K3 + k];
+    if (kE && (a || jy || k)) {
+      double em = 0.5 * (double(v.x) * v.x + double(v.y) * v.y);
+      if (a > 0 && 2 * a != G.k[0]) em *= 2;
+      double gd, wv[6];
+      green_at(G, a, jy, k, fac, kcoul, vol, bm, &gd, wv);
+      e += em * double(g);
+      for (int q = 0; q < 6; ++q) w[q] += em * wv[q];
+    }
+    v.x *= g, v.y *= g;
+    Z[li * K3 + k] = v;
+  }
+  __syncthreads();
+  float2* other = Z == A ? B : A;
+  const float2* R = fft_lines(Z, other, nl, FP.ax[2], tw, -1);
+  for (int t = threadIdx.x; t < kLinesYZ * K3; t += blockDim.x) {
+    const int k = t / kLinesYZ, li = t % kLinesYZ;
+    if (li < nl) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * K3 + k];
+  }
+  if (kE) {
+    // the workgroup's partial sums in a fixed order
+    __shared__ double red[7][256];
+    red[0][threadIdx.x] = e;
+    for (int q = 0; q < 6; ++q) red[1 + q][threadIdx.x] = w[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s)
+        for (int q = 0; q < 7; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x < 7) epart[7 * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+  }
+}
+
+__global__ void k_gather(GridArgs G, int n, const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ grid,
+                         float* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 w = wq[i];
+  const int4 b = base[i];
+  float tx[kMaxOrder], dx[kMaxOrder], ty[kMaxOrder], dy[kMaxOrder], tz[kMaxOrder], dz[kMaxOrder];
+  bspline(w.x, G.p, tx, dx);
+  bspline(w.y, G.p, ty, dy);
+  bspline(w.z, G.p, tz, dz);
+  float g0 = 0, g1 = 0, g2 = 0;
+  for (int c = 0; c < G.p; ++c) {
+    int iz = b.z + c;
+    iz -= iz >= G.k[2] ? G.k[2] : 0;
+    for (int bb = 0; bb < G.p; ++bb) {
+      int iy = b.y + bb;
+      iy -= iy >= G.k[1] ? G.k[1] : 0;
+      const float* row = grid + (size_t(iz) * G.k[1] + iy) * G.k[0];
+      float s0 = 0, s1 = 0;
+      for (int a = 0; a < G.p; ++a) {
+        int ix = b.x + a;
+        ix -= ix >= G.k[0] ? G.k[0] : 0;
+        const float v = row[ix];
+        s0 += dx[a] * v;
+        s1 += tx[a] * v;
+      }
+      g0 += s0 * ty[bb] * tz[c];
+      g1 += s1 * dy[bb] * tz[c];
+      g2 += s1 * ty[bb] * dz[c];
+    }
+  }
+  for (int m = 0; m < 3; ++m) {
+    const double grad = G.k[0] * double(g0) * G.hi[0][m] + G.k[1] * double(g1) * G.hi[1][m] + G.k[2] * double(g2) * G.hi[2][m];
+    f[3 * i + m] = float(-double(w.w) * grad);
+  }
+}
+
+GridArgs grid_args(const Pme& P) {
+  GridArgs G{};
+  for (int d = 0; d < 3; ++d) G.k[d] = P.k[d], G.nt[d] = (P.k[d] + 3) / 4;
+  G.p = P.order;
+  G.h1 = P.h1;
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) G.hi[r][q] = P.hi[r][q];
+  return G;
+}
+
+// B-spline weights at the integers and |b(m)|² with the fix at zeros (f90:601-620)
+void moduli(int k, int p, double* out) {
+  double th[16] = {0};
+  th[1] = 0, th[0] = 1;
+  // bspline(0, p) in double
+  for (int j = 0; j < p; ++j) th[j] = 0;
+  th[1] = 0.0;
+  th[0] = 1.0;
+  for (int kk = 3; kk <= p - 1; ++kk) {
+    const double div = 1.0 / (kk - 1);
+    th[kk - 1] = div * 0.0 * th[kk - 2];
+    for (int j = 1; j <= kk - 2; ++j) th[kk - j - 1] = div * ((0.0 + j) * th[kk - j - 2] + (kk - j - 0.0) * th[kk - j - 1]);
+    th[0] = div * (1 - 0.0) * th[0];
+  }
+  {
+    const double div = 1.0 / (p - 1);
+    th[p - 1] = div * 0.0 * th[p - 2];
+    for (int j = 1; j <= p - 2; ++j) th[p - j - 1] = div * ((0.0 + j) * th[p - j - 2] + (p - j - 0.0) * th[p - j - 1]);
+    th[0] = div * (1 - 0.0) * th[0];
+  }
+  for (int m = 0; m < k; ++m) {
+    double sr = 0, si = 0;
+    for (int j = 0; j <= p - 2; ++j) {
+      const double arg = 2 * kPi * m * j / double(k);
+      sr += th[j] * std::cos(arg);
+      si += th[j] * std::sin(arg);
+    }
+    out[m] = sr * sr + si * si;
+  }
+  for (int m = 0; m < k; ++m)
+    if (out[m] < 1e-7) out[m] = 0.5 * (out[(m - 1 + k) % k] + out[(m + 1) % k]);
+}
+
+FftPlan plan_of(const Pme& P) {
+  FftPlan F{};
+  F.ax[0] = make_fft(P.k[0], 0);
+  F.ax[1] = make_fft(P.k[1], P.k[0]);
+  F.ax[2] = make_fft(P.k[2], P.k[0] + P.k[1]);
+  return F;
+}
+
+}  // namespace
+
+// the grid, its tables and the influence function for the current cell (kept while the cell stays the same)
+void pme_setup(capsgpu_ctx& c) {
+  Pme& P = c.pme;
+  const double* cell = c.cell;
+  bool same = P.green_ok;
+  for (int q = 0; q < 12 && same; ++q) same = P.cell[q] == cell[q];
+  if (same) return;
+  // h = cell rows as columns; h⁻¹ rows the reciprocal vectors (f90:484-500)
+  double h[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) h[r][q] = cell[3 + 3 * q + r];
+  const double v = h[0][0] * (h[1][1] * h[2][2] - h[2][1] * h[1][2]) - h[0][1] * (h[1][0] * h[2][2] - h[2][0] * h[1][2]) +
+                   h[0][2] * (h[1][0] * h[2][1] - h[2][0] * h[1][1]);
+  P.hi[0][0] = (h[1][1] * h[2][2] - h[1][2] * h[2][1]) / v;
+  P.hi[0][1] = (h[0][2] * h[2][1] - h[0][1] * h[2][2]) / v;
+  P.hi[0][2] = (h[0][1] * h[1][2] - h[0][2] * h[1][1]) / v;
+  P.hi[1][0] = (h[1][2] * h[2][0] - h[1][0] * h[2][2]) / v;
+  P.hi[1][1] = (h[0][0] * h[2][2] - h[0][2] * h[2][0]) / v;
+  P.hi[1][2] = (h[0][2] * h[1][0] - h[0][0] * h[1][2]) / v;
+  P.hi[2][0] = (h[1][0] * h[2][1] - h[1][1] * h[2][0]) / v;
+  P.hi[2][1] = (h[0][1] * h[2][0] - h[0][0] * h[2][1]) / v;
+  P.hi[2][2] = (h[0][0] * h[1][1] - h[0][1] * h[1][0]) / v;
+  P.vol = std::fabs(v);
+  // grid: the smallest 2-3-5-7 sizes with spacing at most pme_spacing (kspace.cpp:45-52, f90:23-36)
+  auto good = [](int n) {
+    int m = std::max(n, 1);
+    for (;;) {
+      int k = m;
+      for (int f : {2, 3, 5, 7})
+        while (k % f == 0) k /= f;
+      if (k == 1) return m;
+      ++m;
+    }
+  };
+  for (int d = 0; d < 3; ++d) {
+    const double* e = cell + 3 + 3 * d;
+    const double len = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    P.k[d] = good(std::max(c.opt.pme_order + 1, int(std::ceil(len / c.opt.pme_spacing))));
+    if (P.k[d] > kMaxFft) throw Error(-1, "PME grid larger than 512 points along an axis");
+  }
+  P.order = c.opt.pme_order;
+  if (P.order < 3 || P.order > kMaxOrder) throw Error(-1, "PME order must be 3 … 8");
+  P.h1 = P.k[0] / 2 + 1;
+  hipStream_t s = c.stream;
+  std::vector<double> bm(size_t(P.k[0] + P.k[1] + P.k[2]));
+  moduli(P.k[0], P.order, bm.data());
+  moduli(P.k[1], P.order, bm.data() + P.k[0]);
+  moduli(P.k[2], P.order, bm.data() + P.k[0] + P.k[1]);
+  P.bmod.upload(bm, s);
+  std::vector<float2> tw;
+  for (int d = 0; d < 3; ++d)
+    for (int m = 0; m < P.k[d]; ++m) {
+      const double ang = 2 * kPi * m / P.k[d];
+      tw.push_back(make_float2(float(std::cos(ang)), float(std::sin(ang))));
+    }
+  P.tw.upload(tw, s);
+  const size_t nh = size_t(P.h1) * P.k[1] * P.k[2];
+  P.gtab.resize(nh);
+  const GridArgs G = grid_args(P);
+  const double fac = kPi * kPi / (P.beta * P.beta);
+  hipLaunchKernelGGL(k_green, dim3(blocks_for(nh, 256)), dim3(256), 0, s, G, fac, kCoulomb, P.vol, P.bmod.p, P.gtab.p);
+  LAUNCHCHECK();
+  P.grid.resize(size_t(P.k[0]) * P.k[1] * P.k[2]);
+  P.half.resize(nh);
+  for (int q = 0; q < 12; ++q) P.cell[q] = cell[q];
+  P.green_ok = true;
+}
+
+void pme_forces(capsgpu_ctx& c, bool want) {
+  Pme& P = c.pme;
+  pme_setup(c);
+  hipStream_t s = c.stream;
+  const int n = c.n;
+  const GridArgs G = grid_args(P);
+  const FftPlan F = plan_of(P);
+  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+  P.theta.resize(4 * size_t(n));
+  P.base.resize(4 * size_t(n));
+  P.tcount.resize(size_t(ntiles));
+  P.tcount.zero(s);
+  float4* wq = reinterpret_cast<float4*>(P.theta.p);
+  int4* base = reinterpret_cast<int4*>(P.base.p);
+  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p);
+  LAUNCHCHECK();
+  P.tstart.resize(size_t(ntiles) + 1);
+  c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
+  P.tfill.resize(size_t(ntiles));
+  P.tfill.zero(s);
+  P.tatoms.resize(size_t(n));
+  hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tfill.p, P.tatoms.p);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_tile_sort, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, ntiles, P.tstart.p, P.tatoms.p);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_spread, dim3(blocks_for(ntiles, 4)), dim3(256), 0, s, G, P.tstart.p, P.tatoms.p, wq, base, P.grid.p);
+  LAUNCHCHECK();
+  const int npairs = (P.k[1] * P.k[2] + 1) / 2;
+  const size_t lds_x = 2 * 4 * size_t(P.k[0]) * sizeof(float2);
+  hipLaunchKernelGGL(k_fft_x_r2c, dim3(unsigned((npairs + 3) / 4)), dim3(256), lds_x, s, G, F, P.tw.p, P.grid.p, P.half.p);
+  LAUNCHCHECK();
+  const int mblocks = (P.h1 + kLinesYZ - 1) / kLinesYZ;
+  const size_t lds_y = 2 * kLinesYZ * size_t(P.k[1]) * sizeof(float2), lds_z = 2 * kLinesYZ * size_t(P.k[2]) * sizeof(float2);
+  hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, +1);
+  LAUNCHCHECK();
+  const unsigned zb = unsigned(mblocks * P.k[1]);
+  const double fac = kPi * kPi / (P.beta * P.beta);
+  if (want) {
+    P.epart.resize(7 * size_t(zb));
+    hipLaunchKernelGGL(k_fft_z_conv<true>, dim3(zb), dim3(256), lds_z, s, G, F, P.tw.p, P.half.p, P.gtab.p, fac, kCoulomb, P.vol,
+                       P.bmod.p, P.epart.p);
+  } else {
+    hipLaunchKernelGGL(k_fft_z_conv<false>, dim3(zb), dim3(256), lds_z, s, G, F, P.tw.p, P.half.p, P.gtab.p, fac, kCoulomb, P.vol,
+                       P.bmod.p, nullptr);
+  }
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, -1);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_fft_x_c2r, dim3(unsigned((npairs + 3) / 4)), dim3(256), lds_x, s, G, F, P.tw.p, P.half.p, P.grid.p);
+  LAUNCHCHECK();
+  P.fpme.resize(3 * size_t(n));
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, wq, base, P.grid.p, P.fpme.p);
+  LAUNCHCHECK();
+}
+
+}  // namespace capsgpu

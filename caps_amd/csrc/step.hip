@@ -1,0 +1,509 @@
+// Bonded terms, force summation, integrator steps and deterministic reductions on gfx950, all in FP64.
+//
+//  k_bonded     one thread per atom: every term the atom is in is evaluated and the atom takes its own share, so no
+//               two threads write one force (no atomics); a term's energy and virial are counted by its first atom.
+//               Harmonic bond and angle (core/src/field.cpp:1980-2003), FENE + WCA (:2150-2180).
+//  k_sum_forces the total force per atom: pair forces (by slot, float) + reciprocal (float) + bonded (double)
+//  k_vv_open    velocity Verlet's opening half kick and drift (core/src/dynamics.cpp:405-431, 657-666), Bussi's
+//               scaling of the previous step folded in
+//  k_baoab_open Langevin BAOAB's B, A, O, A (dynamics.cpp:613-621), O with counter-based normal noise
+//  k_kick       the closing half kick, with kinetic-energy partial sums on thermo steps
+//  k_colsum     column sums of per-block partials in a fixed order (bit-reproducible energies and virials)
+#include <cmath>
+
+#include "context.hpp"
+#include "step.hpp"
+
+namespace capsgpu {
+namespace {
+
+struct CellArgs {
+  double a[3], b[3], c[3], hi[3][3];
+  int ortho;
+};
+
+__device__ inline void min_image(const CellArgs& C, double* d) {
+  if (C.ortho) {
+    const double L[3] = {C.a[0], C.b[1], C.c[2]};
+    for (int k = 0; k < 3; ++k) d[k] -= L[k] * round(d[k] / L[k]);
+    return;
+  }
+  // Cell::minimum_image (core/src/system.cpp:27-45)
+  double f[3];
+  for (int k = 0; k < 3; ++k) f[k] = C.hi[k][0] * d[0] + C.hi[k][1] * d[1] + C.hi[k][2] * d[2];
+  for (int k = 0; k < 3; ++k) f[k] -= round(f[k]);
+  double r[3];
+  for (int k = 0; k < 3; ++k) r[k] = C.a[k] * f[0] + C.b[k] * f[1] + C.c[k] * f[2];
+  double best = r[0] * r[0] + r[1] * r[1] + r[2] * r[2], bt[3] = {r[0], r[1], r[2]};
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j)
+      for (int k = -1; k <= 1; ++k) {
+        double t[3];
+        for (int m = 0; m < 3; ++m) t[m] = r[m] + C.a[m] * i + C.b[m] * j + C.c[m] * k;
+        const double q = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        if (q < best - 1e-12) best = q, bt[0] = t[0], bt[1] = t[1], bt[2] = t[2];
+      }
+  d[0] = bt[0], d[1] = bt[1], d[2] = bt[2];
+}
+
+__device__ inline void vir_add(double* A, const double* d, const double* f) {
+  A[2] += d[0] * f[0] + d[1] * f[1] + d[2] * f[2];
+  A[3] += d[0] * f[0], A[4] += d[1] * f[1], A[5] += d[2] * f[2];
+  A[6] += 0.5 * (d[0] * f[1] + d[1] * f[0]), A[7] += 0.5 * (d[0] * f[2] + d[2] * f[0]), A[8] += 0.5 * (d[1] * f[2] + d[2] * f[1]);
+}
+
+constexpr int kEB = 9;   // bond, angle, virial, w[6]
+
+template <bool kE>
+__global__ void __launch_bounds__(256) k_bonded(int n, CellArgs C, const double* __restrict__ x, const int32_t* __restrict__ aoff,
+                                                const int32_t* __restrict__ aterm, const int32_t* __restrict__ bij,
+                                                const double* __restrict__ bkr, const int32_t* __restrict__ aijk,
+                                                const double* __restrict__ akt, const int32_t* __restrict__ fij,
+                                                const double* __restrict__ fp, double* __restrict__ fout, double* __restrict__ epart,
+                                                uint32_t* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    double F[3] = {0, 0, 0};
+    for (int32_t e = aoff[i]; e < aoff[i + 1]; ++e) {
+      const int32_t code = aterm[e], kind = code >> 28, t = code & 0x0fffffff;
+      if (kind == 0 || kind == 2) {
+        const int32_t a = kind == 0 ? bij[2 * t] : fij[2 * t], b = kind == 0 ? bij[2 * t + 1] : fij[2 * t + 1];
+        double d[3] = {x[3 * b] - x[3 * a], x[3 * b + 1] - x[3 * a + 1], x[3 * b + 2] - x[3 * a + 2]};
+        min_image(C, d);
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double en, dEdr;
+        if (kind == 0) {
+          const double k = bkr[2 * t], dr = r - bkr[2 * t + 1];
+          en = k * dr * dr;
+          dEdr = 2 * k * dr;
+        } else {
+          const double K = fp[4 * t], R0 = fp[4 * t + 1], eps = fp[4 * t + 2], sg = fp[4 * t + 3], q = r / R0;
+          if (q >= 1) {
+            atomicOr(&flags[2], 1u);
+            continue;
+          }
+          en = -0.5 * K * R0 * R0 * log(1 - q * q);
+          dEdr = K * r / (1 - q * q);
+          if (eps > 0 && r < 1.122462048309373 * sg) {
+            const double s2 = (sg / r) * (sg / r), s6 = s2 * s2 * s2;
+            en += 4 * eps * (s6 * s6 - s6) + eps;
+            dEdr += -24 * eps * (2 * s6 * s6 - s6) / r;
+          }
+        }
+        double fj[3] = {0, 0, 0};
+        if (r > 1e-12)
+          for (int m = 0; m < 3; ++m) fj[m] = d[m] * (-dEdr / r);
+        const double sgn = i == b ? 1.0 : -1.0;
+        for (int m = 0; m < 3; ++m) F[m] += sgn * fj[m];
+        if (kE && i == a) {
+          A[0] += en;
+          vir_add(A, d, fj);
+        }
+      } else {
+        const int32_t ia = aijk[3 * t], ja = aijk[3 * t + 1], ka = aijk[3 * t + 2];
+        double u[3] = {x[3 * ia] - x[3 * ja], x[3 * ia + 1] - x[3 * ja + 1], x[3 * ia + 2] - x[3 * ja + 2]};
+        double v[3] = {x[3 * ka] - x[3 * ja], x[3 * ka + 1] - x[3 * ja + 1], x[3 * ka + 2] - x[3 * ja + 2]};
+        min_image(C, u);
+        min_image(C, v);
+        const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (lu * lv);
+        cs = cs < -1 ? -1 : cs > 1 ? 1 : cs;
+        const double th = acos(cs), sn = fmax(sqrt(1 - cs * cs), 1e-8);
+        const double dth = th - akt[2 * t + 1], kt = akt[2 * t];
+        const double g = 2 * kt * dth / sn;
+        double fi[3], fk[3];
+        for (int m = 0; m < 3; ++m) {
+          fi[m] = (v[m] * (1 / (lu * lv)) - u[m] * (cs / (lu * lu))) * g;
+          fk[m] = (u[m] * (1 / (lu * lv)) - v[m] * (cs / (lv * lv))) * g;
+        }
+        if (i == ia)
+          for (int m = 0; m < 3; ++m) F[m] += fi[m];
+        else if (i == ka)
+          for (int m = 0; m < 3; ++m) F[m] += fk[m];
+        else
+          for (int m = 0; m < 3; ++m) F[m] -= fi[m] + fk[m];
+        if (kE && i == ia) {
+          A[1] += kt * dth * dth;
+          vir_add(A, u, fi);
+          vir_add(A, v, fk);
+        }
+      }
+    }
+    for (int m = 0; m < 3; ++m) fout[3 * i + m] = F[m];
+  }
+  if (kE) {
+    __shared__ double red[kEB][256];
+    for (int q = 0; q < kEB; ++q) red[q][threadIdx.x] = A[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s)
+        for (int q = 0; q < kEB; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x < kEB) epart[kEB * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+  }
+}
+
+__global__ void k_sum_forces(int n, const uint32_t* __restrict__ iperm, const float4* __restrict__ fnb, const float* __restrict__ frec,
+                             const double* __restrict__ fb, double* __restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double F[3] = {0, 0, 0};
+  if (fnb) {
+    const float4 p = fnb[iperm[i]];
+    F[0] += double(p.x), F[1] += double(p.y), F[2] += double(p.z);
+  }
+  if (frec)
+    for (int m = 0; m < 3; ++m) F[m] += double(frec[3 * i + m]);
+  if (fb)
+    for (int m = 0; m < 3; ++m) F[m] += fb[3 * i + m];
+  for (int m = 0; m < 3; ++m) f[3 * i + m] = F[m];
+}
+
+// Σ_slots x ⊗ F of the pair forces, x the listed positions X + origin (core/src/nbnxm.cpp:585-592)
+__global__ void __launch_bounds__(256) k_slot_virial(int64_t ns, const float4* __restrict__ xq, const double* __restrict__ org,
+                                                     const float4* __restrict__ fnb, double* __restrict__ part) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (s < ns) {
+    const float4 p = xq[s], F = fnb[s];
+    const double* o = org + 3 * (s / 4);
+    const double r[3] = {double(p.x) + o[0], double(p.y) + o[1], double(p.z) + o[2]};
+    const double f[3] = {double(F.x), double(F.y), double(F.z)};
+    for (int u = 0; u < 3; ++u)
+      for (int w = 0; w < 3; ++w) W[3 * u + w] = r[u] * f[w];
+  }
+  __shared__ double red[9][256];
+  for (int q = 0; q < 9; ++q) red[q][threadIdx.x] = W[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 9; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) part[9 * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// column sums of rows × cols partials, each column by one workgroup in a fixed order
+__global__ void __launch_bounds__(256) k_colsum(const double* __restrict__ part, int64_t rows, int cols, double* __restrict__ out) {
+  const int c = blockIdx.x;
+  double s = 0;
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += part[r * cols + c];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// largest squared displacement against the positions of the last build (flag 0) and the last prune (flag 1), in
+// float-ordered bits (non-negative floats compare as their bit patterns)
+__device__ inline void move_flags(bool valid, const double* x, const double* x0, const double* x0p, int i, double lim_b, double lim_p,
+                                  uint32_t* flags) {
+  bool mb = false, mp = false, bad = false;
+  if (valid) {
+    double d0 = 0, d1 = 0;
+    for (int m = 0; m < 3; ++m) {
+      const double a = x[3 * i + m] - x0[3 * i + m], b = x[3 * i + m] - x0p[3 * i + m];
+      d0 += a * a, d1 += b * b;
+    }
+    mb = d0 > lim_b, mp = d1 > lim_p;
+    bad = !isfinite(d0);
+  }
+  const uint64_t bm = __ballot(mb), pm = __ballot(mp), nm = __ballot(bad);
+  if ((threadIdx.x & 63) == 0) {
+    if (bm) atomicOr(&flags[0], 1u);
+    if (pm) atomicOr(&flags[1], 1u);
+    if (nm) atomicOr(&flags[3], 1u);
+  }
+}
+
+__global__ void k_check_moves(int n, const double* __restrict__ x, const double* __restrict__ x0, const double* __restrict__ x0p,
+                              double lim_b, double lim_p, uint32_t* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC 2011): four 32-bit words from a counter and a key
+__device__ inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c[0], p1 = uint64_t(0xCD9E8D57u) * c[2];
+    const uint32_t h0 = uint32_t(p0 >> 32), l0 = uint32_t(p0), h1 = uint32_t(p1 >> 32), l1 = uint32_t(p1);
+    c[0] = h1 ^ c[1] ^ k0, c[1] = l1, c[2] = h0 ^ c[3] ^ k1, c[3] = l0;
+    k0 += 0x9E3779B9u, k1 += 0xBB67AE85u;
+  }
+}
+
+// three standard normals for (atom, step) by Box–Muller on Philox uniforms
+__device__ inline void normals3(uint64_t seed, uint64_t step, uint32_t atom, double* g) {
+  uint32_t c[4] = {atom, uint32_t(step), uint32_t(step >> 32), 0x43415053u};
+  philox(c, uint32_t(seed), uint32_t(seed >> 32));
+  double u[4];
+  for (int k = 0; k < 4; ++k) u[k] = (double(c[k]) + 0.5) * 2.3283064365386963e-10;
+  const double r0 = sqrt(-2 * log(u[0])), r1 = sqrt(-2 * log(u[2]));
+  g[0] = r0 * cos(2 * kPi * u[1]);
+  g[1] = r0 * sin(2 * kPi * u[1]);
+  g[2] = r1 * cos(2 * kPi * u[3]);
+}
+
+template <int kParts>
+__device__ inline void block_sum_store(double (&v)[kParts], double* part) {
+  __shared__ double red[kParts][256];
+  for (int q = 0; q < kParts; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < kParts; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < kParts) part[kParts * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// v ← α v (Bussi, the previous step) then v += h·kAcc/m F; x += dt v
+__global__ void __launch_bounds__(256) k_vv_open(int n, double hk, double dt, const double* __restrict__ minv, const double* __restrict__ alpha,
+                                                 const double* __restrict__ f, double* __restrict__ x, double* __restrict__ v,
+                                                 const double* __restrict__ x0, const double* __restrict__ x0p, double lim_b, double lim_p,
+                                                 uint32_t* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double al = alpha ? *alpha : 1.0, a = hk * minv[i];
+    for (int m = 0; m < 3; ++m) {
+      const double vv = al * v[3 * i + m] + a * f[3 * i + m];
+      v[3 * i + m] = vv;
+      x[3 * i + m] += dt * vv;
+    }
+  }
+  move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
+}
+
+// B (half kick), A (half drift), O, A; partials: kinetic energy before and after O (the bath's share)
+__global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
+                                                    const double* __restrict__ sdv, uint64_t seed, uint64_t step, const double* __restrict__ f,
+                                                    double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
+                                                    const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
+                                                    double* __restrict__ part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke[1] = {0};
+  if (i < n) {
+    const double a = hk * minv[i], mi = 1.0 / minv[i];
+    double g[3];
+    normals3(seed, step, uint32_t(i), g);
+    double k0 = 0, k1 = 0;
+    for (int m = 0; m < 3; ++m) {
+      double vv = v[3 * i + m] + a * f[3 * i + m];
+      double xx = x[3 * i + m] + hdt * vv;
+      k0 += vv * vv;
+      vv = c1 * vv + c2 * sdv[i] * g[m];
+      k1 += vv * vv;
+      xx += hdt * vv;
+      v[3 * i + m] = vv;
+      x[3 * i + m] = xx;
+    }
+    ke[0] = 0.5 * mi * (k1 - k0) / kAcc;
+  }
+  move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
+  block_sum_store<1>(ke, part);
+}
+
+// the closing half kick; kinetic energy partials (Σ ½ m v², and the tensor m v_a v_b) when asked
+template <bool kKE>
+__global__ void __launch_bounds__(256) k_kick(int n, double hk, const double* __restrict__ minv, const double* __restrict__ f,
+                                              double* __restrict__ v, double* __restrict__ part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double k[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const double a = hk * minv[i];
+    double vv[3];
+    for (int m = 0; m < 3; ++m) vv[m] = v[3 * i + m] + a * f[3 * i + m], v[3 * i + m] = vv[m];
+    if (kKE) {
+      const double mi = 1.0 / minv[i];
+      k[0] = 0.5 * mi * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]) / kAcc;
+      k[1] = mi * vv[0] * vv[0], k[2] = mi * vv[1] * vv[1], k[3] = mi * vv[2] * vv[2];
+      k[4] = mi * vv[0] * vv[1], k[5] = mi * vv[0] * vv[2], k[6] = mi * vv[1] * vv[2];
+    }
+  }
+  if (kKE) block_sum_store<7>(k, part);
+}
+
+// Bussi's α from the reduced kinetic energy (dynamics.cpp:466-477); r1 and Σ R² drawn on the host in the reference's
+// order. bath −= K_new − K.
+__global__ void k_bussi(const double* __restrict__ part, int64_t rows, double kt_target, double ndof, double c, double r1, double sum,
+                        double* __restrict__ alpha, double* __restrict__ bath) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += part[7 * r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double k = red[0];
+    double al = 1.0;
+    if (k > 0) {
+      const double knew = k + (1 - c) * (kt_target * (r1 * r1 + sum) / ndof - k) + 2 * r1 * sqrt(c * (1 - c) * kt_target * k / ndof);
+      al = sqrt(fmax(knew, 0.0) / k);
+      *bath -= al * al * k - k;
+    }
+    *alpha = al;
+  }
+}
+
+__global__ void k_scale_v(int n, const double* __restrict__ alpha, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * n) v[i] *= *alpha;
+}
+
+__global__ void k_add_scalar(const double* __restrict__ part, int64_t rows, double* __restrict__ acc) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += part[r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc -= red[0];
+}
+
+CellArgs cell_args(const capsgpu_ctx& c) {
+  CellArgs C{};
+  for (int k = 0; k < 3; ++k) C.a[k] = c.cell[3 + k], C.b[k] = c.cell[6 + k], C.c[k] = c.cell[9 + k];
+  C.ortho = C.a[1] == 0 && C.a[2] == 0 && C.b[0] == 0 && C.b[2] == 0 && C.c[0] == 0 && C.c[1] == 0;
+  const double* a = C.a;
+  const double* b = C.b;
+  const double* cc = C.c;
+  const double bc[3] = {b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]};
+  const double ca[3] = {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]};
+  const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double v = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+  for (int k = 0; k < 3; ++k) C.hi[0][k] = bc[k] / v, C.hi[1][k] = ca[k] / v, C.hi[2][k] = ab[k] / v;
+  return C;
+}
+
+}  // namespace
+
+void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
+  Bonded& B = c.bonded;
+  hipStream_t s = c.stream;
+  B.nb = ff.n_bonds, B.na = ff.n_angles, B.nf = ff.n_fene;
+  const int n = c.n;
+  std::vector<std::vector<int32_t>> per(static_cast<size_t>(n));
+  auto chk = [&](int32_t a) {
+    if (a < 0 || a >= n) throw Error(-1, "a bonded term names an atom out of range");
+  };
+  for (int t = 0; t < B.nb; ++t)
+    for (int k = 0; k < 2; ++k) chk(ff.bond_atoms[2 * t + k]), per[size_t(ff.bond_atoms[2 * t + k])].push_back(t);
+  for (int t = 0; t < B.na; ++t)
+    for (int k = 0; k < 3; ++k) chk(ff.angle_atoms[3 * t + k]), per[size_t(ff.angle_atoms[3 * t + k])].push_back(t | (1 << 28));
+  for (int t = 0; t < B.nf; ++t)
+    for (int k = 0; k < 2; ++k) chk(ff.fene_atoms[2 * t + k]), per[size_t(ff.fene_atoms[2 * t + k])].push_back(t | (2 << 28));
+  std::vector<int32_t> off(size_t(n) + 1, 0), idx;
+  for (int i = 0; i < n; ++i) {
+    off[size_t(i) + 1] = off[size_t(i)] + int32_t(per[size_t(i)].size());
+    idx.insert(idx.end(), per[size_t(i)].begin(), per[size_t(i)].end());
+  }
+  if (idx.empty()) idx.push_back(0);
+  B.aoff.upload(off, s);
+  B.aterm.upload(idx, s);
+  auto up_i = [&](DevBuf<int32_t>& d, const int32_t* p, size_t m) {
+    if (m) d.upload(p, m, s);
+    else d.resize(1);
+  };
+  auto up_d = [&](DevBuf<double>& d, const double* p, size_t m) {
+    if (m) d.upload(p, m, s);
+    else d.resize(1);
+  };
+  up_i(B.bij, ff.bond_atoms, 2 * size_t(B.nb));
+  up_d(B.bkr, ff.bond_param, 2 * size_t(B.nb));
+  up_i(B.aijk, ff.angle_atoms, 3 * size_t(B.na));
+  up_d(B.akt, ff.angle_param, 2 * size_t(B.na));
+  up_i(B.fij, ff.fene_atoms, 2 * size_t(B.nf));
+  up_d(B.fp, ff.fene_param, 4 * size_t(B.nf));
+  B.f.resize(3 * size_t(n));
+}
+
+void bonded_forces(capsgpu_ctx& c, bool want) {
+  Bonded& B = c.bonded;
+  const int n = c.n;
+  const CellArgs C = cell_args(c);
+  const unsigned nb = blocks_for(n, 256);
+  if (want) {
+    B.epart.resize(kEB * size_t(nb));
+    hipLaunchKernelGGL(k_bonded<true>, dim3(nb), dim3(256), 0, c.stream, n, C, c.x.p, B.aoff.p, B.aterm.p, B.bij.p, B.bkr.p, B.aijk.p,
+                       B.akt.p, B.fij.p, B.fp.p, B.f.p, B.epart.p, c.flags.p);
+  } else {
+    hipLaunchKernelGGL(k_bonded<false>, dim3(nb), dim3(256), 0, c.stream, n, C, c.x.p, B.aoff.p, B.aterm.p, B.bij.p, B.bkr.p, B.aijk.p,
+                       B.akt.p, B.fij.p, B.fp.p, B.f.p, nullptr, c.flags.p);
+  }
+  LAUNCHCHECK();
+}
+
+void sum_forces(capsgpu_ctx& c, bool nb, bool rec, bool bond, double* fdst) {
+  hipLaunchKernelGGL(k_sum_forces, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, c.L.iperm.p, nb ? c.L.fnb.p : nullptr,
+                     rec ? c.pme.fpme.p : nullptr, bond ? c.bonded.f.p : nullptr, fdst);
+  LAUNCHCHECK();
+}
+
+void slot_virial(capsgpu_ctx& c, double* part) {
+  const int64_t ns = c.L.g.ns;
+  hipLaunchKernelGGL(k_slot_virial, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.L.xq.p, c.L.org.p, c.L.fnb.p, part);
+  LAUNCHCHECK();
+}
+
+void colsum(capsgpu_ctx& c, const double* part, int64_t rows, int cols, double* out) {
+  hipLaunchKernelGGL(k_colsum, dim3(unsigned(cols)), dim3(256), 0, c.stream, part, rows, cols, out);
+  LAUNCHCHECK();
+}
+
+void check_moves(capsgpu_ctx& c, double lim_build, double lim_prune) {
+  hipLaunchKernelGGL(k_check_moves, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, c.x.p, c.x0.p, c.x0p.p, lim_build, lim_prune,
+                     c.flags.p);
+  LAUNCHCHECK();
+}
+
+void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p) {
+  hipLaunchKernelGGL(k_vv_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, dt, c.d_minv.p, alpha, c.f.p, c.x.p, c.v.p,
+                     c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p);
+  LAUNCHCHECK();
+}
+
+void baoab_open(capsgpu_ctx& c, double hk, double hdt, double c1, double c2, const double* sdv, uint64_t seed, uint64_t step, double lim_b,
+                double lim_p, double* part) {
+  hipLaunchKernelGGL(k_baoab_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, hdt, c1, c2, c.d_minv.p, sdv, seed, step,
+                     c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part);
+  LAUNCHCHECK();
+}
+
+void kick(capsgpu_ctx& c, double hk, bool ke, double* part) {
+  if (ke)
+    hipLaunchKernelGGL(k_kick<true>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, c.f.p, c.v.p, part);
+  else
+    hipLaunchKernelGGL(k_kick<false>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, c.f.p, c.v.p, part);
+  LAUNCHCHECK();
+}
+
+void bussi(capsgpu_ctx& c, const double* part, double kt_target, double ndof, double cc, double r1, double sum, double* alpha, double* bath) {
+  hipLaunchKernelGGL(k_bussi, dim3(1), dim3(256), 0, c.stream, part, int64_t(blocks_for(c.n, 256)), kt_target, ndof, cc, r1, sum, alpha,
+                     bath);
+  LAUNCHCHECK();
+}
+
+void scale_v(capsgpu_ctx& c, const double* alpha) {
+  hipLaunchKernelGGL(k_scale_v, dim3(blocks_for(3 * size_t(c.n), 256)), dim3(256), 0, c.stream, c.n, alpha, c.v.p);
+  LAUNCHCHECK();
+}
+
+void bath_add(capsgpu_ctx& c, const double* part, double* bath) {
+  hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(256), 0, c.stream, part, int64_t(blocks_for(c.n, 256)), bath);
+  LAUNCHCHECK();
+}
+
+}  // namespace capsgpu

@@ -1,0 +1,168 @@
+/* capsgpu — the MI355X force-and-step path of CAPS behind a plain C ABI.
+ *
+ * The library replaces, for one GPU, the two C++ interfaces CAPS's protocols call for dynamics and energies:
+ *   caps::Evaluator (core/include/caps/field.hpp:327-352: construct from a ForceField + EnergyOptions, then
+ *     compute(x, cell, f) -> EnergyTerms, energies(bool), list_builds())                      -> capsgpu_create / _compute
+ *   caps::run_dynamics (core/include/caps/dynamics.hpp:180; loop core/src/dynamics.cpp:606-774) -> capsgpu_run_dynamics
+ * next to the stable C surface of capi (capi/include/caps_c.h:149-193: caps_md, caps_last_error). Conventions follow
+ * capi: every call returns 0 on success or a negative error class, and capsgpu_last_error() holds the message of the
+ * calling thread's last failure (capi/src/caps_c.cpp:353, 373-383).
+ *
+ * Arrays are host memory owned by the caller; the state (positions, velocities, lists, meshes) lives in device memory
+ * owned by the context. A context is single-caller, as an Evaluator is.
+ * Units: Å, fs, kcal/mol, e, g/mol; positions and forces 3N doubles, atom-major (x0 y0 z0 x1 ...).
+ * A cell is double[12]: origin, a, b, c (Cartesian edge vectors), periodic along all three axes.
+ */
+#ifndef CAPSGPU_H
+#define CAPSGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAPSGPU_OK 0
+#define CAPSGPU_EINVAL (-1)     /* a bad argument (std::invalid_argument in the reference) */
+#define CAPSGPU_EFIELD (-2)     /* a force-field error (caps::FieldError): FENE bond past R0, unstable run */
+#define CAPSGPU_EDEVICE (-3)    /* a HIP runtime failure or no GPU */
+#define CAPSGPU_ECANCEL (-4)    /* the progress callback asked to stop (caps::DynamicsCancelled) */
+
+typedef struct capsgpu_ctx capsgpu_ctx;
+
+/* The flattened force field: what caps::ForceField (core/include/caps/field.hpp:154-249) holds for the plain pair,
+ * bonded and FENE terms this path evaluates. */
+typedef struct capsgpu_ff {
+  int32_t n_atoms;
+  int32_t n_types;
+  const int32_t* type;          /* [n] type index (ForceField::type_index) */
+  const double* charge;         /* [n] charges, e (already divided by √εr) */
+  const double* mass;           /* [n] g/mol */
+  const double* eps;            /* [nt·nt] mixed ε of each type pair (caps::mixed_pair, field.cpp:2593-2604) */
+  const double* sigma;          /* [nt·nt] mixed σ */
+  int32_t lj96;                 /* 0: 12-6 4ε[(σ/r)¹² − (σ/r)⁶]; 1: 9-6 ε[2(σ/r)⁹ − 3(σ/r)⁶] */
+  const int32_t* excl_offset;   /* [n+1] CSR of the excluded partners, sorted (ForceField::excluded) */
+  const int32_t* excl_index;
+  const int32_t* p14_offset;    /* [n+1] or NULL: the excluded partners that are 1-4 pairs (Coulomb keeps coul14) */
+  const int32_t* p14_index;
+  double coul14;
+  int32_t n_bonds;              /* harmonic k (r − r0)²: [2·nb] atoms, [2·nb] (k, r0) */
+  const int32_t* bond_atoms;
+  const double* bond_param;
+  int32_t n_angles;             /* harmonic k_θ (θ − θ0)², j the vertex: [3·na] atoms, [2·na] (k_θ, θ0 in radians) */
+  const int32_t* angle_atoms;
+  const double* angle_param;
+  int32_t n_fene;               /* FENE −½KR0² ln(1 − (r/R0)²) + WCA(ε, σ): [2·nf] atoms, [4·nf] (K, R0, ε, σ) */
+  const int32_t* fene_atoms;
+  const double* fene_param;
+} capsgpu_ff;
+
+/* caps::EnergyOptions (core/include/caps/field.hpp:274-308), the fields this path reads */
+typedef struct capsgpu_energy_opts {
+  double cutoff;                /* Å */
+  double skin;                  /* Å */
+  int32_t coulomb;              /* 0 off, 1 on */
+  int32_t electrostatics;       /* 0 damped shifted force, 1 PME */
+  double dsf_alpha;             /* Å⁻¹ */
+  double ewald_rtol;
+  double pme_spacing;           /* Å */
+  int32_t pme_order;            /* 3 … 8 */
+  int32_t tail;                 /* LJ tail correction (else LJ shifted to zero at the cut-off) */
+  int32_t parts;                /* 1 bonded, 2 non-bonded (pairs, k-space, self, tail), 3 both */
+  int32_t no_reciprocal;        /* 1: PME's reciprocal part, self and background terms left out (the real-space part
+                                   alone, for checks); 0 the evaluator as the reference's */
+} capsgpu_energy_opts;
+
+/* caps::EnergyTerms (core/include/caps/field.hpp:310-320) as 15 doubles */
+enum {
+  CAPSGPU_T_BOND = 0, CAPSGPU_T_ANGLE, CAPSGPU_T_DIHEDRAL, CAPSGPU_T_IMPROPER, CAPSGPU_T_VDW, CAPSGPU_T_COULOMB,
+  CAPSGPU_T_VIRIAL, CAPSGPU_T_WXX, CAPSGPU_T_WYY, CAPSGPU_T_WZZ, CAPSGPU_T_WXY, CAPSGPU_T_WXZ, CAPSGPU_T_WYZ,
+  CAPSGPU_T_TAIL_ENERGY, CAPSGPU_T_TAIL_VIRIAL, CAPSGPU_NTERMS
+};
+
+/* The messages of the last failed call on this thread (caps_last_error, capi/include/caps_c.h:63). */
+const char* capsgpu_last_error(void);
+
+/* Evaluator(ff, options) (field.hpp:329): uploads the force field; the arrays may be freed afterwards. */
+int32_t capsgpu_create(const capsgpu_ff* ff, const capsgpu_energy_opts* opts, capsgpu_ctx** out);
+int32_t capsgpu_destroy(capsgpu_ctx* ctx);
+
+/* Evaluator::compute(x, cell, f) (field.hpp:337, field.cpp:1249-2539): energy terms and forces at x. The pair list is
+ * kept between calls and rebuilt when an atom has moved more than half the skin (field.cpp:800-838), pruned at
+ * r_c + 0.8 Å when one has moved 0.4 Å (field.cpp:1306-1312). want_energies = 0 skips the pair energies and the virial
+ * (Evaluator::energies(false), field.hpp:341). f: 3N out (overwritten). terms: CAPSGPU_NTERMS out. */
+int32_t capsgpu_compute(capsgpu_ctx* ctx, const double* x, const double cell[12], double* f, double* terms,
+                        int32_t want_energies);
+
+/* Evaluator::list_builds() (field.hpp:338) */
+int32_t capsgpu_list_builds(capsgpu_ctx* ctx, int32_t* builds);
+
+/* The device state for dynamics: positions (3N), velocities (3N, Å/fs, NULL: zero), cell. get_state reads them back
+ * (a checkpoint, core/src/dynamics.cpp:759-763); any pointer may be NULL. */
+int32_t capsgpu_set_state(capsgpu_ctx* ctx, const double* x, const double* v, const double cell[12]);
+int32_t capsgpu_get_state(capsgpu_ctx* ctx, double* x, double* v, double* cell);
+/* the forces of the last evaluation (3N) */
+int32_t capsgpu_get_forces(capsgpu_ctx* ctx, double* f);
+
+/* caps::DynamicsOptions (core/include/caps/dynamics.hpp:60-154), the fields this path reads */
+typedef struct capsgpu_dyn_opts {
+  double dt;                    /* fs */
+  int64_t steps;
+  double temperature;           /* K, thermostat target and initial velocities */
+  int32_t thermostat;           /* 0 none (NVE), 1 Bussi, 2 Langevin (BAOAB) */
+  double tau_t;                 /* fs: Bussi relaxation time, Langevin 1/γ */
+  uint64_t seed;
+  int32_t thermo_every;         /* steps between thermo rows */
+  int32_t new_velocities;       /* 1: Maxwell–Boltzmann at temperature (dynamics.cpp:236-256), else the set state */
+  int32_t respa;                /* r-RESPA inner steps for the bonded forces (1: off), Bussi or NVE only */
+  int64_t step_offset;          /* added to reported steps */
+  /* learned r-RESPA (no reference counterpart): 0 off; k > 1 a full PME evaluation every k steps, the MLP's predicted
+   * increment of the reciprocal-space force in between, falling back to plain PME when its force RMSE exceeds
+   * learned_tol (relative to the full force) at a full step */
+  int32_t learned_k;
+  double learned_tol;
+} capsgpu_dyn_opts;
+
+/* caps::ThermoRow (core/include/caps/dynamics.hpp:28-43), the fields this path fills */
+typedef struct capsgpu_thermo {
+  int64_t step;
+  double time_ps, temperature, kinetic, potential, total, pressure, conserved;
+  double terms[CAPSGPU_NTERMS];
+  int32_t learned;              /* learned r-RESPA: 1 this row's forces came from the MLP path */
+  double learned_rmse;          /* learned r-RESPA: the last judged relative force RMSE (0 before the first) */
+} capsgpu_thermo;
+
+/* Progress callback with each thermo row: return 0 to cancel (caps::DynamicsOptions::progress, dynamics.hpp:~141) */
+typedef int32_t (*capsgpu_progress_fn)(const capsgpu_thermo* row, void* user);
+
+typedef struct capsgpu_dyn_report {
+  int64_t steps;
+  double seconds, ns_per_day;
+  int32_t list_builds, prunes;
+  int32_t learned_steps, full_steps, fallbacks;
+} capsgpu_dyn_report;
+
+/* run_dynamics (dynamics.cpp:71-815) from the set state: velocity Verlet + Bussi, Langevin BAOAB, NVE, r-RESPA; the
+ * state stays on the device and can be read back with capsgpu_get_state. rows: up to cap thermo rows written (NULL:
+ * none), *nrows their count. */
+int32_t capsgpu_run_dynamics(capsgpu_ctx* ctx, const capsgpu_dyn_opts* opts, capsgpu_progress_fn progress, void* user,
+                             capsgpu_thermo* rows, int32_t cap, int32_t* nrows, capsgpu_dyn_report* report);
+
+/* ---- inspection, for the parity checks (no counterpart in the reference's interfaces) ----------------------------- */
+/* sizes of the current cluster list: atoms, clusters, slots, outer entries, image codes, columns nu, nv, pruned full
+ * entries */
+int32_t capsgpu_list_sizes(capsgpu_ctx* ctx, int64_t out[8]);
+/* the half list of the last build, as nbnxm::List holds it (core/src/nbnxm.hpp:29-48); any pointer may be NULL */
+int32_t capsgpu_list_get(capsgpu_ctx* ctx, uint32_t* perm, uint32_t* iperm, int64_t* first, uint32_t* cj, uint16_t* code,
+                         uint16_t* mask, uint16_t* ex, uint16_t* ex14, double* org, double* off, int32_t* ioff,
+                         int32_t* qint, double* qvec);
+/* the slot coordinates of the last evaluation (List::positions) and the pruned masks per outer entry (List::prune) */
+int32_t capsgpu_list_pruned(capsgpu_ctx* ctx, float* X, float* Y, float* Z, uint16_t* pmask, uint16_t* pex,
+                            uint16_t* pex14);
+/* per-stage device times of the last capsgpu_compute, ms: list build, prune, pair kernel, PME, bonded, rest */
+int32_t capsgpu_stage_times(capsgpu_ctx* ctx, double out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
