@@ -87,6 +87,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libcapsgpu.so is not built: run `python -m caps_amd.build` (hipcc, gfx950)")
+        # one HIP runtime per process: when PyTorch is installed its libamdhip64 is loaded first, so the library binds
+        # to the same runtime as the process's tensors and streams rather than bringing a second copy
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.capsgpu_last_error.restype = C.c_char_p
         L.capsgpu_build_error.restype = C.c_char_p
@@ -180,7 +186,8 @@ class EnergyOptions:
     pme_spacing: float = 1.0
     pme_order: int = 5
     tail: bool = True
-    parts: int = 3
+    parts: int = 3                # 1 bonded, 2 non-bonded (pairs, k-space, self, tail), 3 both; 4 PME's reciprocal part
+                                  # with its self and background terms alone (the slow force of learned r-RESPA)
     no_reciprocal: bool = False   # checks only: the real-space part of PME alone
 
 

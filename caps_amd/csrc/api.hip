@@ -61,9 +61,26 @@ struct Sums {
   double pair[11], slot[9], rec[7], bond[9];
 };
 
-void assemble(const capsgpu_ctx& c, const Sums& S, bool nonb, bool bonded, double* t) {
+// nonb: the pair part (pairs, DSF self, tail); recip: PME's reciprocal part with the Ewald self and background terms
+void assemble(const capsgpu_ctx& c, const Sums& S, bool nonb, bool recip, bool bonded, double* t) {
   for (int k = 0; k < CAPSGPU_NTERMS; ++k) t[k] = 0;
   double w[6] = {0, 0, 0, 0, 0, 0}, vir = 0;
+  double q2 = 0, qt = 0;
+  for (double q : c.charge) q2 += q * q, qt += q;
+  const double rc = c.opt.cutoff, V = cell_volume(c.cell);
+  if (recip) {
+    t[CAPSGPU_T_COULOMB] += S.rec[0];
+    vir += S.rec[1] + S.rec[2] + S.rec[3];
+    for (int q = 0; q < 6; ++q) w[q] += S.rec[1 + q];
+    const double beta = c.pme.beta;
+    t[CAPSGPU_T_COULOMB] -= kCoulomb * beta / std::sqrt(kPi) * q2;
+    if (std::fabs(qt) > 1e-10) {
+      const double eb = -kCoulomb * kPi * qt * qt / (2 * V * beta * beta);
+      t[CAPSGPU_T_COULOMB] += eb;
+      vir += 3 * eb;
+      for (int q = 0; q < 3; ++q) w[q] += eb;
+    }
+  }
   if (bonded) {
     t[CAPSGPU_T_BOND] = S.bond[0];
     t[CAPSGPU_T_ANGLE] = S.bond[1];
@@ -72,8 +89,8 @@ void assemble(const capsgpu_ctx& c, const Sums& S, bool nonb, bool bonded, doubl
   }
   if (nonb) {
     // the full list counts every pair twice
-    t[CAPSGPU_T_VDW] = 0.5 * S.pair[0];
-    t[CAPSGPU_T_COULOMB] = 0.5 * S.pair[1];
+    t[CAPSGPU_T_VDW] += 0.5 * S.pair[0];
+    t[CAPSGPU_T_COULOMB] += 0.5 * S.pair[1];
     double W[9];
     for (int q = 0; q < 9; ++q) W[q] = S.slot[q];
     const double* A[3] = {c.cell + 3, c.cell + 6, c.cell + 9};
@@ -83,29 +100,10 @@ void assemble(const capsgpu_ctx& c, const Sums& S, bool nonb, bool bonded, doubl
     vir += W[0] + W[4] + W[8];
     w[0] += W[0], w[1] += W[4], w[2] += W[8];
     w[3] += 0.5 * (W[1] + W[3]), w[4] += 0.5 * (W[2] + W[6]), w[5] += 0.5 * (W[5] + W[7]);
-    double q2 = 0, qt = 0;
-    for (double q : c.charge) q2 += q * q, qt += q;
-    const double rc = c.opt.cutoff, V = cell_volume(c.cell);
-    if (c.opt.coulomb && ewald_on(c)) {
-      if (pme_on(c)) {
-        t[CAPSGPU_T_COULOMB] += S.rec[0];
-        vir += S.rec[1] + S.rec[2] + S.rec[3];
-        for (int q = 0; q < 6; ++q) w[q] += S.rec[1 + q];
-        const double beta = c.pme.beta;
-        t[CAPSGPU_T_COULOMB] -= kCoulomb * beta / std::sqrt(kPi) * q2;
-        if (std::fabs(qt) > 1e-10) {
-          const double eb = -kCoulomb * kPi * qt * qt / (2 * V * beta * beta);
-          t[CAPSGPU_T_COULOMB] += eb;
-          vir += 3 * eb;
-          for (int q = 0; q < 3; ++q) w[q] += eb;
-        }
-      }
-    } else if (c.opt.coulomb) {
-      {
-        const double a = c.opt.dsf_alpha, erfc_rc = erfc_as(a * rc);
-        const double f0 = erfc_rc / (rc * rc) + 2 * a / std::sqrt(kPi) * std::exp(-a * a * rc * rc) / rc;
-        t[CAPSGPU_T_COULOMB] -= (erfc_rc / (2 * rc) + f0 * rc / 2 + a / std::sqrt(kPi)) * q2 * kCoulomb;
-      }
+    if (c.opt.coulomb && !ewald_on(c)) {
+      const double a = c.opt.dsf_alpha, erfc_rc = erfc_as(a * rc);
+      const double f0 = erfc_rc / (rc * rc) + 2 * a / std::sqrt(kPi) * std::exp(-a * a * rc * rc) / rc;
+      t[CAPSGPU_T_COULOMB] -= (erfc_rc / (2 * rc) + f0 * rc / 2 + a / std::sqrt(kPi)) * q2 * kCoulomb;
     }
     if (c.opt.tail) {
       const double rc2 = rc * rc, rc3 = rc2 * rc, rc9 = rc3 * rc3 * rc3;
@@ -146,7 +144,7 @@ void ev_record(capsgpu_ctx& c, int k) {
 // rules (field.cpp:1286-1312): rebuilt when an atom moved more than skin/2 since the build (flag 0), pruned at
 // r_c + b (b = min(0.8, skin)) when one moved more than b/2 since the prune (flag 1) and after every build.
 void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known) {
-  const bool nonb = c.opt.parts & 2, bonded = c.opt.parts & 1;
+  const bool nonb = c.opt.parts & 2, bonded = c.opt.parts & 1, recip = pme_on(c) && (c.opt.parts & 6);
   const double b = std::min(0.8, c.opt.skin);
   ev_record(c, 0);
   if (nonb) {
@@ -175,15 +173,15 @@ void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known) {
     }
     ev_record(c, 2);
     pair_forces(c, want);
-    ev_record(c, 3);
-    if (pme_on(c)) pme_forces(c, want);
-    ev_record(c, 4);
   } else {
-    for (int k = 1; k <= 4; ++k) ev_record(c, k);
+    for (int k = 1; k <= 2; ++k) ev_record(c, k);
   }
+  ev_record(c, 3);
+  if (recip) pme_forces(c, want);
+  ev_record(c, 4);
   if (bonded) bonded_forces(c, want);
   ev_record(c, 5);
-  sum_forces(c, nonb, nonb && pme_on(c), bonded, c.f.p);
+  sum_forces(c, nonb, recip, bonded, c.f.p);
   if (want) {
     Sums S{};
     double* out = c.terms.p;
@@ -192,19 +190,19 @@ void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known) {
       c.partials.resize(9 * size_t(blocks_for(c.L.g.ns, 256)));
       slot_virial(c, c.partials.p);
       colsum(c, c.partials.p, blocks_for(c.L.g.ns, 256), 9, out + 11);
-      if (pme_on(c)) {
-        const int mblocks = (c.pme.h1 + 15) / 16;
-        colsum(c, c.pme.epart.p, int64_t(mblocks) * c.pme.k[1], 7, out + 20);
-      }
+    }
+    if (recip) {
+      const int mblocks = (c.pme.h1 + 15) / 16;
+      colsum(c, c.pme.epart.p, int64_t(mblocks) * c.pme.k[1], 7, out + 20);
     }
     if (bonded) colsum(c, c.bonded.epart.p, blocks_for(c.n, 256), 9, out + 27);
     HIPCHECK(hipMemcpyAsync(c.h_terms, out, 36 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
     std::memcpy(S.pair, c.h_terms, sizeof S.pair);
     std::memcpy(S.slot, c.h_terms + 11, sizeof S.slot);
-    if (nonb && pme_on(c)) std::memcpy(S.rec, c.h_terms + 20, sizeof S.rec);
+    if (recip) std::memcpy(S.rec, c.h_terms + 20, sizeof S.rec);
     if (bonded) std::memcpy(S.bond, c.h_terms + 27, sizeof S.bond);
-    assemble(c, S, nonb, bonded, t);
+    assemble(c, S, nonb, recip, bonded, t);
   }
   ev_record(c, 6);
 }
@@ -237,14 +235,16 @@ int32_t capsgpu_create(const capsgpu_ff* ff, const capsgpu_energy_opts* o, capsg
   return guard([&] {
     if (!ff || !o || !out) throw Error(CAPSGPU_EINVAL, "null argument");
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(CAPSGPU_EDEVICE, "no GPU visible to HIP");
     const int n = ff->n_atoms, nt = ff->n_types;
     if (n < 1 || nt < 1) throw Error(CAPSGPU_EINVAL, "the force field needs atoms and types");
     if (!ff->type || !ff->charge || !ff->mass || !ff->eps || !ff->sigma || !ff->excl_offset)
       throw Error(CAPSGPU_EINVAL, "the force field lacks types, charges, masses, LJ tables or exclusions");
     if (o->cutoff <= 0 || o->skin < 0) throw Error(CAPSGPU_EINVAL, "the cut-off must be positive and the skin not negative");
-    if (o->parts < 1 || o->parts > 3) throw Error(CAPSGPU_EINVAL, "parts must be 1, 2 or 3");
+    if (o->parts < 1 || o->parts > 7) throw Error(CAPSGPU_EINVAL, "parts must be 1 … 7 (1 bonded, 2 non-bonded, 4 reciprocal space)");
+    if ((o->parts & 4) && !(o->coulomb && o->electrostatics == 1 && !o->no_reciprocal))
+      throw Error(CAPSGPU_EINVAL, "the reciprocal-space part needs PME electrostatics");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(CAPSGPU_EDEVICE, "no GPU visible to HIP");
     auto c = std::make_unique<capsgpu_ctx>();
     c->n = n, c->nt = nt, c->opt = *o;
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

@@ -45,7 +45,8 @@ struct DevBuf {
 #ifdef CAPSGPU_DEBUG_GUARD
     if (p) guard_free(p);
 #else
-    if (p) (void)hipFree(p);
+    // work queued on a non-blocking stream may still use the memory: wait for the device before giving it back
+    if (p) (void)hipDeviceSynchronize(), (void)hipFree(p);
 #endif
     p = nullptr, n = cap = 0;
   }

@@ -68,7 +68,9 @@ typedef struct capsgpu_energy_opts {
   double pme_spacing;           /* Å */
   int32_t pme_order;            /* 3 … 8 */
   int32_t tail;                 /* LJ tail correction (else LJ shifted to zero at the cut-off) */
-  int32_t parts;                /* 1 bonded, 2 non-bonded (pairs, k-space, self, tail), 3 both */
+  int32_t parts;                /* bit 1 bonded, bit 2 non-bonded (pairs, k-space, self, tail), 3 both (field.hpp:301-303);
+                                   bit 4 (no reference counterpart): PME's reciprocal part with the Ewald self and
+                                   background terms, without the pair list (the slow force of learned r-RESPA) */
   int32_t no_reciprocal;        /* 1: PME's reciprocal part, self and background terms left out (the real-space part
                                    alone, for checks); 0 the evaluator as the reference's */
 } capsgpu_energy_opts;

@@ -137,3 +137,40 @@ def oracle_total(system, opts):
     out["w"] = w
     out["virial"] = w[:3].sum()
     return out, f
+
+
+def melt_case(chains=16, seed=11):
+    """the all-atom melt (128-atom chains, 1.0 g/cm³) of the 32,768- and 262,144-atom workloads, at a small size"""
+    return ca.build_melt(chains, seed=seed)
+
+
+def ref_pairs(system, rc, skin, coul, kappa=0.0, shift=False, dsf=(0.0, 0.0), prune=True, threads=4):
+    """the reference's cluster path (oracle/_ref: List::build, positions, prune, kernel, pair and shift virial):
+    (vdw, coulomb), forces (N, 3), w[6]"""
+    ff = system.ff
+    R = O.ref()
+    c12, c6 = lj_tables(ff)
+    f12, f6 = np.sqrt(np.diag(c12)).copy(), np.sqrt(np.diag(c6)).copy()
+    x = np.ascontiguousarray(system.x, np.float64).reshape(-1)
+    h = R.ref_eval_create(x, ff.n, np.ascontiguousarray(system.cell, np.float64), np.ascontiguousarray(ff.type, np.int32), ff.ntypes,
+                          f12, f6, np.ascontiguousarray(ff.charge, np.float64), 2 if ff.lj96 else 1, coul, rc, skin, kappa, dsf[0],
+                          dsf[1], int(shift), O._opt(ff.excl_offset, np.int32), O._opt(ff.excl_index, np.int32),
+                          O._opt(ff.p14_offset, np.int32), O._opt(ff.p14_index, np.int32), ff.coul14, 0, None, threads)
+    if not h:
+        raise RuntimeError(R.ref_last_error().decode())
+    try:
+        f, e, w = np.zeros(x.size), np.zeros(2), np.zeros(6)
+        if R.ref_eval_compute(h, x, 1, int(prune), f, e, w) != 0:
+            raise RuntimeError(R.ref_last_error().decode())
+    finally:
+        R.ref_eval_free(h)
+    return e, f.reshape(-1, 3), w
+
+
+def dsf_constants(alpha, rc):
+    """the DSF shift constants e0 = erfc(αrc)/rc, f0 = erfc(αrc)/rc² + 2α/√π e^{−α²rc²}/rc with A&S 7.1.26 erfc, as the
+    evaluator computes them (field.cpp:585-590)"""
+    P, A = 0.3275911, (0.254829592, -0.284496736, 1.421413741, -1.453152027, 1.061405429)
+    t = 1 / (1 + P * alpha * rc)
+    erfc_rc = t * (A[0] + t * (A[1] + t * (A[2] + t * (A[3] + t * A[4])))) * np.exp(-(alpha * rc) ** 2)
+    return erfc_rc / rc, erfc_rc / rc ** 2 + 2 * alpha / np.sqrt(np.pi) * np.exp(-alpha * alpha * rc * rc) / rc

@@ -1,0 +1,69 @@
+"""GPU dynamics (velocity Verlet + Bussi, Langevin BAOAB, NVE; dynamics.cpp:606-774) on the bonded forces alone, through
+the C ABI: the reference's invariants (tests/test_dynamics.cpp:97-151) and the exact initial velocities of the run's own
+mt19937_64 stream."""
+import numpy as np
+import pytest
+
+import caps_amd as ca
+import helpers as H
+import orcpy as O
+
+pytestmark = pytest.mark.gpu
+BONDED = ca.EnergyOptions(cutoff=10.0, coulomb=False, parts=1, tail=False)
+
+
+def test_initial_velocities_are_the_runs_own_stream():
+    s = H.melt_case(chains=4)
+    print("case initial velocities", flush=True)
+    ca.run_dynamics(s, ca.DynamicsOptions(dt=0.5, steps=0, temperature=300, thermostat="none", seed=17, new_velocities=True,
+                                          energy=BONDED))
+    v = np.zeros(3 * s.ff.n)
+    O.orc().orc_initial_velocities(O.Rng(17).ptr(), np.ascontiguousarray(s.ff.mass), s.ff.n, 300.0, v)
+    assert np.abs(s.v.reshape(-1) - v).max() <= 1e-12 * np.abs(v).max()
+
+
+def test_nve_conserves_the_energy():
+    s = ca.kremer_grest(chains=20, beads=30, density=0.85, seed=3)
+    print("case nve kg bonded", flush=True)
+    r = ca.run_dynamics(s, ca.DynamicsOptions(dt=0.25, steps=400, temperature=200, thermostat="none", seed=5, thermo_every=20,
+                                              new_velocities=True, energy=BONDED))
+    # test_dynamics.cpp:97-118: after the first rows the total stays in a band of 1 % of ⟨K⟩ and does not drift
+    E = np.array([row["total"] for row in r.thermo])[5:]
+    K = np.array([row["kinetic"] for row in r.thermo])[5:].mean()
+    assert E.max() - E.min() < 0.01 * K
+    q = len(E) // 5
+    assert abs(E[-q:].mean() - E[:q].mean()) < 0.005 * K
+
+
+def test_bussi_conserved_quantity():
+    s = H.melt_case(chains=4)
+    print("case bussi melt bonded", flush=True)
+    r = ca.run_dynamics(s, ca.DynamicsOptions(dt=0.25, steps=400, temperature=300, thermostat="bussi", tau_t=50, seed=9,
+                                              thermo_every=20, new_velocities=True, energy=BONDED))
+    # test_dynamics.cpp:138-151
+    C = np.array([row["conserved"] for row in r.thermo])[5:]
+    K = np.array([row["kinetic"] for row in r.thermo])[5:].mean()
+    assert C.max() - C.min() < 0.01 * K
+
+
+def test_langevin_is_reproducible_and_holds_the_temperature():
+    print("case langevin melt bonded", flush=True)
+    out = []
+    for _ in range(2):
+        s = H.melt_case(chains=4)
+        r = ca.run_dynamics(s, ca.DynamicsOptions(dt=0.5, steps=2000, temperature=300, thermostat="langevin", tau_t=100, seed=4,
+                                                  thermo_every=20, new_velocities=True, energy=BONDED))
+        out.append((s.x.copy(), s.v.copy(), r))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    T = np.array([row["temperature"] for row in out[0][2].thermo])
+    assert abs(T[len(T) // 2:].mean() - 300) < 30
+
+
+def test_state_round_trip():
+    s = H.melt_case(chains=2)
+    print("case state round trip", flush=True)
+    ev = ca.Evaluator(s.ff, BONDED)
+    v = np.random.default_rng(1).normal(0, 0.01, s.x.shape)
+    ev.set_state(s.x, v, s.cell)
+    x2, v2, c2 = ev.get_state()
+    assert np.array_equal(x2, s.x) and np.array_equal(v2, v) and np.array_equal(c2, s.cell)

