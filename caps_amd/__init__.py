@@ -109,6 +109,8 @@ def lib():
         L.capsgpu_list_get.argtypes = [C.c_void_p] + [C.c_void_p] * 13
         L.capsgpu_list_pruned.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         L.capsgpu_stage_times.argtypes = [C.c_void_p, C.c_void_p]
+        L.capsgpu_profile.argtypes = [C.c_void_p, C.c_int32]
+        L.capsgpu_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.capsgpu_build_melt.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_build_kg.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_system_free.argtypes = [C.POINTER(Sys)]
@@ -290,6 +292,16 @@ class Evaluator:
         t = np.zeros(6)
         _check(lib().capsgpu_stage_times(self._h, _p(t)))
         return dict(zip(("list_build", "prune", "pairs", "pme", "bonded", "rest"), t))
+
+    def profile(self, on=True):
+        """start (zeroing the sums) or stop the stage profiling of every later evaluation, dynamics included"""
+        _check(lib().capsgpu_profile(self._h, int(on)))
+
+    def profile_read(self):
+        """{stage: (summed device ms, evaluations)} since profiling started"""
+        ms, n = np.zeros(6), np.zeros(6, np.int64)
+        _check(lib().capsgpu_profile_read(self._h, _p(ms), _p(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("list_build", "prune", "pairs", "pme", "bonded", "rest"))}
 
     # inspection of the cluster list (parity checks)
     def list_arrays(self):

@@ -136,8 +136,38 @@ void read_flags(capsgpu_ctx& c) {
   HIPCHECK(hipStreamSynchronize(c.stream));
 }
 
+// the summed stage times of the recorded evaluations (waits for the stream)
+void prof_harvest(capsgpu_ctx& c) {
+  if (!c.pev_used) return;
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  for (size_t q = 0; q < c.pev_used; ++q)
+    for (int k = 0; k < 6; ++k) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, c.pev[q][size_t(k)], c.pev[q][size_t(k) + 1]));
+      c.prof_ms[k] += ms;
+      ++c.prof_n[k];
+    }
+  c.pev_used = 0;
+}
+
+// a set of events for the next profiled evaluation
+void prof_begin(capsgpu_ctx& c) {
+  if (!c.profile || c.timing) return;
+  if (c.pev_used == c.pev.size()) {
+    if (c.pev.size() >= 1024) {
+      prof_harvest(c);
+    } else {
+      std::array<hipEvent_t, 7> e{};
+      for (auto& x : e) HIPCHECK(hipEventCreate(&x));
+      c.pev.push_back(e);
+    }
+  }
+  c.pslot = c.pev_used++;
+}
+
 void ev_record(capsgpu_ctx& c, int k) {
   if (c.timing) HIPCHECK(hipEventRecord(c.ev[k], c.stream));
+  else if (c.profile) HIPCHECK(hipEventRecord(c.pev[c.pslot][size_t(k)], c.stream));
 }
 
 // forces at the device positions c.x into c.f; the energy terms into t when want. The list follows the reference's
@@ -146,6 +176,7 @@ void ev_record(capsgpu_ctx& c, int k) {
 void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known) {
   const bool nonb = c.opt.parts & 2, bonded = c.opt.parts & 1, recip = pme_on(c) && (c.opt.parts & 6);
   const double b = std::min(0.8, c.opt.skin);
+  prof_begin(c);
   ev_record(c, 0);
   if (nonb) {
     if (!flags_known && c.L.built) {
@@ -224,6 +255,8 @@ capsgpu_ctx::~capsgpu_ctx() {
   if (h_terms) (void)hipHostFree(h_terms);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& set : pev)
+    for (auto& e : set) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -408,6 +441,23 @@ int32_t capsgpu_stage_times(capsgpu_ctx* c, double out[6]) {
   return guard([&] {
     if (!c || !out) throw Error(CAPSGPU_EINVAL, "null argument");
     for (int k = 0; k < 6; ++k) out[k] = c->stage_ms[k];
+  });
+}
+
+int32_t capsgpu_profile(capsgpu_ctx* c, int32_t on) {
+  return guard([&] {
+    if (!c) throw Error(CAPSGPU_EINVAL, "null context");
+    prof_harvest(*c);
+    c->profile = on != 0;
+    for (int k = 0; k < 6; ++k) c->prof_ms[k] = 0, c->prof_n[k] = 0;
+  });
+}
+
+int32_t capsgpu_profile_read(capsgpu_ctx* c, double ms[6], int64_t counts[6]) {
+  return guard([&] {
+    if (!c || !ms || !counts) throw Error(CAPSGPU_EINVAL, "null argument");
+    prof_harvest(*c);
+    for (int k = 0; k < 6; ++k) ms[k] = c->prof_ms[k], counts[k] = c->prof_n[k];
   });
 }
 

@@ -143,6 +143,12 @@ struct capsgpu_ctx {
   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
   bool timing = false;
   hipEvent_t ev[8] = {};
+  // stage profiling of every evaluation (dynamics included): event sets recorded on the stream, summed when harvested
+  bool profile = false;
+  std::vector<std::array<hipEvent_t, 7>> pev;
+  size_t pev_used = 0, pslot = 0;
+  double prof_ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t prof_n[6] = {0, 0, 0, 0, 0, 0};
   ~capsgpu_ctx();
 };
 

@@ -163,6 +163,11 @@ int32_t capsgpu_list_pruned(capsgpu_ctx* ctx, float* X, float* Y, float* Z, uint
                             uint16_t* pex14);
 /* per-stage device times of the last capsgpu_compute, ms: list build, prune, pair kernel, PME, bonded, rest */
 int32_t capsgpu_stage_times(capsgpu_ctx* ctx, double out[6]);
+/* stage profiling of every later evaluation, dynamics included (events on the context's stream): on = 1 starts it and
+ * zeroes the sums, 0 stops it. read: device time summed per stage (ms, the stages of capsgpu_stage_times) and the number
+ * of evaluations summed, since profiling started */
+int32_t capsgpu_profile(capsgpu_ctx* ctx, int32_t on);
+int32_t capsgpu_profile_read(capsgpu_ctx* ctx, double ms[6], int64_t counts[6]);
 
 #ifdef __cplusplus
 }
