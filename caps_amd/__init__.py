@@ -111,6 +111,7 @@ def lib():
         L.capsgpu_stage_times.argtypes = [C.c_void_p, C.c_void_p]
         L.capsgpu_profile.argtypes = [C.c_void_p, C.c_int32]
         L.capsgpu_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.capsgpu_profile_kernels.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
         L.capsgpu_build_melt.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_build_kg.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_system_free.argtypes = [C.POINTER(Sys)]
@@ -302,6 +303,16 @@ class Evaluator:
         ms, n = np.zeros(6), np.zeros(6, np.int64)
         _check(lib().capsgpu_profile_read(self._h, _p(ms), _p(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("list_build", "prune", "pairs", "pme", "bonded", "rest"))}
+
+    def profile_kernels(self):
+        """{kernel: (summed device ms, launches)} since profiling started"""
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().capsgpu_profile_kernels(self._h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, n = line.split()
+            out[name] = (float(ms), int(n))
+        return out
 
     # inspection of the cluster list (parity checks)
     def list_arrays(self):

@@ -3,6 +3,8 @@
 
 #include <array>
 #include <memory>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/capsgpu.h"
@@ -149,6 +151,14 @@ struct capsgpu_ctx {
   size_t pev_used = 0, pslot = 0;
   double prof_ms[6] = {0, 0, 0, 0, 0, 0};
   int64_t prof_n[6] = {0, 0, 0, 0, 0, 0};
+  // per-kernel launch timing while profiling: event pairs around each launch, summed per kernel name when harvested
+  struct KTime {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<KTime> kev;
+  size_t kev_used = 0;
+  std::vector<std::pair<std::string, std::pair<double, int64_t>>> ktot;
   ~capsgpu_ctx();
 };
 

@@ -9,6 +9,7 @@
 // Virial: Σ d⊗f over pairs = Σ_slots x⊗F + ½ Σ_full S⊗f_(j←i), S = qa·a + qb·b + qc·c, kept as three vectors per
 // lattice coefficient (9 floats per lane); the slot part is added in double by k_slot_virial.
 #include "context.hpp"
+#include "prof.hpp"
 
 namespace capsgpu {
 namespace {
@@ -257,9 +258,11 @@ void pair_forces(capsgpu_ctx& c, bool want) {
   P.epart = L.epart.p;
   const unsigned grid = unsigned((ncl + 3) / 4);
   if (grid == 0) return;
+  kt_begin(c, "k_pairs");
   if (c.lj96) launch_coul<2>(P, coul, want, c.stream, grid);
   else launch_coul<1>(P, coul, want, c.stream, grid);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 }  // namespace capsgpu

@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "context.hpp"
+#include "prof.hpp"
 
 namespace capsgpu {
 namespace {
@@ -580,8 +581,10 @@ void pme_setup(capsgpu_ctx& c) {
   P.gtab.resize(nh);
   const GridArgs G = grid_args(P);
   const double fac = kPi * kPi / (P.beta * P.beta);
+  kt_begin(c, "k_green");
   hipLaunchKernelGGL(k_green, dim3(blocks_for(nh, 256)), dim3(256), 0, s, G, fac, kCoulomb, P.vol, P.bmod.p, P.gtab.p);
   LAUNCHCHECK();
+  kt_end(c);
   P.grid.resize(size_t(P.k[0]) * P.k[1] * P.k[2]);
   P.half.resize(nh);
   for (int q = 0; q < 12; ++q) P.cell[q] = cell[q];
@@ -602,29 +605,42 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   P.tcount.zero(s);
   float4* wq = reinterpret_cast<float4*>(P.theta.p);
   int4* base = reinterpret_cast<int4*>(P.base.p);
+  kt_begin(c, "k_splines");
   hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p);
   LAUNCHCHECK();
+  kt_end(c);
   P.tstart.resize(size_t(ntiles) + 1);
   c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
   P.tfill.resize(size_t(ntiles));
   P.tfill.zero(s);
   P.tatoms.resize(size_t(n));
+  kt_begin(c, "k_tile_fill");
   hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tfill.p, P.tatoms.p);
   LAUNCHCHECK();
+  kt_end(c);
+  kt_begin(c, "k_tile_sort");
   hipLaunchKernelGGL(k_tile_sort, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, ntiles, P.tstart.p, P.tatoms.p);
   LAUNCHCHECK();
+  kt_end(c);
+  kt_begin(c, "k_spread");
   hipLaunchKernelGGL(k_spread, dim3(blocks_for(ntiles, 4)), dim3(256), 0, s, G, P.tstart.p, P.tatoms.p, wq, base, P.grid.p);
   LAUNCHCHECK();
+  kt_end(c);
   const int npairs = (P.k[1] * P.k[2] + 1) / 2;
   const size_t lds_x = 2 * 4 * size_t(P.k[0]) * sizeof(float2);
+  kt_begin(c, "k_fft_x_r2c");
   hipLaunchKernelGGL(k_fft_x_r2c, dim3(unsigned((npairs + 3) / 4)), dim3(256), lds_x, s, G, F, P.tw.p, P.grid.p, P.half.p);
   LAUNCHCHECK();
+  kt_end(c);
   const int mblocks = (P.h1 + kLinesYZ - 1) / kLinesYZ;
   const size_t lds_y = 2 * kLinesYZ * size_t(P.k[1]) * sizeof(float2), lds_z = 2 * kLinesYZ * size_t(P.k[2]) * sizeof(float2);
+  kt_begin(c, "k_fft_y");
   hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, +1);
   LAUNCHCHECK();
+  kt_end(c);
   const unsigned zb = unsigned(mblocks * P.k[1]);
   const double fac = kPi * kPi / (P.beta * P.beta);
+  kt_begin(c, "k_fft_z_conv");
   if (want) {
     P.epart.resize(7 * size_t(zb));
     hipLaunchKernelGGL(k_fft_z_conv<true>, dim3(zb), dim3(256), lds_z, s, G, F, P.tw.p, P.half.p, P.gtab.p, fac, kCoulomb, P.vol,
@@ -634,13 +650,20 @@ void pme_forces(capsgpu_ctx& c, bool want) {
                        P.bmod.p, nullptr);
   }
   LAUNCHCHECK();
+  kt_end(c);
+  kt_begin(c, "k_fft_y");
   hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, -1);
   LAUNCHCHECK();
+  kt_end(c);
+  kt_begin(c, "k_fft_x_c2r");
   hipLaunchKernelGGL(k_fft_x_c2r, dim3(unsigned((npairs + 3) / 4)), dim3(256), lds_x, s, G, F, P.tw.p, P.half.p, P.grid.p);
   LAUNCHCHECK();
+  kt_end(c);
   P.fpme.resize(3 * size_t(n));
+  kt_begin(c, "k_gather");
   hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, wq, base, P.grid.p, P.fpme.p);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 }  // namespace capsgpu

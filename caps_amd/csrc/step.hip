@@ -12,6 +12,7 @@
 #include <cmath>
 
 #include "context.hpp"
+#include "prof.hpp"
 #include "step.hpp"
 
 namespace capsgpu {
@@ -435,6 +436,7 @@ void bonded_forces(capsgpu_ctx& c, bool want) {
   const int n = c.n;
   const CellArgs C = cell_args(c);
   const unsigned nb = blocks_for(n, 256);
+  kt_begin(c, "k_bonded");
   if (want) {
     B.epart.resize(kEB * size_t(nb));
     hipLaunchKernelGGL(k_bonded<true>, dim3(nb), dim3(256), 0, c.stream, n, C, c.x.p, B.aoff.p, B.aterm.p, B.bij.p, B.bkr.p, B.aijk.p,
@@ -444,66 +446,88 @@ void bonded_forces(capsgpu_ctx& c, bool want) {
                        B.akt.p, B.fij.p, B.fp.p, B.f.p, nullptr, c.flags.p);
   }
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void sum_forces(capsgpu_ctx& c, bool nb, bool rec, bool bond, double* fdst) {
+  kt_begin(c, "k_sum_forces");
   hipLaunchKernelGGL(k_sum_forces, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, c.L.iperm.p, nb ? c.L.fnb.p : nullptr,
                      rec ? c.pme.fpme.p : nullptr, bond ? c.bonded.f.p : nullptr, fdst);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void slot_virial(capsgpu_ctx& c, double* part) {
   const int64_t ns = c.L.g.ns;
+  kt_begin(c, "k_slot_virial");
   hipLaunchKernelGGL(k_slot_virial, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, c.L.xq.p, c.L.org.p, c.L.fnb.p, part);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void colsum(capsgpu_ctx& c, const double* part, int64_t rows, int cols, double* out) {
+  kt_begin(c, "k_colsum");
   hipLaunchKernelGGL(k_colsum, dim3(unsigned(cols)), dim3(256), 0, c.stream, part, rows, cols, out);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void check_moves(capsgpu_ctx& c, double lim_build, double lim_prune) {
+  kt_begin(c, "k_check_moves");
   hipLaunchKernelGGL(k_check_moves, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, c.x.p, c.x0.p, c.x0p.p, lim_build, lim_prune,
                      c.flags.p);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
-void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p) {
-  hipLaunchKernelGGL(k_vv_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, dt, c.d_minv.p, alpha, c.f.p, c.x.p, c.v.p,
-                     c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p);
+void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p, const double* f) {
+  kt_begin(c, "k_vv_open");
+  hipLaunchKernelGGL(k_vv_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, dt, c.d_minv.p, alpha, f ? f : c.f.p, c.x.p,
+                     c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void baoab_open(capsgpu_ctx& c, double hk, double hdt, double c1, double c2, const double* sdv, uint64_t seed, uint64_t step, double lim_b,
                 double lim_p, double* part) {
+  kt_begin(c, "k_baoab_open");
   hipLaunchKernelGGL(k_baoab_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, hdt, c1, c2, c.d_minv.p, sdv, seed, step,
                      c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
-void kick(capsgpu_ctx& c, double hk, bool ke, double* part) {
+void kick(capsgpu_ctx& c, double hk, bool ke, double* part, const double* f) {
+  if (!f) f = c.f.p;
+  kt_begin(c, "k_kick");
   if (ke)
-    hipLaunchKernelGGL(k_kick<true>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, c.f.p, c.v.p, part);
+    hipLaunchKernelGGL(k_kick<true>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, f, c.v.p, part);
   else
-    hipLaunchKernelGGL(k_kick<false>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, c.f.p, c.v.p, part);
+    hipLaunchKernelGGL(k_kick<false>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, f, c.v.p, part);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void bussi(capsgpu_ctx& c, const double* part, double kt_target, double ndof, double cc, double r1, double sum, double* alpha, double* bath) {
+  kt_begin(c, "k_bussi");
   hipLaunchKernelGGL(k_bussi, dim3(1), dim3(256), 0, c.stream, part, int64_t(blocks_for(c.n, 256)), kt_target, ndof, cc, r1, sum, alpha,
                      bath);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void scale_v(capsgpu_ctx& c, const double* alpha) {
+  kt_begin(c, "k_scale_v");
   hipLaunchKernelGGL(k_scale_v, dim3(blocks_for(3 * size_t(c.n), 256)), dim3(256), 0, c.stream, c.n, alpha, c.v.p);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void bath_add(capsgpu_ctx& c, const double* part, double* bath) {
+  kt_begin(c, "k_add_scalar");
   hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(256), 0, c.stream, part, int64_t(blocks_for(c.n, 256)), bath);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 }  // namespace capsgpu

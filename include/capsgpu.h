@@ -168,6 +168,9 @@ int32_t capsgpu_stage_times(capsgpu_ctx* ctx, double out[6]);
  * of evaluations summed, since profiling started */
 int32_t capsgpu_profile(capsgpu_ctx* ctx, int32_t on);
 int32_t capsgpu_profile_read(capsgpu_ctx* ctx, double ms[6], int64_t counts[6]);
+/* per-kernel launch times since profiling started, as text: one "name summed_ms launches" line per kernel, truncated
+ * to cap − 1 characters and NUL-terminated */
+int32_t capsgpu_profile_kernels(capsgpu_ctx* ctx, char* buf, int32_t cap);
 
 #ifdef __cplusplus
 }

@@ -67,3 +67,35 @@ def test_state_round_trip():
     ev.set_state(s.x, v, s.cell)
     x2, v2, c2 = ev.get_state()
     assert np.array_equal(x2, s.x) and np.array_equal(v2, v) and np.array_equal(c2, s.cell)
+
+
+def _band(respa, dt, steps):
+    s = H.melt_case(chains=4)
+    op = ca.EnergyOptions(cutoff=8.0, coulomb=True, electrostatics="pme", pme_order=4, pme_spacing=1.0, tail=False, parts=5)
+    r = ca.run_dynamics(s, ca.DynamicsOptions(dt=dt, steps=steps, respa=respa, temperature=300, thermostat="none", seed=7,
+                                              thermo_every=5, new_velocities=True, energy=op))
+    E = np.array([row["total"] for row in r.thermo])[4:]
+    K = np.array([row["kinetic"] for row in r.thermo])[4:].mean()
+    return E.max() - E.min(), K
+
+
+def test_respa_conserves_energy_with_long_outer_steps():
+    # test_dynamics.cpp:248-273, with PME's reciprocal part as the slow force and the bonded terms as the fast one: bonded
+    # forces at 0.5 fs inside 2 fs steps conserve energy as plain 0.5 fs steps do, far better than plain 2 fs steps
+    print("case r-RESPA bands", flush=True)
+    plain_fine, kin = _band(1, 0.5, 1200)
+    respa, kin2 = _band(4, 2.0, 300)
+    plain_coarse, _ = _band(1, 2.0, 300)
+    print("bands: plain 0.5 fs %.4g, r-RESPA 4 x 0.5 fs %.4g, plain 2 fs %.4g (kinetic %.4g)" % (plain_fine, respa, plain_coarse, kin2), flush=True)
+    assert respa < 0.02 * kin2
+    assert respa < 0.5 * plain_coarse
+    assert plain_fine < 0.01 * kin
+
+
+def test_respa_needs_bussi_or_nve_and_both_parts():
+    s = H.melt_case(chains=2)
+    op = ca.EnergyOptions(cutoff=8.0, coulomb=True, electrostatics="pme", pme_order=4, tail=False, parts=5)
+    with pytest.raises(ValueError):
+        ca.run_dynamics(s, ca.DynamicsOptions(dt=1.0, steps=2, respa=2, thermostat="langevin", energy=op))
+    with pytest.raises(ValueError):
+        ca.run_dynamics(s, ca.DynamicsOptions(dt=1.0, steps=2, respa=2, thermostat="none", energy=BONDED))
