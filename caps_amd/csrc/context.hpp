@@ -91,6 +91,7 @@ struct Pme {
   DevBuf<double> bmod;                  // |b(m)|² per axis, K1 + K2 + K3
   DevBuf<float> theta;                  // per atom: p weights and p derivatives per axis (6p floats)
   DevBuf<int32_t> base;                 // per atom: the first grid index per axis (3 ints)
+  DevBuf<float> wts;                    // per atom: θ and dθ of the three axes (6p floats)
   DevBuf<uint32_t> tile, tcount, tfill, tatoms;
   DevBuf<int64_t> tstart;
   DevBuf<float> fpme;                   // 3 per atom

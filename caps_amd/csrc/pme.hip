@@ -2,10 +2,10 @@
 // core/src/kspace.cpp:80-175 and core/fortran/caps_kspace.f90 (splines :776-796, spread :800-822, r2c/c2r lines
 // :392-461, FFT lines :348-387, influence function :518-564 / :880-928, gather :931-961).
 //
-//  k_splines      per atom: fractional grid coordinate, first grid point and B-spline offset w; its home tile
+//  k_splines      per atom: fractional grid coordinate, first grid point, B-spline weights and derivatives; home tile
 //  k_tile_*       atoms binned by the 4x4x4 grid tile of their first point (sorted by atom in each tile)
-//  k_spread       one wavefront per tile, one grid point per lane: each point sums q·θa·θb·θc over the atoms whose
-//                 support covers it (every point written once, no atomics, a fixed order)
+//  k_spread       one 64-lane workgroup per tile, one grid point per lane: each point sums q·θa·θb·θc over the atoms
+//                 whose support covers it, staged in LDS (every point written once, no atomics, a fixed order)
 //  k_fft_x_r2c    two real lines per complex transform along a (as the reference), the half spectrum written
 //  k_fft_y        complex lines along b, 16 lines of consecutive a-frequencies per workgroup (coalesced)
 //  k_fft_z_conv   lines along c: forward transform, × influence function (energy and virial on energy steps),
@@ -54,7 +54,7 @@ struct GridArgs {
 
 // fractional coordinate s = h⁻¹ x, u = (s − ⌊s⌋) K, the first point ⌊u⌋ − p + 1 (mod K) and w = u − ⌊u⌋
 __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
-                          int4* __restrict__ base, uint32_t* __restrict__ tcount) {
+                          int4* __restrict__ base, uint32_t* __restrict__ tcount, float* __restrict__ wts) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float w[3];
@@ -72,6 +72,13 @@ __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const
   wq[i] = make_float4(w[0], w[1], w[2], float(q[i]));
   base[i] = make_int4(b[0], b[1], b[2], tile);
   atomicAdd(&tcount[tile], 1u);
+  // the weights θ and derivatives dθ of the three axes, 6p floats per atom, for the spread and the gather
+  float* o = wts + size_t(i) * 6 * G.p;
+  for (int d = 0; d < 3; ++d) {
+    float th[kMaxOrder], dth[kMaxOrder];
+    bspline(w[d], G.p, th, dth);
+    for (int j = 0; j < G.p; ++j) o[d * G.p + j] = th[j], o[3 * G.p + d * G.p + j] = dth[j];
+  }
 }
 
 __global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
@@ -94,13 +101,6 @@ __global__ void k_tile_sort(int ntiles, const int64_t* __restrict__ tstart, uint
   }
 }
 
-__device__ inline float pick(const float* th, int j, int p) {
-  float r = 0.0f;
-#pragma unroll
-  for (int k = 0; k < kMaxOrder; ++k) r = (k == j && k < p) ? th[k] : r;
-  return r;
-}
-
 // distinct home tiles (along one axis) of the first points that reach points [4T, 4T + 3]
 __device__ inline int home_tiles(int T, int K, int p, int* out) {
   int m = 0;
@@ -115,41 +115,67 @@ __device__ inline int home_tiles(int T, int K, int p, int* out) {
   return m;
 }
 
-__global__ void __launch_bounds__(256) k_spread(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
-                                                const float4* __restrict__ wq, const int4* __restrict__ base, float* __restrict__ grid) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// one 64-lane workgroup per 4x4x4 tile, one grid point per lane; the atoms whose first point lies in a tile that can
+// reach this one are staged in LDS 64 at a time (their base points, charges and weights), then every lane sums over
+// them in the same fixed order
+constexpr int kSpreadChunk = 64, kMaxHome = 64;   // at most 4 home tiles per axis reach a tile (p <= 8, partial last tile)
+__global__ void __launch_bounds__(64) k_spread(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
+                                               const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ wts,
+                                               float* __restrict__ grid) {
+  __shared__ float s_w[kSpreadChunk][3 * kMaxOrder];
+  __shared__ int s_b[kSpreadChunk][3];
+  __shared__ float s_q[kSpreadChunk];
+  __shared__ int64_t s_hb[kMaxHome], s_pre[kMaxHome + 1];
+  __shared__ int s_nh;
+  const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
-  const int tile = blockIdx.x * 4 + wave;
+  const int tile = blockIdx.x;
   if (tile >= ntiles) return;
+  const int p = G.p;
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
   const int gx = 4 * tx + (lane & 3), gy = 4 * ty + ((lane >> 2) & 3), gz = 4 * tz + (lane >> 4);
-  int hx[8], hy[8], hz[8];
-  const int mx = home_tiles(tx, G.k[0], G.p, hx), my = home_tiles(ty, G.k[1], G.p, hy), mz = home_tiles(tz, G.k[2], G.p, hz);
-  float acc = 0.0f;
-  for (int iz = 0; iz < mz; ++iz)
-    for (int iy = 0; iy < my; ++iy)
-      for (int ix = 0; ix < mx; ++ix) {
-        const int h = (hz[iz] * G.nt[1] + hy[iy]) * G.nt[0] + hx[ix];
-        for (int64_t k = tstart[h]; k < tstart[h + 1]; ++k) {
-          const uint32_t a = tatoms[k];
-          const float4 w = wq[a];
-          const int4 b = base[a];
-          int dx = gx - b.x, dy = gy - b.y, dz = gz - b.z;
-          dx += dx < 0 ? G.k[0] : 0;
-          dy += dy < 0 ? G.k[1] : 0;
-          dz += dz < 0 ? G.k[2] : 0;
-          if (dx < G.p && dy < G.p && dz < G.p) {
-            float th[kMaxOrder], dth[kMaxOrder];
-            bspline(w.x, G.p, th, dth);
-            const float ta = pick(th, dx, G.p);
-            bspline(w.y, G.p, th, dth);
-            const float tb = pick(th, dy, G.p);
-            bspline(w.z, G.p, th, dth);
-            const float tc = pick(th, dz, G.p);
-            acc += w.w * tb * tc * ta;
-          }
+  // the candidate tiles in a fixed order and the running count of their atoms
+  if (lane == 0) {
+    int hx[8], hy[8], hz[8];
+    const int mx = home_tiles(tx, G.k[0], p, hx), my = home_tiles(ty, G.k[1], p, hy), mz = home_tiles(tz, G.k[2], p, hz);
+    int nh = 0;
+    s_pre[0] = 0;
+    for (int iz = 0; iz < mz; ++iz)
+      for (int iy = 0; iy < my; ++iy)
+        for (int ix = 0; ix < mx && nh < kMaxHome; ++ix) {
+          const int h = (hz[iz] * G.nt[1] + hy[iy]) * G.nt[0] + hx[ix];
+          s_hb[nh] = tstart[h];
+          s_pre[nh + 1] = s_pre[nh] + (tstart[h + 1] - tstart[h]);
+          ++nh;
         }
-      }
+    s_nh = nh;
+  }
+  __syncthreads();
+  const int64_t m = s_pre[s_nh];
+  float acc = 0.0f;
+  for (int64_t c0 = 0; c0 < m; c0 += kSpreadChunk) {
+    const int64_t f = c0 + lane;
+    if (f < m) {
+      int t = 0;
+      while (s_pre[t + 1] <= f) ++t;
+      const uint32_t a = tatoms[s_hb[t] + (f - s_pre[t])];
+      const int4 b = base[a];
+      s_b[lane][0] = b.x, s_b[lane][1] = b.y, s_b[lane][2] = b.z;
+      s_q[lane] = wq[a].w;
+      const float* w = wts + size_t(a) * 6 * p;
+      for (int k = 0; k < 3 * p; ++k) s_w[lane][k] = w[k];
+    }
+    __syncthreads();
+    const int cnt = int(m - c0 < kSpreadChunk ? m - c0 : kSpreadChunk);
+    for (int jj = 0; jj < cnt; ++jj) {
+      int dx = gx - s_b[jj][0], dy = gy - s_b[jj][1], dz = gz - s_b[jj][2];
+      dx += dx < 0 ? G.k[0] : 0;
+      dy += dy < 0 ? G.k[1] : 0;
+      dz += dz < 0 ? G.k[2] : 0;
+      if (dx < p && dy < p && dz < p) acc += s_q[jj] * s_w[jj][p + dy] * s_w[jj][2 * p + dz] * s_w[jj][dx];
+    }
+    __syncthreads();
+  }
   if (gx < G.k[0] && gy < G.k[1] && gz < G.k[2]) grid[(size_t(gz) * G.k[1] + gy) * G.k[0] + gx] = acc;
 }
 
@@ -431,16 +457,20 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
   }
 }
 
-__global__ void k_gather(GridArgs G, int n, const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ grid,
-                         float* __restrict__ f) {
+__global__ void k_gather(GridArgs G, int n, const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ wts,
+                         const float* __restrict__ grid, float* __restrict__ f) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 w = wq[i];
   const int4 b = base[i];
+  const int p = G.p;
+  const float* o = wts + size_t(i) * 6 * p;
   float tx[kMaxOrder], dx[kMaxOrder], ty[kMaxOrder], dy[kMaxOrder], tz[kMaxOrder], dz[kMaxOrder];
-  bspline(w.x, G.p, tx, dx);
-  bspline(w.y, G.p, ty, dy);
-  bspline(w.z, G.p, tz, dz);
+  for (int j = 0; j < kMaxOrder; ++j) {
+    const bool in = j < p;
+    tx[j] = in ? o[j] : 0.0f, ty[j] = in ? o[p + j] : 0.0f, tz[j] = in ? o[2 * p + j] : 0.0f;
+    dx[j] = in ? o[3 * p + j] : 0.0f, dy[j] = in ? o[4 * p + j] : 0.0f, dz[j] = in ? o[5 * p + j] : 0.0f;
+  }
   float g0 = 0, g1 = 0, g2 = 0;
   for (int c = 0; c < G.p; ++c) {
     int iz = b.z + c;
@@ -606,7 +636,8 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   float4* wq = reinterpret_cast<float4*>(P.theta.p);
   int4* base = reinterpret_cast<int4*>(P.base.p);
   kt_begin(c, "k_splines");
-  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p);
+  P.wts.resize(6 * size_t(G.p) * size_t(n));
+  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p, P.wts.p);
   LAUNCHCHECK();
   kt_end(c);
   P.tstart.resize(size_t(ntiles) + 1);
@@ -623,7 +654,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_spread");
-  hipLaunchKernelGGL(k_spread, dim3(blocks_for(ntiles, 4)), dim3(256), 0, s, G, P.tstart.p, P.tatoms.p, wq, base, P.grid.p);
+  hipLaunchKernelGGL(k_spread, dim3(unsigned(ntiles)), dim3(64), 0, s, G, P.tstart.p, P.tatoms.p, wq, base, P.wts.p, P.grid.p);
   LAUNCHCHECK();
   kt_end(c);
   const int npairs = (P.k[1] * P.k[2] + 1) / 2;
@@ -661,7 +692,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   kt_end(c);
   P.fpme.resize(3 * size_t(n));
   kt_begin(c, "k_gather");
-  hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, wq, base, P.grid.p, P.fpme.p);
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, wq, base, P.wts.p, P.grid.p, P.fpme.p);
   LAUNCHCHECK();
   kt_end(c);
 }
