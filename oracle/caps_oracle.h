@@ -3,10 +3,15 @@
  * A plain-C restatement of the reference algorithms, used by tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg as the checker. Nothing in caps_amd/ links, loads or calls it.
  *
- * Each function names the reference file:line it restates. Every function is pinned by at least one independent check
- * (tests/test_oracle_*.py): the reference build in oracle/_ref (list build, cluster kernel, PME), closed forms
- * (Madelung constant, B-spline identities, DFT), finite differences (bonded forces, virial), invariants (list
- * completeness, momentum, NVE drift), or published known-answer values (mt19937_64's 10000th output).
+ * Each function names the reference file:line it restates and is checked independently (tests/test_oracle_*.py): against
+ * the reference build in oracle/_ref (list build and prune bit-exact, the cluster kernel within nbnxm_check.cpp's bars,
+ * the Fortran PME, ewald_beta, good sizes, grid choice, the rng.hpp Normal/Gamma/UniformReal streams bit-exact), closed
+ * forms (Madelung constant, DFT, the tail integral), finite differences (pair, bonded and reciprocal forces, the virial
+ * as the strain derivative, the MLP gradients), invariants (every pair within r_c listed once, momentum, temperature,
+ * KG bond lengths and angles) and published known-answer values (mt19937_64's 10000th output).
+ * Parity unpinned: the exact draw order of orc_initial_velocities (dynamics.cpp:236-256) and of orc_kremer_grest
+ * (kremer_grest.cpp:15-83) — their distributions' streams are pinned, but the reference functions that consume them
+ * link the whole core and do not build here, so only their invariants are checked.
  *
  * Conventions: a cell is double[12] = origin(3), a(3), b(3), c(3), fully periodic; positions are 3N doubles (Å);
  * forces 3N doubles (kcal/mol/Å); energies kcal/mol; the virial tensor w[6] = xx yy zz xy xz yz (Σ r_a f_b, symmetrised).

@@ -126,6 +126,9 @@ def ref():
         L.ref_eval_compute.argtypes = [C.c_void_p, f64p, C.c_int, C.c_int, f64p, f64p, f64p]
         L.ref_eval_threads.restype = C.c_int
         L.ref_eval_threads.argtypes = [C.c_void_p]
+        L.ref_rng_normal.argtypes = [C.c_uint64, C.c_int, f64p]
+        L.ref_rng_bussi.argtypes = [C.c_uint64, C.c_double, C.c_int, f64p]
+        L.ref_rng_uniform.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_int, f64p]
         _ref = L
     return _ref
 

@@ -15,7 +15,10 @@
 #include <string>
 #include <vector>
 
+#include <random>
+
 #include "caps/kspace.hpp"
+#include "caps/rng.hpp"
 #include "kspace_pool.hpp"
 #include "nbnxm.hpp"
 #include "parallel.hpp"
@@ -265,5 +268,30 @@ int ref_eval_compute(void* h, const double* x, int energies, int prune, double* 
   }
 }
 int ref_eval_threads(void* h) { return static_cast<RefEval*>(h)->pool->size(); }
+
+// ---- the reference's random numbers (core/include/caps/rng.hpp, header-only) -----------------------------------------
+// n draws of caps::Normal<double>(0, 1) from std::mt19937_64(seed), one distribution object (its spare kept)
+void ref_rng_normal(uint64_t seed, int n, double* out) {
+  std::mt19937_64 g(seed);
+  caps::Normal<double> N(0, 1);
+  for (int i = 0; i < n; ++i) out[i] = N(g);
+}
+// the Bussi draw pattern of dynamics.cpp:466-477: per step r1 from the run's normal generator, then Σ R² from a fresh
+// caps::Gamma((ndof − 1)/2, 2) on the same engine; out = (r1, sum) pairs
+void ref_rng_bussi(uint64_t seed, double ndof, int steps, double* out) {
+  std::mt19937_64 g(seed);
+  caps::Normal<double> gauss(0, 1);
+  for (int s = 0; s < steps; ++s) {
+    const double r1 = gauss(g);
+    caps::Gamma<double> chi((ndof - 1) / 2, 2.0);
+    out[2 * s] = r1, out[2 * s + 1] = chi(g);
+  }
+}
+// n draws of caps::UniformReal<double>(a, b)
+void ref_rng_uniform(uint64_t seed, double a, double b, int n, double* out) {
+  std::mt19937_64 g(seed);
+  caps::UniformReal<double> U(a, b);
+  for (int i = 0; i < n; ++i) out[i] = U(g);
+}
 
 }  // extern "C"

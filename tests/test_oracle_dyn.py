@@ -1,6 +1,6 @@
 """The oracle's random numbers, thermostat draw, Kremer–Grest builder and MLP (oracle/orc_dyn.c), pinned by the C++
-standard's mt19937_64 check value, distribution moments, Bussi's limits and mean, tests/test_provenance.cpp's KG
-invariants, and central differences of the MLP loss."""
+standard's mt19937_64 check value, the reference's own rng.hpp streams (compiled into oracle/_ref), distribution moments,
+Bussi's limits and mean, tests/test_provenance.cpp's KG invariants, and central differences of the MLP loss."""
 import numpy as np
 import pytest
 
@@ -8,6 +8,7 @@ import caps_amd as ca
 import orcpy as O
 
 KB = 0.0019872041
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="the reference build oracle/_ref is absent (no reference sources)")
 
 
 def test_mt19937_64_check_value():
@@ -118,3 +119,35 @@ def test_mlp_forward_is_silu_layers_and_backward_its_gradient():
         fd = (_loss(Pp, x, t)[0] - _loss(Pm, x, t)[0]) / (2 * h)
         flat = np.ravel_multi_index(idx, P[li].shape) if len(idx) > 1 else idx[0]
         assert G[li][flat] == pytest.approx(fd, rel=1e-6, abs=1e-9)
+
+
+@needs_ref
+def test_normal_stream_is_the_references():
+    # caps::Normal (rng.hpp:70-89), the polar method with its spare, compiled from the reference header
+    n = 5001
+    ref = np.zeros(n)
+    O.ref().ref_rng_normal(123, n, ref)
+    r = O.Rng(123)
+    assert np.array_equal(np.array([r.normal() for _ in range(n)]), ref)
+
+
+@needs_ref
+def test_bussi_draws_are_the_references():
+    # dynamics.cpp:466-477: r1 from the run's normal generator, then a fresh caps::Gamma((ndof − 1)/2, 2) per step
+    steps, ndof = 300, 3.0 * 512 - 3
+    ref = np.zeros(2 * steps)
+    O.ref().ref_rng_bussi(77, ndof, steps, ref)
+    r = O.Rng(77)
+    got = []
+    for _ in range(steps):
+        got.append(r.normal())
+        got.append(r.gamma((ndof - 1) / 2, 2.0))
+    assert np.array_equal(np.array(got), ref)
+
+
+@needs_ref
+def test_uniform_stream_is_the_references():
+    ref = np.zeros(1000)
+    O.ref().ref_rng_uniform(5, 0.0, 1.0, 1000, ref)
+    r = O.Rng(5)
+    assert np.array_equal(np.array([r.uniform01() for _ in range(1000)]), ref)
