@@ -89,8 +89,8 @@ struct Pme {
   DevBuf<float2> half;                  // (K1/2 + 1)·K2·K3
   DevBuf<float> gtab;                   // influence function on the half spectrum
   DevBuf<double> bmod;                  // |b(m)|² per axis, K1 + K2 + K3
-  DevBuf<float> theta;                  // per atom: p weights and p derivatives per axis (6p floats)
-  DevBuf<int32_t> base;                 // per atom: the first grid index per axis (3 ints)
+  DevBuf<float> theta;                  // per atom: the B-spline offsets w of the three axes and the charge (float4)
+  DevBuf<int32_t> base;                 // per atom: the first grid index per axis and the home tile (int4)
   DevBuf<float> wts;                    // per atom: θ and dθ of the three axes (6p floats)
   DevBuf<uint32_t> tile, tcount, tfill, tatoms;
   DevBuf<int64_t> tstart;
