@@ -54,13 +54,15 @@ class EOpts(C.Structure):
 class DOpts(C.Structure):
     _fields_ = [("dt", C.c_double), ("steps", C.c_int64), ("temperature", C.c_double), ("thermostat", C.c_int32),
                 ("tau_t", C.c_double), ("seed", C.c_uint64), ("thermo_every", C.c_int32), ("new_velocities", C.c_int32),
-                ("respa", C.c_int32), ("step_offset", C.c_int64), ("learned_k", C.c_int32), ("learned_tol", C.c_double)]
+                ("respa", C.c_int32), ("step_offset", C.c_int64), ("learned_k", C.c_int32), ("learned_tol", C.c_double),
+                ("learned_train", C.c_int32)]
 
 
 class Thermo(C.Structure):
     _fields_ = [("step", C.c_int64), ("time_ps", C.c_double), ("temperature", C.c_double), ("kinetic", C.c_double),
                 ("potential", C.c_double), ("total", C.c_double), ("pressure", C.c_double), ("conserved", C.c_double),
-                ("terms", C.c_double * NTERMS), ("learned", C.c_int32), ("learned_rmse", C.c_double)]
+                ("terms", C.c_double * NTERMS), ("learned", C.c_int32), ("learned_rmse", C.c_double),
+                ("learned_rmse_stale", C.c_double)]
 
 
 class DReport(C.Structure):
@@ -112,6 +114,10 @@ def lib():
         L.capsgpu_profile.argtypes = [C.c_void_p, C.c_int32]
         L.capsgpu_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.capsgpu_profile_kernels.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        L.capsgpu_learned_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+        L.capsgpu_learned_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                               C.POINTER(C.c_double)]
+        L.capsgpu_learned_descriptors.argtypes = [C.c_void_p, C.c_void_p]
         L.capsgpu_build_melt.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_build_kg.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_system_free.argtypes = [C.POINTER(Sys)]
@@ -304,6 +310,26 @@ class Evaluator:
         _check(lib().capsgpu_profile_read(self._h, _p(ms), _p(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("list_build", "prune", "pairs", "pme", "bonded", "rest"))}
 
+    # learned r-RESPA inspection (parity checks)
+    def learned_forward(self, params, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((len(x), 3), np.float32)
+        _check(lib().capsgpu_learned_forward(self._h, _p(np.ascontiguousarray(params, np.float32)), _p(x), len(x), _p(y)))
+        return y
+
+    def learned_gradient(self, params, x, t):
+        x, t = np.ascontiguousarray(x, np.float32), np.ascontiguousarray(t, np.float32)
+        g = np.zeros(len(params), np.float32)
+        loss = C.c_double()
+        _check(lib().capsgpu_learned_gradient(self._h, _p(np.ascontiguousarray(params, np.float32)), _p(x), _p(t), len(x), _p(g),
+                                              C.byref(loss)))
+        return loss.value, g
+
+    def learned_descriptors(self):
+        d = np.zeros((self.ff.n, 96), np.float32)
+        _check(lib().capsgpu_learned_descriptors(self._h, _p(d)))
+        return d
+
     def profile_kernels(self):
         """{kernel: (summed device ms, launches)} since profiling started"""
         buf = C.create_string_buffer(1 << 16)
@@ -368,8 +394,9 @@ class DynamicsOptions:
     step_offset: int = 0
     energy: EnergyOptions = field(default_factory=EnergyOptions)
     progress: object = None          # callable(row) -> bool
-    learned_k: int = 0
-    learned_tol: float = 1e-3
+    learned_k: int = 0                # learned r-RESPA: a full PME evaluation every k steps (0: off)
+    learned_tol: float = 1e-3         # relative force RMSE above which the next interval runs plain PME
+    learned_train: int = 2000         # PME frames the network trains on before it predicts
 
 
 @dataclass
@@ -400,7 +427,7 @@ def run_dynamics(system, o, evaluator=None):
     d.dt, d.steps, d.temperature = o.dt, o.steps, o.temperature
     d.thermostat, d.tau_t, d.seed = _THERMOSTATS[o.thermostat.lower()], o.tau_t, o.seed
     d.thermo_every, d.new_velocities, d.respa, d.step_offset = o.thermo_every, int(not has_v), o.respa, o.step_offset
-    d.learned_k, d.learned_tol = o.learned_k, o.learned_tol
+    d.learned_k, d.learned_tol, d.learned_train = o.learned_k, o.learned_tol, o.learned_train
     cap = int(o.steps // max(1, o.thermo_every) + 2)
     rows = (Thermo * cap)()
     nr = C.c_int32()
@@ -421,7 +448,7 @@ def run_dynamics(system, o, evaluator=None):
 def _row(r):
     return dict(step=r.step, time_ps=r.time_ps, temperature=r.temperature, kinetic=r.kinetic, potential=r.potential, total=r.total,
                 pressure=r.pressure, conserved=r.conserved, terms=EnergyTerms.from_array(r.terms), learned=r.learned,
-                learned_rmse=r.learned_rmse)
+                learned_rmse=r.learned_rmse, learned_rmse_stale=r.learned_rmse_stale)
 
 
 # ---- synthetic systems ------------------------------------------------------------------------------------------------

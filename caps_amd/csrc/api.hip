@@ -176,11 +176,15 @@ void ev_record(capsgpu_ctx& c, int k) {
 // forces at the device positions c.x into c.f; the energy terms into t when want. The list follows the reference's
 // rules (field.cpp:1286-1312): rebuilt when an atom moved more than skin/2 since the build (flag 0), pruned at
 // r_c + b (b = min(0.8, skin)) when one moved more than b/2 since the prune (flag 1) and after every build.
-// parts: the terms to evaluate (-1: the options'), fdst: where the forces go (nullptr: c.f)
-void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts = -1, double* fdst = nullptr) {
+}  // namespace
+
+namespace capsgpu {
+// parts: the terms to evaluate (-1: the options'; bit 8 leaves PME's reciprocal part out of the non-bonded terms),
+// fdst: where the forces go (nullptr: c.f)
+void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts, double* fdst) {
   if (parts < 0) parts = c.opt.parts;
   if (!fdst) fdst = c.f.p;
-  const bool nonb = parts & 2, bonded = parts & 1, recip = pme_on(c) && (parts & 6);
+  const bool nonb = parts & 2, bonded = parts & 1, recip = pme_on(c) && (parts & 6) && !(parts & 8);
   const double b = std::min(0.8, c.opt.skin);
   prof_begin(c);
   ev_record(c, 0);
@@ -248,6 +252,10 @@ void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts 
   }
   ev_record(c, 6);
 }
+}  // namespace capsgpu
+
+namespace {
+void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known) { capsgpu::evaluate(c, want, t, flags_known, -1, nullptr); }
 
 void upload_cell(capsgpu_ctx& c, const double* cell) {
   bool same = true;
@@ -268,6 +276,8 @@ capsgpu_ctx::~capsgpu_ctx() {
     if (e) (void)hipEventDestroy(e);
   for (auto& set : pev)
     for (auto& e : set) (void)hipEventDestroy(e);
+  for (auto& k : kev) (void)hipEventDestroy(k.a), (void)hipEventDestroy(k.b);
+  learned_free(learned);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -487,6 +497,30 @@ int32_t capsgpu_profile_kernels(capsgpu_ctx* c, char* buf, int32_t cap) {
   });
 }
 
+int32_t capsgpu_learned_forward(capsgpu_ctx* c, const float* params, const float* x, int32_t m, float* y) {
+  return guard([&] {
+    if (!c || !params || !x || !y || m < 1) throw Error(CAPSGPU_EINVAL, "null argument");
+    learned_forward_host(*c, params, x, m, y);
+  });
+}
+
+int32_t capsgpu_learned_gradient(capsgpu_ctx* c, const float* params, const float* x, const float* t, int32_t m, float* grad,
+                                 double* loss) {
+  return guard([&] {
+    if (!c || !params || !x || !t || !grad || m < 1) throw Error(CAPSGPU_EINVAL, "null argument");
+    const double l = learned_gradient_host(*c, params, x, t, m, grad);
+    if (loss) *loss = l;
+  });
+}
+
+int32_t capsgpu_learned_descriptors(capsgpu_ctx* c, float* raw) {
+  return guard([&] {
+    if (!c || !raw) throw Error(CAPSGPU_EINVAL, "null argument");
+    if (!c->state_set) throw Error(CAPSGPU_EINVAL, "set the positions first (capsgpu_set_state)");
+    learned_descriptors_host(*c, raw);
+  });
+}
+
 int32_t capsgpu_list_sizes(capsgpu_ctx* c, int64_t out[8]) {
   return guard([&] {
     if (!c || !out) throw Error(CAPSGPU_EINVAL, "null argument");
@@ -609,6 +643,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       r.conserved = r.total + hb[1];
       r.learned = last_learned ? 1 : 0;
       r.learned_rmse = last_rmse;
+      r.learned_rmse_stale = LR ? learned_rmse_stale(*LR) : 0.0;
       if (rows && nr < cap) rows[nr] = r;
       ++nr;
       if (progress && !progress(&r, user)) throw Error(CAPSGPU_ECANCEL, "dynamics cancelled");

@@ -40,6 +40,15 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), cap(o.cap) { o.p = nullptr, o.n = o.cap = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p, n = o.n, cap = o.cap;
+      o.p = nullptr, o.n = o.cap = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
   void release() {
 #ifdef CAPSGPU_DEBUG_GUARD

@@ -12,6 +12,7 @@
 #include "scan.hpp"
 
 namespace capsgpu {
+struct Learned;
 
 // Geometry of one cluster-list build, fixed on the host the way core/src/nbnxm.cpp:40-47 and 146-164 fix it
 struct ListGeom {
@@ -160,6 +161,8 @@ struct capsgpu_ctx {
   std::vector<KTime> kev;
   size_t kev_used = 0;
   std::vector<std::pair<std::string, std::pair<double, int64_t>>> ktot;
+  // learned r-RESPA: the network and its state persist across dynamics runs of the context
+  capsgpu::Learned* learned = nullptr;
   ~capsgpu_ctx();
 };
 

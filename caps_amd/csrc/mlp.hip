@@ -1,12 +1,826 @@
-// Learned r-RESPA (placeholder until the MLP kernels land).
+// Learned r-RESPA (no counterpart in the reference): between full PME evaluations, every k steps, the slow force is
+// the last PME force plus an MLP's predicted increment: ΔF_i = q_i·[N(ΔD_i) − N(0)], where ΔD_i is the change of the
+// atom's 96 radial/charge descriptors within 8 Å since the last full step and N the network (96-128-128-3, SiLU) on
+// BF16 MFMA tiles with FP32 accumulation. It is trained online (Adam, FP32 master weights) on the PME forces
+// themselves: pairs of full steps k apart give (ΔD, ΔF/q). At every full step the prediction is judged against the
+// fresh PME force: above the tolerance (relative force RMSE against the full force) the next interval runs plain PME.
+//
+//  k_cell_*      atoms binned into cells of at least 8 Å (fractional coordinates), sorted by atom in each cell
+//  k_desc        per atom: 24 hat-function radial bins × {Σ q_j h, Σ q_j h d̂_x, d̂_y, d̂_z} over neighbours within
+//                8 Å (smooth cosine cut-off), accumulated in LDS, FP32
+//  k_delta_in    the network input: (D − D0)·s per feature, BF16
+//  k_gemm        C = A·B with 32x32x16 BF16 MFMA tiles, any strides, split K with ordered partial sums
+//  k_bias_silu   bias, SiLU and its derivative, the BF16 copy of the activation
+//  k_adam        Adam on FP32 master weights, BF16 copies for the MFMA tiles
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "mlp.hpp"
+#include "prof.hpp"
+#include "step.hpp"
 
 namespace capsgpu {
-struct Learned {
-  double rmse = 0;
+
+// the dynamics-side evaluation (api.hip)
+void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts, double* fdst);
+
+namespace {
+
+constexpr int kIn = 96, kHid = 128, kOut = 3, kOutPad = 32, kBins = 24;
+constexpr float kRcut = 8.0f;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline __bf16 to_bf16(float v) { return __bf16(v); }
+__device__ inline float from_bf16(__bf16 v) { return float(v); }
+
+// ---- cells for the descriptors --------------------------------------------------------------------------------------
+struct DescGeom {
+  double o[3], a[3], b[3], c[3], hi[3][3];
+  int nc[3];
 };
-Learned* learned_begin(capsgpu_ctx&, const capsgpu_dyn_opts&) { throw Error(-1, "learned r-RESPA is not available in this build"); }
-int learned_step(capsgpu_ctx&, Learned&, int64_t, bool, double*) { return 0; }
+
+__global__ void k_cell_bin(DescGeom G, int n, const double* __restrict__ x, float4* __restrict__ xw, int* __restrict__ cell,
+                           uint32_t* __restrict__ ccount) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[3];
+  const double d[3] = {x[3 * i] - G.o[0], x[3 * i + 1] - G.o[1], x[3 * i + 2] - G.o[2]};
+  int ci[3];
+  for (int k = 0; k < 3; ++k) {
+    f[k] = G.hi[k][0] * d[0] + G.hi[k][1] * d[1] + G.hi[k][2] * d[2];
+    f[k] -= floor(f[k]);
+    if (!(f[k] >= 0.0 && f[k] < 1.0)) f[k] = 0.0;   // not a number, or the rounding up to 1
+    ci[k] = min(G.nc[k] - 1, int(f[k] * G.nc[k]));
+  }
+  float4 w;
+  w.x = float(G.o[0] + G.a[0] * f[0] + G.b[0] * f[1] + G.c[0] * f[2]);
+  w.y = float(G.o[1] + G.a[1] * f[0] + G.b[1] * f[1] + G.c[1] * f[2]);
+  w.z = float(G.o[2] + G.a[2] * f[0] + G.b[2] * f[1] + G.c[2] * f[2]);
+  w.w = 0.0f;
+  xw[i] = w;
+  const int c = (ci[2] * G.nc[1] + ci[1]) * G.nc[0] + ci[0];
+  cell[i] = c;
+  atomicAdd(&ccount[c], 1u);
+}
+
+__global__ void k_cell_fill(int n, const int* __restrict__ cell, const int64_t* __restrict__ cstart, uint32_t* __restrict__ cfill,
+                            uint32_t* __restrict__ catoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell[i];
+  catoms[cstart[c] + atomicAdd(&cfill[c], 1u)] = uint32_t(i);
+}
+
+// atoms of each cell in index order, so the descriptor sums have a fixed order
+__global__ void k_cell_sort(int ncells, const int64_t* __restrict__ cstart, uint32_t* __restrict__ catoms) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int64_t b = cstart[c], e = cstart[c + 1];
+  for (int64_t k = b + 1; k < e; ++k) {
+    const uint32_t v = catoms[k];
+    int64_t j = k;
+    while (j > b && catoms[j - 1] > v) catoms[j] = catoms[j - 1], --j;
+    catoms[j] = v;
+  }
+}
+
+// 64 atoms per workgroup, each lane's 96 sums in its own LDS row (odd stride: no bank conflicts)
+constexpr int kDescRow = kIn + 1;
+__global__ void __launch_bounds__(64) k_desc(DescGeom G, int n, const float4* __restrict__ xw, const double* __restrict__ q,
+                                             const int* __restrict__ cell, const int64_t* __restrict__ cstart,
+                                             const uint32_t* __restrict__ catoms, float* __restrict__ out) {
+  __shared__ float acc[64 * kDescRow];
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * 64 + lane;
+  float* A = acc + lane * kDescRow;
+  for (int k = 0; k < kIn; ++k) A[k] = 0.0f;
+  if (i < n) {
+    const float4 xi = xw[i];
+    const int c = cell[i];
+    const int cx = c % G.nc[0], cy = (c / G.nc[0]) % G.nc[1], cz = c / (G.nc[0] * G.nc[1]);
+    const float dlt = kRcut / float(kBins - 1);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int jx = cx + dx, jy = cy + dy, jz = cz + dz;
+          // the lattice vector of the image the wrapped cell stands for
+          const int sx = jx < 0 ? -1 : jx >= G.nc[0] ? 1 : 0, sy = jy < 0 ? -1 : jy >= G.nc[1] ? 1 : 0, sz = jz < 0 ? -1 : jz >= G.nc[2] ? 1 : 0;
+          jx -= sx * G.nc[0], jy -= sy * G.nc[1], jz -= sz * G.nc[2];
+          const float shx = float(sx * G.a[0] + sy * G.b[0] + sz * G.c[0]);
+          const float shy = float(sx * G.a[1] + sy * G.b[1] + sz * G.c[1]);
+          const float shz = float(sx * G.a[2] + sy * G.b[2] + sz * G.c[2]);
+          const int h = (jz * G.nc[1] + jy) * G.nc[0] + jx;
+          for (int64_t k = cstart[h]; k < cstart[h + 1]; ++k) {
+            const uint32_t j = catoms[k];
+            const float4 xj = xw[j];
+            const float ex = xj.x + shx - xi.x, ey = xj.y + shy - xi.y, ez = xj.z + shz - xi.z;
+            const float r2 = ex * ex + ey * ey + ez * ez;
+            if (r2 >= kRcut * kRcut || r2 < 1e-6f) continue;
+            const float r = sqrtf(r2), ir = 1.0f / r;
+            const float fc = 0.5f * (1.0f + cosf(3.14159265f * r / kRcut));
+            const float u = r / dlt;
+            const int b0 = min(int(u), kBins - 2);
+            const float w1 = u - float(b0), w0 = 1.0f - w1;
+            const float qf = float(q[j]) * fc;
+            const float v0 = qf * w0, v1 = qf * w1;
+            A[b0] += v0, A[b0 + 1] += v1;
+            A[kBins + b0] += v0 * ex * ir, A[kBins + b0 + 1] += v1 * ex * ir;
+            A[2 * kBins + b0] += v0 * ey * ir, A[2 * kBins + b0 + 1] += v1 * ey * ir;
+            A[3 * kBins + b0] += v0 * ez * ir, A[3 * kBins + b0 + 1] += v1 * ez * ir;
+          }
+        }
+    for (int k = 0; k < kIn; ++k) out[size_t(i) * kIn + k] = A[k];
+  }
+}
+
+// the network input of the rows idx (or 0 … m−1): X = (D − D0)·s, BF16 (zero rows when D is null)
+__global__ void k_delta_in(int m, const uint32_t* __restrict__ idx, const float* __restrict__ D, const float* __restrict__ D0,
+                           const float* __restrict__ sc, __bf16* __restrict__ X) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(m) * kIn) return;
+  const int r = int(e / kIn), k = int(e % kIn);
+  float v = 0.0f;
+  if (D) {
+    const size_t a = idx ? size_t(idx[r]) : size_t(r);
+    v = (D[a * kIn + k] - D0[a * kIn + k]) * sc[k];
+  }
+  X[e] = to_bf16(v);
+}
+
+// ---- BF16 MFMA GEMM ---------------------------------------------------------------------------------------------------
+// C[i, j] (+)= Σ_k A(i, k) B(k, j); A(i, k) = A[i·sai + k·sak], B(k, j) = B[k·sbk + j·sbj], BF16; C FP32 at C[i·ldc + j].
+// One wavefront per 32x32 tile of C and split s of K: grid (tiles_m·tiles_n, splits). With splits > 1 the partial tiles go
+// to part[s][M][N] and k_reduce_splits sums them in split order.
+struct GemmArgs {
+  int M, N, K, ksplit;
+  const __bf16* A;
+  int64_t sai, sak;
+  const __bf16* B;
+  int64_t sbk, sbj;
+  float* C;
+  int64_t ldc;
+  int accumulate;
+};
+
+__global__ void __launch_bounds__(64) k_gemm(GemmArgs g) {
+  const int lane = threadIdx.x;
+  const int tn = (g.N + 31) / 32;
+  const int ti = blockIdx.x / tn, tj = blockIdx.x % tn;
+  const int s = blockIdx.y;
+  const int i0 = ti * 32, j0 = tj * 32;
+  const int kb = s * g.ksplit, ke = min(g.K, kb + g.ksplit);
+  f32x16 acc;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  const int row = i0 + (lane & 31), col = j0 + (lane & 31), kofs = (lane >> 5) * 8;
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+    bf16x8 a, b;
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 + kofs + t;
+      a[t] = (row < g.M && k < ke) ? g.A[row * g.sai + k * g.sak] : __bf16(0.0f);
+      b[t] = (col < g.N && k < ke) ? g.B[k * g.sbk + col * g.sbj] : __bf16(0.0f);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+  float* C = gridDim.y > 1 ? g.C + size_t(s) * g.M * g.N : g.C;
+  const int64_t ldc = gridDim.y > 1 ? g.N : g.ldc;
+  for (int q = 0; q < 4; ++q)
+    for (int t = 0; t < 4; ++t) {
+      const int i = i0 + q * 8 + (lane >> 5) * 4 + t;
+      if (i < g.M && col < g.N) {
+        float* p = C + i * ldc + col;
+        *p = (g.accumulate && gridDim.y == 1 ? *p : 0.0f) + acc[q * 4 + t];
+      }
+    }
+}
+
+__global__ void k_reduce_splits(int M, int N, int splits, const float* __restrict__ part, float* __restrict__ C, int64_t ldc) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(M) * N) return;
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += part[size_t(k) * M * N + e];
+  C[(e / N) * ldc + e % N] = s;
+}
+
+// Z (M x H, FP32, pre-activation without bias) → Z + b; H = SiLU(Z + b) as BF16; dS = SiLU'(Z + b) (optional)
+__global__ void k_bias_silu(int64_t M, int H, const float* __restrict__ b, float* __restrict__ Z, __bf16* __restrict__ out,
+                            float* __restrict__ dsilu) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= M * H) return;
+  const float z = Z[e] + b[e % H];
+  const float sg = 1.0f / (1.0f + __expf(-z));
+  out[e] = to_bf16(z * sg);
+  if (dsilu) dsilu[e] = sg * (1.0f + z * (1.0f - sg));
+}
+
+// dY = (Y + b3 − T) / m on the valid rows (FP32 Y M x 32, only 3 columns used); loss partials per block
+__global__ void k_out_grad(int M, const float* __restrict__ Y, const float* __restrict__ b3, const float* __restrict__ T,
+                           float inv_m, __bf16* __restrict__ dY, float* __restrict__ dYf, double* __restrict__ lpart) {
+  __shared__ double red[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0;
+  if (i < M)
+    for (int c = 0; c < kOutPad; ++c) {
+      float d = 0.0f;
+      if (c < kOut) {
+        d = Y[size_t(i) * kOutPad + c] + b3[c] - T[size_t(i) * kOut + c];
+        l += 0.5 * double(d) * double(d) * inv_m;
+        d *= inv_m;
+      }
+      dY[size_t(i) * kOutPad + c] = to_bf16(d);
+      dYf[size_t(i) * kOutPad + c] = d;
+    }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+// dZ = dH ⊙ SiLU' (FP32 → BF16)
+__global__ void k_mul_dsilu(int64_t n, const float* __restrict__ dH, const float* __restrict__ ds, __bf16* __restrict__ dZ,
+                            float* __restrict__ dZf) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float v = dH[e] * ds[e];
+  dZ[e] = to_bf16(v);
+  dZf[e] = v;
+}
+
+// column sums of an M x N FP32 matrix (bias gradients), one thread per column, rows in order
+__global__ void k_colsum_f(int M, int N, const float* __restrict__ X, float* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  double s = 0;
+  for (int i = 0; i < M; ++i) s += X[size_t(i) * N + j];
+  out[j] = float(s);
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights; the BF16 copy for the MFMA tiles
+__global__ void k_adam(int64_t n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1, float* __restrict__ m2,
+                       float lr, float b1, float b2, float c1, float c2, __bf16* __restrict__ wb) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gg = g[e];
+  m1[e] = b1 * m1[e] + (1 - b1) * gg;
+  m2[e] = b2 * m2[e] + (1 - b2) * gg * gg;
+  const float mh = m1[e] / c1, vh = m2[e] / c2;
+  w[e] -= lr * mh / (sqrtf(vh) + 1e-8f);
+  if (wb) wb[e] = to_bf16(w[e]);
+}
+
+__global__ void k_to_bf16(int64_t n, const float* __restrict__ a, __bf16* __restrict__ b) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n) b[e] = to_bf16(a[e]);
+}
+
+// the targets of the sampled atoms: T = (F − F0) / q, scaled by 1/e
+__global__ void k_gather_targets(int m, const uint32_t* __restrict__ idx, const double* __restrict__ f, const double* __restrict__ f0,
+                                 const double* __restrict__ q, float inv_e, float* __restrict__ T) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const uint32_t a = idx[s];
+  const double qa = q[a];
+  for (int c = 0; c < kOut; ++c) T[size_t(s) * kOut + c] = float((f[3 * size_t(a) + c] - f0[3 * size_t(a) + c]) / qa * inv_e);
+}
+
+// the slow force estimate: F = F0 + q (Y − Y_zero) e (the network's FP32 outputs; b3 cancels)
+__global__ void k_slow_estimate(int n, const double* __restrict__ f0, const float* __restrict__ Y, const float* __restrict__ Yz,
+                                const double* __restrict__ q, float e_scale, double* __restrict__ fest) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < kOut; ++c)
+    fest[3 * size_t(i) + c] = f0[3 * size_t(i) + c] + q[i] * double(Y[size_t(i) * kOutPad + c] - Yz[c]) * e_scale;
+}
+
+// Σ |a − b|², Σ |c|² per block (the relative force RMSE of a prediction a of b against the full force c)
+__global__ void k_err_parts(int n, const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ c,
+                            double* __restrict__ part) {
+  __shared__ double r0[256], r1[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0, t = 0;
+  if (i < n)
+    for (int k = 0; k < 3; ++k) {
+      const double d = a[3 * size_t(i) + k] - b[3 * size_t(i) + k];
+      e += d * d, t += c[3 * size_t(i) + k] * c[3 * size_t(i) + k];
+    }
+  r0[threadIdx.x] = e, r1[threadIdx.x] = t;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) r0[threadIdx.x] += r0[threadIdx.x + s], r1[threadIdx.x] += r1[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = r0[0], part[2 * blockIdx.x + 1] = r1[0];
+}
+
+__global__ void k_axpy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n) y[e] += x[e];
+}
+
+// per feature 1 / RMS of D − D0 over the atoms (one workgroup per feature, fixed order)
+__global__ void k_feature_scale(int n, const float* __restrict__ D, const float* __restrict__ D0, float* __restrict__ sc) {
+  __shared__ double s2[256];
+  const int k = blockIdx.x;
+  double b = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = double(D[size_t(i) * kIn + k]) - double(D0[size_t(i) * kIn + k]);
+    b += v * v;
+  }
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) s2[threadIdx.x] += s2[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(s2[0] / n);
+    sc[k] = float(rms > 1e-12 ? 1.0 / rms : 1.0);
+  }
+}
+
+}  // namespace
+
+// ---- host side ----------------------------------------------------------------------------------------------------------
+struct Learned {
+  int k = 4, train_frames = 2000, frames = 0, samples = 8192, iters = 4;
+  double tol = 1e-3, rmse = 0, rmse_stale = 0, e_scale = 1.0;
+  int64_t last_full = -1;
+  bool learned_mode = false, scaled = false;
+  uint64_t rng = 0x9E3779B97F4A7C15ull;
+  float lr = 1e-3f;
+  int64_t adam_t = 0;
+  double t_slow[CAPSGPU_NTERMS] = {};
+  // weights: FP32 masters, BF16 copies, Adam moments, gradients
+  DevBuf<float> w1, b1, w2, b2, w3, b3, m1, m2, grad;
+  DevBuf<__bf16> w1b, w2b, w3b;
+  // descriptors and cells
+  DevBuf<float4> xw;
+  DevBuf<int> cell;
+  DevBuf<uint32_t> ccount, cfill, catoms;
+  DevBuf<int64_t> cstart;
+  DevBuf<float> sc, Dcur, D0;
+  std::vector<DevBuf<float>> Dring;
+  std::vector<DevBuf<double>> Fring;
+  int ring_n = 0;
+  // inference and training activations
+  DevBuf<float> Z1, Z2, Y, Yz, dsilu1, dsilu2, Tt, dYf, dH, dZf, part;
+  DevBuf<__bf16> H1, H2, X, Xs, dYb, dZb;
+  DevBuf<uint32_t> sidx;
+  DevBuf<double> f0, fslow, fest, ftot, epart;
+};
+
+namespace {
+
+DescGeom desc_geom(const capsgpu_ctx& c) {
+  DescGeom G{};
+  const double *a = c.cell + 3, *b = c.cell + 6, *cc = c.cell + 9;
+  for (int k = 0; k < 3; ++k) G.o[k] = c.cell[k], G.a[k] = a[k], G.b[k] = b[k], G.c[k] = cc[k];
+  const double bc[3] = {b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]};
+  const double ca[3] = {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]};
+  const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double v = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+  const double* rows[3] = {bc, ca, ab};
+  for (int d = 0; d < 3; ++d) {
+    for (int k = 0; k < 3; ++k) G.hi[d][k] = rows[d][k] / v;
+    const double w = std::fabs(v) / std::sqrt(rows[d][0] * rows[d][0] + rows[d][1] * rows[d][1] + rows[d][2] * rows[d][2]);
+    G.nc[d] = int(w / double(kRcut));
+    if (G.nc[d] < 3) throw Error(CAPSGPU_EINVAL, "learned r-RESPA needs a cell at least 3 x 8 A wide along each axis");
+  }
+  return G;
+}
+
+void gemm(capsgpu_ctx& c, Learned& L, int M, int N, int K, const __bf16* A, int64_t sai, int64_t sak, const __bf16* B, int64_t sbk,
+          int64_t sbj, float* C, int64_t ldc) {
+  GemmArgs g{M, N, K, K, A, sai, sak, B, sbk, sbj, C, ldc, 0};
+  int splits = 1;
+  if (M * N <= 128 * 128 && K > 1024) {
+    splits = std::min(64, (K + 511) / 512);
+    g.ksplit = ((K + splits - 1) / splits + 15) / 16 * 16;
+    splits = (K + g.ksplit - 1) / g.ksplit;
+    L.part.resize(size_t(splits) * M * N);
+    g.C = L.part.p;
+  }
+  const unsigned tiles = unsigned(((M + 31) / 32) * ((N + 31) / 32));
+  kt_begin(c, "k_gemm");
+  hipLaunchKernelGGL(k_gemm, dim3(tiles, unsigned(splits)), dim3(64), 0, c.stream, g);
+  LAUNCHCHECK();
+  kt_end(c);
+  if (splits > 1) {
+    hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(size_t(M) * N, 256)), dim3(256), 0, c.stream, M, N, splits, L.part.p, C, ldc);
+    LAUNCHCHECK();
+  }
+}
+
+// raw descriptors of every atom at the device positions into out (n x 96, FP32)
+void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
+  const int n = c.n;
+  hipStream_t s = c.stream;
+  const DescGeom G = desc_geom(c);
+  const int ncells = G.nc[0] * G.nc[1] * G.nc[2];
+  L.xw.resize(size_t(n));
+  L.cell.resize(size_t(n));
+  L.ccount.resize(size_t(ncells));
+  L.ccount.zero(s);
+  kt_begin(c, "k_cell_bin");
+  hipLaunchKernelGGL(k_cell_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, L.xw.p, L.cell.p, L.ccount.p);
+  LAUNCHCHECK();
+  kt_end(c);
+  L.cstart.resize(size_t(ncells) + 1);
+  c.scan.exclusive(L.ccount.p, L.cstart.p, size_t(ncells), s);
+  L.cfill.resize(size_t(ncells));
+  L.cfill.zero(s);
+  L.catoms.resize(size_t(n));
+  hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.cfill.p, L.catoms.p);
+  LAUNCHCHECK();
+  kt_begin(c, "k_cell_sort");
+  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(ncells, 64)), dim3(64), 0, s, ncells, L.cstart.p, L.catoms.p);
+  LAUNCHCHECK();
+  kt_end(c);
+  kt_begin(c, "k_desc");
+  hipLaunchKernelGGL(k_desc, dim3(blocks_for(n, 64)), dim3(64), 0, s, G, n, L.xw.p, c.d_charge.p, L.cell.p, L.cstart.p, L.catoms.p, out);
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+// the network on M rows of X (BF16, M x 96): FP32 outputs (without b3) in Y (M x 32); activations kept for training
+void forward(capsgpu_ctx& c, Learned& L, int M, const __bf16* X, float* Y, bool keep) {
+  hipStream_t s = c.stream;
+  L.Z1.resize(size_t(M) * kHid);
+  L.Z2.resize(size_t(M) * kHid);
+  L.H1.resize(size_t(M) * kHid);
+  L.H2.resize(size_t(M) * kHid);
+  if (keep) {
+    L.dsilu1.resize(size_t(M) * kHid);
+    L.dsilu2.resize(size_t(M) * kHid);
+  }
+  // Z1 = X W1ᵀ: A = X (M x 96), B(k, j) = W1[j][k]
+  gemm(c, L, M, kHid, kIn, X, kIn, 1, L.w1b.p, 1, kIn, L.Z1.p, kHid);
+  kt_begin(c, "k_bias_silu");
+  hipLaunchKernelGGL(k_bias_silu, dim3(blocks_for(size_t(M) * kHid, 256)), dim3(256), 0, s, int64_t(M), kHid, L.b1.p, L.Z1.p, L.H1.p,
+                     keep ? L.dsilu1.p : nullptr);
+  LAUNCHCHECK();
+  kt_end(c);
+  gemm(c, L, M, kHid, kHid, L.H1.p, kHid, 1, L.w2b.p, 1, kHid, L.Z2.p, kHid);
+  kt_begin(c, "k_bias_silu");
+  hipLaunchKernelGGL(k_bias_silu, dim3(blocks_for(size_t(M) * kHid, 256)), dim3(256), 0, s, int64_t(M), kHid, L.b2.p, L.Z2.p, L.H2.p,
+                     keep ? L.dsilu2.p : nullptr);
+  LAUNCHCHECK();
+  kt_end(c);
+  // Y = H2 W3ᵀ with W3 padded to 32 output rows (rows 3 … 31 zero)
+  gemm(c, L, M, kOutPad, kHid, L.H2.p, kHid, 1, L.w3b.p, 1, kHid, Y, kOutPad);
+}
+
+// the network's output at zero input, subtracted from every prediction (an increment of zero gives zero)
+void zero_output(capsgpu_ctx& c, Learned& L) {
+  L.X.resize(size_t(32) * kIn);
+  hipLaunchKernelGGL(k_delta_in, dim3(blocks_for(32 * kIn, 256)), dim3(256), 0, c.stream, 32, nullptr, nullptr, nullptr, nullptr, L.X.p);
+  LAUNCHCHECK();
+  L.Yz.resize(size_t(32) * kOutPad);
+  forward(c, L, 32, L.X.p, L.Yz.p, false);
+}
+
+// one Adam step on the sampled rows (Xs, Tt)
+void train_step(capsgpu_ctx& c, Learned& L, int M) {
+  hipStream_t s = c.stream;
+  L.Y.resize(size_t(M) * kOutPad);
+  forward(c, L, M, L.Xs.p, L.Y.p, true);
+  const unsigned nb = blocks_for(M, 256);
+  L.dYb.resize(size_t(M) * kOutPad);
+  L.dYf.resize(size_t(M) * kOutPad);
+  L.epart.resize(nb);
+  hipLaunchKernelGGL(k_out_grad, dim3(nb), dim3(256), 0, s, M, L.Y.p, L.b3.p, L.Tt.p, 1.0f / float(M), L.dYb.p, L.dYf.p, L.epart.p);
+  LAUNCHCHECK();
+  // gradient layout in L.grad: w1 (128·96), b1 (128), w2 (128·128), b2 (128), w3 (32·128), b3 (32)
+  float* gw1 = L.grad.p;
+  float* gb1 = gw1 + kHid * kIn;
+  float* gw2 = gb1 + kHid;
+  float* gb2 = gw2 + kHid * kHid;
+  float* gw3 = gb2 + kHid;
+  float* gb3 = gw3 + kOutPad * kHid;
+  // dW3 = dYᵀ H2 (32 x 128, K = M); db3 = Σ dY
+  gemm(c, L, kOutPad, kHid, M, L.dYb.p, 1, kOutPad, L.H2.p, kHid, 1, gw3, kHid);
+  hipLaunchKernelGGL(k_colsum_f, dim3(1), dim3(64), 0, s, M, kOutPad, L.dYf.p, gb3);
+  LAUNCHCHECK();
+  // dH2 = dY W3 (M x 128, K = 32)
+  L.dH.resize(size_t(M) * kHid);
+  L.dZb.resize(size_t(M) * kHid);
+  L.dZf.resize(size_t(M) * kHid);
+  gemm(c, L, M, kHid, kOutPad, L.dYb.p, kOutPad, 1, L.w3b.p, kHid, 1, L.dH.p, kHid);
+  hipLaunchKernelGGL(k_mul_dsilu, dim3(blocks_for(size_t(M) * kHid, 256)), dim3(256), 0, s, int64_t(M) * kHid, L.dH.p, L.dsilu2.p, L.dZb.p,
+                     L.dZf.p);
+  LAUNCHCHECK();
+  // dW2 = dZ2ᵀ H1 (128 x 128, K = M); db2 = Σ dZ2
+  gemm(c, L, kHid, kHid, M, L.dZb.p, 1, kHid, L.H1.p, kHid, 1, gw2, kHid);
+  hipLaunchKernelGGL(k_colsum_f, dim3(blocks_for(kHid, 64)), dim3(64), 0, s, M, kHid, L.dZf.p, gb2);
+  LAUNCHCHECK();
+  // dH1 = dZ2 W2 (M x 128, K = 128)
+  gemm(c, L, M, kHid, kHid, L.dZb.p, kHid, 1, L.w2b.p, kHid, 1, L.dH.p, kHid);
+  hipLaunchKernelGGL(k_mul_dsilu, dim3(blocks_for(size_t(M) * kHid, 256)), dim3(256), 0, s, int64_t(M) * kHid, L.dH.p, L.dsilu1.p, L.dZb.p,
+                     L.dZf.p);
+  LAUNCHCHECK();
+  // dW1 = dZ1ᵀ X (128 x 96, K = M); db1 = Σ dZ1
+  gemm(c, L, kHid, kIn, M, L.dZb.p, 1, kHid, L.Xs.p, kIn, 1, gw1, kIn);
+  hipLaunchKernelGGL(k_colsum_f, dim3(blocks_for(kHid, 64)), dim3(64), 0, s, M, kHid, L.dZf.p, gb1);
+  LAUNCHCHECK();
+  // Adam on every parameter block
+  ++L.adam_t;
+  const float b1 = 0.9f, b2 = 0.999f, c1 = 1.0f - std::pow(b1, float(L.adam_t)), c2 = 1.0f - std::pow(b2, float(L.adam_t));
+  struct Blk {
+    float* w;
+    __bf16* wb;
+    int64_t n;
+    int64_t off;
+  } blks[6] = {{L.w1.p, L.w1b.p, kHid * kIn, 0},
+               {L.b1.p, nullptr, kHid, kHid * kIn},
+               {L.w2.p, L.w2b.p, kHid * kHid, kHid * kIn + kHid},
+               {L.b2.p, nullptr, kHid, kHid * kIn + kHid + kHid * kHid},
+               {L.w3.p, L.w3b.p, kOutPad * kHid, kHid * kIn + 2 * kHid + kHid * kHid},
+               {L.b3.p, nullptr, kOutPad, kHid * kIn + 2 * kHid + kHid * kHid + kOutPad * kHid}};
+  for (const Blk& b : blks) {
+    kt_begin(c, "k_adam");
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(size_t(b.n), 256)), dim3(256), 0, s, b.n, b.w, L.grad.p + b.off, L.m1.p + b.off, L.m2.p + b.off,
+                       L.lr, b1, b2, c1, c2, b.wb);
+    LAUNCHCHECK();
+    kt_end(c);
+  }
+}
+
+uint64_t splitmix(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// a training pair: the descriptor change and the PME force change (per unit charge) of sampled charged atoms between
+// an earlier full step (Dold, Fold) and this one (Dcur, fslow)
+void train_on_pair(capsgpu_ctx& c, Learned& L, const float* Dold, const double* Fold) {
+  hipStream_t s = c.stream;
+  if (!L.scaled) {
+    // the feature scales and the target scale from the first pair, so inputs and targets are O(1)
+    L.sc.resize(kIn);
+    hipLaunchKernelGGL(k_feature_scale, dim3(kIn), dim3(256), 0, s, c.n, L.Dcur.p, Dold, L.sc.p);
+    LAUNCHCHECK();
+    std::vector<double> f = L.fslow.to_host(s), g(3 * size_t(c.n));
+    HIPCHECK(hipMemcpyAsync(g.data(), Fold, g.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    double acc = 0;
+    int64_t cnt = 0;
+    for (int i = 0; i < c.n; ++i) {
+      const double q = c.charge[size_t(i)];
+      if (std::fabs(q) > 1e-3)
+        for (int k = 0; k < 3; ++k) acc += ((f[3 * size_t(i) + k] - g[3 * size_t(i) + k]) / q) * ((f[3 * size_t(i) + k] - g[3 * size_t(i) + k]) / q), ++cnt;
+    }
+    L.e_scale = cnt && acc > 0 ? std::sqrt(acc / double(cnt)) : 1.0;
+    L.scaled = true;
+  }
+  std::vector<uint32_t> pool;
+  pool.reserve(size_t(c.n));
+  for (int i = 0; i < c.n; ++i)
+    if (std::fabs(c.charge[size_t(i)]) > 1e-3) pool.push_back(uint32_t(i));
+  const int M = std::min<int>(L.samples, int(pool.size())) / 32 * 32;
+  if (M < 32) return;
+  std::vector<uint32_t> pick(static_cast<size_t>(M));
+  for (int k = 0; k < M; ++k) pick[size_t(k)] = pool[size_t(splitmix(L.rng) % pool.size())];
+  L.sidx.upload(pick, s);
+  L.Xs.resize(size_t(M) * kIn);
+  L.Tt.resize(size_t(M) * kOut);
+  hipLaunchKernelGGL(k_delta_in, dim3(blocks_for(size_t(M) * kIn, 256)), dim3(256), 0, s, M, L.sidx.p, L.Dcur.p, Dold, L.sc.p, L.Xs.p);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(k_gather_targets, dim3(blocks_for(M, 256)), dim3(256), 0, s, M, L.sidx.p, L.fslow.p, Fold, c.d_charge.p,
+                     float(1.0 / L.e_scale), L.Tt.p);
+  LAUNCHCHECK();
+  for (int it = 0; it < L.iters; ++it) train_step(c, L, M);
+  zero_output(c, L);
+}
+
+double rel_rmse(capsgpu_ctx& c, Learned& L, const double* pred, const double* truth, const double* full) {
+  const unsigned nb = blocks_for(c.n, 256);
+  L.epart.resize(2 * size_t(nb));
+  hipLaunchKernelGGL(k_err_parts, dim3(nb), dim3(256), 0, c.stream, c.n, pred, truth, full, L.epart.p);
+  LAUNCHCHECK();
+  std::vector<double> h = L.epart.to_host(c.stream);
+  double e = 0, t = 0;
+  for (unsigned b = 0; b < nb; ++b) e += h[2 * b], t += h[2 * b + 1];
+  return t > 0 ? std::sqrt(e / t) : 0.0;
+}
+
+// the slow-force estimate at the current descriptors from the reference point (D0, f0) into L.fest
+void estimate(capsgpu_ctx& c, Learned& L) {
+  L.X.resize(size_t(c.n) * kIn);
+  hipLaunchKernelGGL(k_delta_in, dim3(blocks_for(size_t(c.n) * kIn, 256)), dim3(256), 0, c.stream, c.n, nullptr, L.Dcur.p, L.D0.p, L.sc.p,
+                     L.X.p);
+  LAUNCHCHECK();
+  L.Y.resize(size_t(c.n) * kOutPad);
+  forward(c, L, c.n, L.X.p, L.Y.p, false);
+  hipLaunchKernelGGL(k_slow_estimate, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, L.f0.p, L.Y.p, L.Yz.p, c.d_charge.p,
+                     float(L.e_scale), L.fest.p);
+  LAUNCHCHECK();
+}
+
+void init_weights(capsgpu_ctx& c, Learned& L) {
+  hipStream_t s = c.stream;
+  uint64_t st = 0x5EEDull;
+  auto normal = [&](double sd) {
+    const double u1 = (double(splitmix(st) >> 11) + 0.5) * 0x1.0p-53, u2 = (double(splitmix(st) >> 11) + 0.5) * 0x1.0p-53;
+    return sd * std::sqrt(-2 * std::log(u1)) * std::cos(2 * kPi * u2);
+  };
+  auto fill = [&](DevBuf<float>& d, DevBuf<__bf16>* db, int rows, int cols, int valid_rows, double sd) {
+    std::vector<float> h(size_t(rows) * cols, 0.0f);
+    for (int r = 0; r < valid_rows; ++r)
+      for (int k = 0; k < cols; ++k) h[size_t(r) * cols + k] = float(normal(sd));
+    d.upload(h, s);
+    if (db) {
+      db->resize(h.size());
+      hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(h.size(), 256)), dim3(256), 0, s, int64_t(h.size()), d.p, db->p);
+      LAUNCHCHECK();
+    }
+  };
+  fill(L.w1, &L.w1b, kHid, kIn, kHid, std::sqrt(1.0 / kIn));
+  fill(L.w2, &L.w2b, kHid, kHid, kHid, std::sqrt(1.0 / kHid));
+  fill(L.w3, &L.w3b, kOutPad, kHid, kOut, 0.1 * std::sqrt(1.0 / kHid));
+  std::vector<float> z1(kHid, 0.0f), z3(kOutPad, 0.0f);
+  L.b1.upload(z1, s);
+  L.b2.upload(z1, s);
+  L.b3.upload(z3, s);
+  const size_t np = size_t(kHid) * kIn + kHid + size_t(kHid) * kHid + kHid + size_t(kOutPad) * kHid + kOutPad;
+  L.grad.resize(np);
+  L.m1.resize(np);
+  L.m2.resize(np);
+  L.grad.zero(s);
+  L.m1.zero(s);
+  L.m2.zero(s);
+  std::vector<float> one(kIn, 1.0f);
+  L.sc.upload(one, s);
+  zero_output(c, L);
+}
+
+Learned& learned_of(capsgpu_ctx& c) {
+  if (!c.learned) {
+    c.learned = new Learned;
+    init_weights(c, *c.learned);
+  }
+  return *c.learned;
+}
+
+}  // namespace
+
+Learned* learned_begin(capsgpu_ctx& c, const capsgpu_dyn_opts& o) {
+  Learned& L = learned_of(c);
+  L.k = o.learned_k;
+  L.tol = o.learned_tol > 0 ? o.learned_tol : 1e-3;
+  L.train_frames = o.learned_train >= 0 ? o.learned_train : 2000;
+  L.last_full = -1;
+  L.ring_n = 0;
+  (void)desc_geom(c);   // the cell must hold 3 cells of 8 Å per axis
+  const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
+  L.f0.resize(n3);
+  L.fslow.resize(n3);
+  L.fest.resize(n3);
+  L.ftot.resize(n3);
+  L.Dcur.resize(nd);
+  L.D0.resize(nd);
+  L.Dring.resize(size_t(L.k));
+  L.Fring.resize(size_t(L.k));
+  for (int r = 0; r < L.k; ++r) L.Dring[size_t(r)].resize(nd), L.Fring[size_t(r)].resize(n3);
+  return &L;
+}
+
+// one step's forces into c.f (and the terms): the fast part (everything but PME's reciprocal part) at every step; the
+// slow part by PME at full steps (training, judging, a new reference point) or by the network in between
+int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t) {
+  hipStream_t s = c.stream;
+  const int fast_parts = (c.opt.parts & 3) | 8;
+  double tf[CAPSGPU_NTERMS];
+  evaluate(c, want, tf, true, fast_parts, c.f.p);
+  const bool training = L.frames < L.train_frames;
+  const bool full = training || !L.learned_mode || L.last_full < 0 || step - L.last_full >= L.k;
+  const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
+  descriptors(c, L, L.Dcur.p);
+  int result;
+  if (full) {
+    double ts[CAPSGPU_NTERMS];
+    evaluate(c, true, ts, true, 4, L.fslow.p);
+    std::memcpy(L.t_slow, ts, sizeof ts);
+    // judge the prediction made from the last full step, and the stale force, against this one (after training)
+    if (L.last_full >= 0 && !training) {
+      estimate(c, L);
+      HIPCHECK(hipMemcpyAsync(L.ftot.p, c.f.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fslow.p, L.ftot.p);
+      LAUNCHCHECK();
+      L.rmse = rel_rmse(c, L, L.fest.p, L.fslow.p, L.ftot.p);
+      L.rmse_stale = rel_rmse(c, L, L.f0.p, L.fslow.p, L.ftot.p);
+    }
+    // train on the pair (k steps ago, now): from the ring while every step is full, else from the last full step
+    if (training) {
+      const size_t slot = size_t(L.frames % L.k);
+      if (L.ring_n >= L.k) train_on_pair(c, L, L.Dring[slot].p, L.Fring[slot].p);
+      HIPCHECK(hipMemcpyAsync(L.Dring[slot].p, L.Dcur.p, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(L.Fring[slot].p, L.fslow.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      L.ring_n = std::min(L.ring_n + 1, L.k);
+    } else if (L.last_full >= 0 && step - L.last_full == L.k) {
+      train_on_pair(c, L, L.D0.p, L.f0.p);
+    }
+    ++L.frames;
+    // the reference point of the next interval
+    HIPCHECK(hipMemcpyAsync(L.D0.p, L.Dcur.p, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(L.f0.p, L.fslow.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const bool was = L.learned_mode;
+    L.learned_mode = L.frames >= L.train_frames && L.scaled && (training || L.rmse <= L.tol);
+    L.last_full = step;
+    result = (was && !L.learned_mode) ? 2 : 0;
+    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fslow.p, c.f.p);
+    LAUNCHCHECK();
+  } else {
+    estimate(c, L);
+    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fest.p, c.f.p);
+    LAUNCHCHECK();
+    result = 1;
+  }
+  if (want)
+    for (int k = 0; k < CAPSGPU_NTERMS; ++k) t[k] = tf[k] + L.t_slow[k];
+  return result;
+}
+
 double learned_rmse(const Learned& L) { return L.rmse; }
-void learned_end(capsgpu_ctx&, Learned* L) { delete L; }
+double learned_rmse_stale(const Learned& L) { return L.rmse_stale; }
+void learned_end(capsgpu_ctx&, Learned*) {}
+void learned_free(Learned* L) { delete L; }
+
+// ---- inspection for the parity checks: the network on given inputs, one training step's gradients, the descriptors ---
+void learned_forward_host(capsgpu_ctx& c, const float* params, const float* x, int m, float* y) {
+  Learned& L = learned_of(c);
+  hipStream_t s = c.stream;
+  const size_t n1 = size_t(kHid) * kIn, n2 = size_t(kHid) * kHid;
+  std::vector<float> w1(params, params + n1), b1(params + n1, params + n1 + kHid), w2(params + n1 + kHid, params + n1 + kHid + n2),
+      b2(params + n1 + kHid + n2, params + n1 + 2 * kHid + n2);
+  std::vector<float> w3(size_t(kOutPad) * kHid, 0.0f), b3(kOutPad, 0.0f);
+  std::copy(params + n1 + 2 * kHid + n2, params + n1 + 2 * kHid + n2 + kOut * kHid, w3.begin());
+  std::copy(params + n1 + 2 * kHid + n2 + kOut * kHid, params + n1 + 2 * kHid + n2 + kOut * kHid + kOut, b3.begin());
+  L.w1.upload(w1, s), L.b1.upload(b1, s), L.w2.upload(w2, s), L.b2.upload(b2, s), L.w3.upload(w3, s), L.b3.upload(b3, s);
+  L.w1b.resize(w1.size()), L.w2b.resize(w2.size()), L.w3b.resize(w3.size());
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(w1.size(), 256)), dim3(256), 0, s, int64_t(w1.size()), L.w1.p, L.w1b.p);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(w2.size(), 256)), dim3(256), 0, s, int64_t(w2.size()), L.w2.p, L.w2b.p);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(w3.size(), 256)), dim3(256), 0, s, int64_t(w3.size()), L.w3.p, L.w3b.p);
+  LAUNCHCHECK();
+  const int M = (m + 31) / 32 * 32;
+  std::vector<float> xp(size_t(M) * kIn, 0.0f);
+  std::copy(x, x + size_t(m) * kIn, xp.begin());
+  DevBuf<float> xf;
+  xf.upload(xp, s);
+  L.Xs.resize(xp.size());
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(xp.size(), 256)), dim3(256), 0, s, int64_t(xp.size()), xf.p, L.Xs.p);
+  LAUNCHCHECK();
+  L.Y.resize(size_t(M) * kOutPad);
+  forward(c, L, M, L.Xs.p, L.Y.p, false);
+  std::vector<float> yh = L.Y.to_host(s);
+  for (int i = 0; i < m; ++i)
+    for (int k = 0; k < kOut; ++k) y[size_t(i) * kOut + k] = yh[size_t(i) * kOutPad + k] + b3[size_t(k)];
+}
+
+// gradients of L = ½ Σ (y − t)² / m at the given parameters (the layout of the forward call), into grad (same layout)
+double learned_gradient_host(capsgpu_ctx& c, const float* params, const float* x, const float* t, int m, float* grad) {
+  if (m % 32) throw Error(CAPSGPU_EINVAL, "the gradient check takes a multiple of 32 rows");
+  std::vector<float> y(size_t(m) * kOut);
+  learned_forward_host(c, params, x, m, y.data());
+  Learned& L = *c.learned;
+  hipStream_t s = c.stream;
+  L.Tt.upload(t, size_t(m) * kOut, s);
+  // one train_step with a zero learning rate; the Adam state restored afterwards
+  const float lr = L.lr;
+  L.lr = 0.0f;
+  std::vector<float> keep_m1 = L.m1.to_host(s), keep_m2 = L.m2.to_host(s);
+  const int64_t keep_t = L.adam_t;
+  train_step(c, L, m);
+  L.lr = lr;
+  L.m1.upload(keep_m1, s);
+  L.m2.upload(keep_m2, s);
+  L.adam_t = keep_t;
+  std::vector<float> g = L.grad.to_host(s);
+  std::vector<double> lp = L.epart.to_host(s);
+  double loss = 0;
+  for (double v : lp) loss += v;
+  const size_t n1 = size_t(kHid) * kIn, n2 = size_t(kHid) * kHid;
+  std::copy(g.begin(), g.begin() + long(n1 + kHid + n2 + kHid), grad);
+  std::copy(g.begin() + long(n1 + 2 * kHid + n2), g.begin() + long(n1 + 2 * kHid + n2 + kOut * kHid), grad + n1 + 2 * kHid + n2);
+  std::copy(g.begin() + long(n1 + 2 * kHid + n2 + kOutPad * kHid), g.begin() + long(n1 + 2 * kHid + n2 + kOutPad * kHid + kOut),
+            grad + n1 + 2 * kHid + n2 + kOut * kHid);
+  return loss;
+}
+
+// the raw descriptors of every atom at the set positions
+void learned_descriptors_host(capsgpu_ctx& c, float* raw_out) {
+  Learned& L = learned_of(c);
+  L.Dcur.resize(size_t(c.n) * kIn);
+  descriptors(c, L, L.Dcur.p);
+  L.Dcur.download(raw_out, size_t(c.n) * kIn, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+}
+
 }  // namespace capsgpu

@@ -123,6 +123,8 @@ typedef struct capsgpu_dyn_opts {
    * learned_tol (relative to the full force) at a full step */
   int32_t learned_k;
   double learned_tol;
+  int32_t learned_train;        /* learned r-RESPA: PME frames (plain steps) the network trains on before it predicts;
+                                   < 0: 2000. The network persists in the context across runs. */
 } capsgpu_dyn_opts;
 
 /* caps::ThermoRow (core/include/caps/dynamics.hpp:28-43), the fields this path fills */
@@ -132,6 +134,7 @@ typedef struct capsgpu_thermo {
   double terms[CAPSGPU_NTERMS];
   int32_t learned;              /* learned r-RESPA: 1 this row's forces came from the MLP path */
   double learned_rmse;          /* learned r-RESPA: the last judged relative force RMSE (0 before the first) */
+  double learned_rmse_stale;    /* learned r-RESPA: the same for the stale slow force held since the last full step */
 } capsgpu_thermo;
 
 /* Progress callback with each thermo row: return 0 to cancel (caps::DynamicsOptions::progress, dynamics.hpp:~141) */
@@ -171,6 +174,18 @@ int32_t capsgpu_profile_read(capsgpu_ctx* ctx, double ms[6], int64_t counts[6]);
 /* per-kernel launch times since profiling started, as text: one "name summed_ms launches" line per kernel, truncated
  * to cap − 1 characters and NUL-terminated */
 int32_t capsgpu_profile_kernels(capsgpu_ctx* ctx, char* buf, int32_t cap);
+
+/* ---- learned r-RESPA inspection, for the parity checks (no counterpart in the reference) --------------------------- */
+/* The network (96-128-128-3, SiLU; BF16 MFMA, FP32 accumulation) at the given FP32 parameters, laid out w1 [128][96],
+ * b1 [128], w2 [128][128], b2 [128], w3 [3][128], b3 [3] (29,315 values): y [m][3] for x [m][96]. Replaces the
+ * context's network. */
+int32_t capsgpu_learned_forward(capsgpu_ctx* ctx, const float* params, const float* x, int32_t m, float* y);
+/* the gradient of L = ½ Σ (y − t)² / m with respect to the parameters (same layout) and L; m a multiple of 32 */
+int32_t capsgpu_learned_gradient(capsgpu_ctx* ctx, const float* params, const float* x, const float* t, int32_t m, float* grad,
+                                 double* loss);
+/* the raw descriptors [n][96] of the set positions: 24 hat-function radial bins (0 … 8 Å) × {Σ q_j h, Σ q_j h d̂_x,
+ * Σ q_j h d̂_y, Σ q_j h d̂_z} with the cosine cut-off ½(1 + cos πr/8) */
+int32_t capsgpu_learned_descriptors(capsgpu_ctx* ctx, float* raw);
 
 #ifdef __cplusplus
 }

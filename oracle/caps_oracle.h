@@ -130,6 +130,9 @@ double orc_mlp_backward(const double* w1, const double* b1, const double* w2, co
                         const double* b3, const double* x, const double* t, int32_t n, double* gw1, double* gb1,
                         double* gw2, double* gb2, double* gw3, double* gb3);
 
+/* the 96 descriptors per atom (24 hat bins on 0 … 8 Å × {q, q d̂_x, q d̂_y, q d̂_z}, cosine cut-off): out [n][96] */
+void orc_descriptors(const double* x, const double* q, int32_t n, const double cell[12], double* out);
+
 const char* orc_last_error(void);
 
 #ifdef __cplusplus

@@ -233,3 +233,36 @@ double orc_mlp_backward(const double* w1, const double* b1, const double* w2, co
   }
   return loss;
 }
+
+/* Learned r-RESPA descriptors (no reference counterpart): per atom i, over the atoms j within 8 Å (nearest image; the
+ * cell is at least 24 Å wide), 24 hat functions h_k of r on centres k·8/23 (k = 0 … 23) times the cut-off
+ * ½(1 + cos πr/8) and q_j: out[i][k] = Σ q_j f h_k, out[i][24 + 24c + k] = Σ q_j f h_k d_c / r (c = x, y, z). */
+void orc_descriptors(const double* x, const double* q, int32_t n, const double cell[12], double* out) {
+  const double *a = cell + 3, *b = cell + 6, *c = cell + 9;
+  double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+  double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+  double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double v = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+  const double dlt = 8.0 / 23.0;
+  for (int i = 0; i < n; ++i) {
+    double* o = out + (size_t)i * 96;
+    for (int k = 0; k < 96; ++k) o[k] = 0;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      double d[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+      const double f[3] = {(d[0] * bc[0] + d[1] * bc[1] + d[2] * bc[2]) / v, (d[0] * ca[0] + d[1] * ca[1] + d[2] * ca[2]) / v,
+                           (d[0] * ab[0] + d[1] * ab[1] + d[2] * ab[2]) / v};
+      const double m[3] = {floor(f[0] + 0.5), floor(f[1] + 0.5), floor(f[2] + 0.5)};
+      for (int k = 0; k < 3; ++k) d[k] -= m[0] * a[k] + m[1] * b[k] + m[2] * c[k];
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 >= 64.0 || r2 < 1e-6) continue;
+      const double r = sqrt(r2), fc = 0.5 * (1 + cos(3.14159265358979323846 * r / 8.0));
+      for (int k = 0; k < 24; ++k) {
+        const double h = fmax(0.0, 1.0 - fabs(r - k * dlt) / dlt) * fc * q[j];
+        if (h == 0) continue;
+        o[k] += h;
+        for (int cc = 0; cc < 3; ++cc) o[24 + 24 * cc + k] += h * d[cc] / r;
+      }
+    }
+  }
+}

@@ -83,6 +83,7 @@ def orc():
         L.orc_mlp_forward.argtypes = [f64p] * 7 + [C.c_int32, f64p]
         L.orc_mlp_backward.restype = C.c_double
         L.orc_mlp_backward.argtypes = [f64p] * 8 + [C.c_int32] + [f64p] * 6
+        L.orc_descriptors.argtypes = [f64p, f64p, C.c_int32, f64p, f64p]
         L.orc_last_error.restype = C.c_char_p
         _orc = L
     return _orc

@@ -151,3 +151,48 @@ def test_uniform_stream_is_the_references():
     O.ref().ref_rng_uniform(5, 0.0, 1.0, 1000, ref)
     r = O.Rng(5)
     assert np.array_equal(np.array([r.uniform01() for _ in range(1000)]), ref)
+
+
+def test_descriptor_bins_are_hat_functions_with_the_cosine_cutoff():
+    # two atoms at distance r: only the two bins around r/Δ (Δ = 8/23) hold q_j·f(r)·(1 − frac, frac); the vector
+    # channels carry the same times the unit vector
+    L = 30.0
+    cell = np.array([0, 0, 0, L, 0, 0, 0, L, 0, 0, 0, L], float)
+    for r in (0.9, 3.3, 5.0, 7.95):
+        x = np.array([[5.0, 5.0, 5.0], [5.0 + r * 0.6, 5.0 + r * 0.8, 5.0]])
+        q = np.array([0.3, -0.7])
+        out = np.zeros(2 * 96)
+        O.orc().orc_descriptors(x.reshape(-1).copy(), q, 2, cell, out)
+        d0 = out[:96]
+        u = r / (8.0 / 23.0)
+        k0, fr = int(np.floor(u)), u - np.floor(u)
+        fc = 0.5 * (1 + np.cos(np.pi * r / 8.0))
+        expect = np.zeros(24)
+        expect[k0] = q[1] * fc * (1 - fr)
+        if k0 + 1 < 24:
+            expect[k0 + 1] = q[1] * fc * fr
+        assert np.abs(d0[:24] - expect).max() < 1e-12
+        assert np.abs(d0[24:48] - 0.6 * expect).max() < 1e-12 and np.abs(d0[48:72] - 0.8 * expect).max() < 1e-12
+        assert np.abs(d0[72:]).max() < 1e-12
+
+
+def test_descriptor_sums_are_the_cutoff_weighted_charges():
+    # the hat functions sum to one on [0, 8): Σ_k scalar channel = Σ_j q_j f(r_ij), Σ_k vector channel = Σ_j q_j f d̂
+    rng = np.random.default_rng(2)
+    L = 25.0
+    n = 150
+    x = rng.uniform(0, L, (n, 3))
+    q = rng.normal(0, 0.3, n)
+    cell = np.array([0, 0, 0, L, 0, 0, 0, L, 0, 0, 0, L], float)
+    out = np.zeros(n * 96)
+    O.orc().orc_descriptors(x.reshape(-1).copy(), q, n, cell, out)
+    D = out.reshape(n, 96)
+    for i in (0, 17, 99):
+        d = x - x[i]
+        d -= L * np.round(d / L)
+        r = np.linalg.norm(d, axis=1)
+        m = (r > 1e-3) & (r < 8.0)
+        fc = 0.5 * (1 + np.cos(np.pi * r[m] / 8.0))
+        assert D[i, :24].sum() == pytest.approx((q[m] * fc).sum(), abs=1e-12)
+        for c in range(3):
+            assert D[i, 24 + 24 * c:48 + 24 * c].sum() == pytest.approx((q[m] * fc * d[m, c] / r[m]).sum(), abs=1e-12)
