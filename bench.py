@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline")
+    ap.add_argument("--learned-train", type=int, default=2000, help="PME frames the learned r-RESPA network trains on (-1: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d asked)" % a.gpus, file=sys.stderr)
@@ -137,7 +138,7 @@ def main():
     dyn.steps = a.steps
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rep = ca.run_dynamics(system, dyn, evaluator=ev)
+    ca.run_dynamics(system, dyn, evaluator=ev)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     kernels = ev.profile_kernels()
@@ -168,9 +169,29 @@ def main():
     out["pme"] = {"ms_per_eval": pme_ms, "algorithmic_bytes": pme_bytes(grid, n),
                   "achieved_GBs": pme_bytes(grid, n) / (pme_ms * 1e-3) / 1e9 if pme_ms else None}
     out["atom_steps_per_s"] = n * a.steps / secs
-    if not a.no_cpu:
-        print("bench: timing the reference CPU path", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(system)
+    # learned r-RESPA on the same step: full PME every 4 steps, the network's predicted increment in between, after
+    # training on PME frames; judged at every full step against PME and against the stale force
+    if a.learned_train >= 0:
+        print("bench: learned r-RESPA (%d training frames)" % a.learned_train, file=sys.stderr, flush=True)
+        lo = ca.DynamicsOptions(dt=DT, temperature=TEMP, thermostat="langevin", tau_t=TAU, seed=SEED, new_velocities=False,
+                                thermo_every=4, energy=dyn.energy, learned_k=4, learned_tol=1e-3, learned_train=a.learned_train)
+        lo.steps = a.learned_train + 4
+        ca.run_dynamics(system, lo, evaluator=ev)
+        lo.steps, lo.learned_train = a.steps, 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = ca.run_dynamics(system, lo, evaluator=ev)
+        torch.cuda.synchronize()
+        lsecs = time.perf_counter() - t0
+        judged = [r for r in rep.thermo if r["learned_rmse"] > 0]
+        out["learned_respa"] = {
+            "ns_per_day": DT * 1e-6 * 86400.0 / (lsecs / a.steps), "ms_per_step": 1e3 * lsecs / a.steps, "k": 4,
+            "training_frames": a.learned_train, "learned_steps": rep.learned_steps, "full_steps": rep.full_steps,
+            "fallbacks": rep.fallbacks,
+            "force_rmse_learned": float(np.mean([r["learned_rmse"] for r in judged])) if judged else None,
+            "force_rmse_stale": float(np.mean([r["learned_rmse_stale"] for r in judged])) if judged else None,
+            "rmse_definition": "|F_slow predicted - F_slow PME| / |F total| over all atoms at each full step",
+            "speedup_vs_plain": (secs / lsecs)}
     print(json.dumps(out))
 
 

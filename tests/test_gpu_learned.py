@@ -105,6 +105,8 @@ def test_learned_respa_runs_and_judges_itself():
     print("judged relative force RMSE: learned %.3e, stale %.3e (mean of %d judgements)" % (lr.mean(), st.mean(), len(lr)), flush=True)
     assert r.learned_steps > 0 and r.full_steps >= 200 and len(lr) > 10
     assert np.isfinite(s.x).all() and np.isfinite(lr).all()
+    # what the network gains over the method without it: a smaller slow-force error than the stale force
+    assert lr.mean() < st.mean()
     # with a tolerance below every judged error the run falls back to plain PME
     s2 = H.melt_case(chains=16)
     r2 = ca.run_dynamics(s2, ca.DynamicsOptions(dt=0.5, steps=60, temperature=300, thermostat="langevin", seed=3, thermo_every=4,
