@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "mlp.hpp"
@@ -75,17 +76,17 @@ __global__ void k_cell_fill(int n, const int* __restrict__ cell, const int64_t* 
   catoms[cstart[c] + atomicAdd(&cfill[c], 1u)] = uint32_t(i);
 }
 
-// atoms of each cell in index order, so the descriptor sums have a fixed order
-__global__ void k_cell_sort(int ncells, const int64_t* __restrict__ cstart, uint32_t* __restrict__ catoms) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
+// atoms of each cell in index order, so the descriptor sums have a fixed order: each atom's rank among its cell's
+// atoms (filled unordered by k_cell_fill into tmp) gives its place
+__global__ void k_cell_rank(int n, const int* __restrict__ cell, const int64_t* __restrict__ cstart, const uint32_t* __restrict__ tmp,
+                            uint32_t* __restrict__ catoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell[i];
   const int64_t b = cstart[c], e = cstart[c + 1];
-  for (int64_t k = b + 1; k < e; ++k) {
-    const uint32_t v = catoms[k];
-    int64_t j = k;
-    while (j > b && catoms[j - 1] > v) catoms[j] = catoms[j - 1], --j;
-    catoms[j] = v;
-  }
+  int64_t r = 0;
+  for (int64_t k = b; k < e; ++k) r += tmp[k] < uint32_t(i) ? 1 : 0;
+  catoms[b + r] = uint32_t(i);
 }
 
 // 64 atoms per workgroup, each lane's 96 sums in its own LDS row (odd stride: no bank conflicts)
@@ -95,7 +96,9 @@ __global__ void __launch_bounds__(64) k_desc(DescGeom G, int n, const float4* __
                                              const uint32_t* __restrict__ catoms, float* __restrict__ out) {
   __shared__ float acc[64 * kDescRow];
   const int lane = threadIdx.x;
-  const int i = blockIdx.x * 64 + lane;
+  // lanes take the atoms in cell order, so a wavefront's lanes walk the same neighbour cells and read the same atoms
+  const int slot = blockIdx.x * 64 + lane;
+  const int i = slot < n ? int(catoms[slot]) : n;
   float* A = acc + lane * kDescRow;
   for (int k = 0; k < kIn; ++k) A[k] = 0.0f;
   if (i < n) {
@@ -295,7 +298,7 @@ __global__ void k_slow_estimate(int n, const double* __restrict__ f0, const floa
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int c = 0; c < kOut; ++c)
-    fest[3 * size_t(i) + c] = f0[3 * size_t(i) + c] + q[i] * double(Y[size_t(i) * kOutPad + c] - Yz[c]) * e_scale;
+    fest[3 * size_t(i) + c] = f0[3 * size_t(i) + c] + q[i] * double(Y[size_t(i) * kOut + c] - Yz[c]) * e_scale;
 }
 
 // Σ |a − b|², Σ |c|² per block (the relative force RMSE of a prediction a of b against the full force c)
@@ -344,6 +347,97 @@ __global__ void k_feature_scale(int n, const float* __restrict__ D, const float*
   }
 }
 
+// the network on every atom's input (D − D0)·s, fused: one workgroup of four wavefronts per 128 atoms, the weights in
+// LDS, the hidden layers kept on chip (BF16 through a per-wavefront LDS tile between layers); Y [n][3] FP32 without b3
+constexpr int kInferWaves = 4, kW1Row = kIn + 8, kWRow = kHid + 8;
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* __restrict__ D, const float* __restrict__ D0, const float* __restrict__ sc,
+                                                   const __bf16* __restrict__ W1, const float* __restrict__ b1, const __bf16* __restrict__ W2,
+                                                   const float* __restrict__ b2, const __bf16* __restrict__ W3, float* __restrict__ Y) {
+  __shared__ __bf16 sW1[kHid * kW1Row];
+  __shared__ __bf16 sW2[kHid * kWRow];
+  __shared__ __bf16 sW3[kOutPad * kWRow];
+  __shared__ __bf16 sH[kInferWaves][32 * kWRow];
+  __shared__ float sb1[kHid], sb2[kHid], ssc[kIn];
+  for (int e = threadIdx.x; e < kHid * kIn; e += blockDim.x) sW1[(e / kIn) * kW1Row + e % kIn] = W1[e];
+  for (int e = threadIdx.x; e < kHid * kHid; e += blockDim.x) sW2[(e / kHid) * kWRow + e % kHid] = W2[e];
+  for (int e = threadIdx.x; e < kOutPad * kHid; e += blockDim.x) sW3[(e / kHid) * kWRow + e % kHid] = W3[e];
+  for (int e = threadIdx.x; e < kHid; e += blockDim.x) sb1[e] = b1[e], sb2[e] = b2[e];
+  for (int e = threadIdx.x; e < kIn; e += blockDim.x) ssc[e] = sc[e];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = (blockIdx.x * kInferWaves + wave) * 32;
+  if (r0 >= n) return;
+  const int lr = lane & 31, kh = (lane >> 5) * 8;
+  const int row = r0 + lr;
+  __bf16* H = sH[wave];
+  f32x16 acc[4];
+  // layer 1: Z1 = X W1ᵀ with X = (D − D0)·s formed on the fly
+  for (int jt = 0; jt < 4; ++jt)
+    for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+  for (int k0 = 0; k0 < kIn; k0 += 16) {
+    bf16x8 a;
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 + kh + t;
+      a[t] = row < n ? to_bf16((D[size_t(row) * kIn + k] - D0[size_t(row) * kIn + k]) * ssc[k]) : to_bf16(0.0f);
+    }
+    for (int jt = 0; jt < 4; ++jt) {
+      bf16x8 b;
+      const __bf16* w = sW1 + (jt * 32 + lr) * kW1Row + k0 + kh;
+      for (int t = 0; t < 8; ++t) b[t] = w[t];
+      acc[jt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[jt], 0, 0, 0);
+    }
+  }
+  for (int jt = 0; jt < 4; ++jt)
+    for (int q = 0; q < 4; ++q)
+      for (int t = 0; t < 4; ++t) {
+        const int rr = q * 8 + (lane >> 5) * 4 + t, col = jt * 32 + lr;
+        const float z = acc[jt][q * 4 + t] + sb1[col];
+        H[rr * kWRow + col] = to_bf16(z / (1.0f + __expf(-z)));
+      }
+  wave_lds_sync();
+  // layer 2: Z2 = H1 W2ᵀ
+  for (int jt = 0; jt < 4; ++jt)
+    for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+  for (int k0 = 0; k0 < kHid; k0 += 16) {
+    bf16x8 a;
+    for (int t = 0; t < 8; ++t) a[t] = H[lr * kWRow + k0 + kh + t];
+    for (int jt = 0; jt < 4; ++jt) {
+      bf16x8 b;
+      const __bf16* w = sW2 + (jt * 32 + lr) * kWRow + k0 + kh;
+      for (int t = 0; t < 8; ++t) b[t] = w[t];
+      acc[jt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[jt], 0, 0, 0);
+    }
+  }
+  wave_lds_sync();
+  for (int jt = 0; jt < 4; ++jt)
+    for (int q = 0; q < 4; ++q)
+      for (int t = 0; t < 4; ++t) {
+        const int rr = q * 8 + (lane >> 5) * 4 + t, col = jt * 32 + lr;
+        const float z = acc[jt][q * 4 + t] + sb2[col];
+        H[rr * kWRow + col] = to_bf16(z / (1.0f + __expf(-z)));
+      }
+  wave_lds_sync();
+  // layer 3: Y = H2 W3ᵀ (one 32x32 tile, 3 columns used)
+  f32x16 y;
+  for (int r = 0; r < 16; ++r) y[r] = 0.0f;
+  for (int k0 = 0; k0 < kHid; k0 += 16) {
+    bf16x8 a, b;
+    for (int t = 0; t < 8; ++t) a[t] = H[lr * kWRow + k0 + kh + t], b[t] = sW3[lr * kWRow + k0 + kh + t];
+    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, y, 0, 0, 0);
+  }
+  if (lr < kOut)
+    for (int q = 0; q < 4; ++q)
+      for (int t = 0; t < 4; ++t) {
+        const int rr = r0 + q * 8 + (lane >> 5) * 4 + t;
+        if (rr < n) Y[size_t(rr) * kOut + lr] = y[q * 4 + t];
+      }
+}
+
 }  // namespace
 
 // ---- host side ----------------------------------------------------------------------------------------------------------
@@ -362,14 +456,15 @@ struct Learned {
   // descriptors and cells
   DevBuf<float4> xw;
   DevBuf<int> cell;
-  DevBuf<uint32_t> ccount, cfill, catoms;
+  DevBuf<uint32_t> ccount, cfill, catoms, ctmp;
   DevBuf<int64_t> cstart;
   DevBuf<float> sc, Dcur, D0;
   std::vector<DevBuf<float>> Dring;
   std::vector<DevBuf<double>> Fring;
   int ring_n = 0;
   // inference and training activations
-  DevBuf<float> Z1, Z2, Y, Yz, dsilu1, dsilu2, Tt, dYf, dH, dZf, part;
+  DevBuf<float> Z1, Z2, Y, Yz, Yi, dsilu1, dsilu2, Tt, dYf, dH, dZf, part;
+  std::vector<uint32_t> charged;   // the atoms with a charge (the training pool)
   DevBuf<__bf16> H1, H2, X, Xs, dYb, dZb;
   DevBuf<uint32_t> sidx;
   DevBuf<double> f0, fslow, fest, ftot, epart;
@@ -436,10 +531,11 @@ void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
   L.cfill.resize(size_t(ncells));
   L.cfill.zero(s);
   L.catoms.resize(size_t(n));
-  hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.cfill.p, L.catoms.p);
+  L.ctmp.resize(size_t(n));
+  hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.cfill.p, L.ctmp.p);
   LAUNCHCHECK();
-  kt_begin(c, "k_cell_sort");
-  hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(ncells, 64)), dim3(64), 0, s, ncells, L.cstart.p, L.catoms.p);
+  kt_begin(c, "k_cell_rank");
+  hipLaunchKernelGGL(k_cell_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.ctmp.p, L.catoms.p);
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_desc");
@@ -580,10 +676,10 @@ void train_on_pair(capsgpu_ctx& c, Learned& L, const float* Dold, const double* 
     L.e_scale = cnt && acc > 0 ? std::sqrt(acc / double(cnt)) : 1.0;
     L.scaled = true;
   }
-  std::vector<uint32_t> pool;
-  pool.reserve(size_t(c.n));
-  for (int i = 0; i < c.n; ++i)
-    if (std::fabs(c.charge[size_t(i)]) > 1e-3) pool.push_back(uint32_t(i));
+  if (L.charged.empty())
+    for (int i = 0; i < c.n; ++i)
+      if (std::fabs(c.charge[size_t(i)]) > 1e-3) L.charged.push_back(uint32_t(i));
+  const std::vector<uint32_t>& pool = L.charged;
   const int M = std::min<int>(L.samples, int(pool.size())) / 32 * 32;
   if (M < 32) return;
   std::vector<uint32_t> pick(static_cast<size_t>(M));
@@ -613,13 +709,13 @@ double rel_rmse(capsgpu_ctx& c, Learned& L, const double* pred, const double* tr
 
 // the slow-force estimate at the current descriptors from the reference point (D0, f0) into L.fest
 void estimate(capsgpu_ctx& c, Learned& L) {
-  L.X.resize(size_t(c.n) * kIn);
-  hipLaunchKernelGGL(k_delta_in, dim3(blocks_for(size_t(c.n) * kIn, 256)), dim3(256), 0, c.stream, c.n, nullptr, L.Dcur.p, L.D0.p, L.sc.p,
-                     L.X.p);
+  L.Yi.resize(size_t(c.n) * kOut);
+  kt_begin(c, "k_mlp_infer");
+  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(c.n, 32 * kInferWaves)), dim3(64 * kInferWaves), 0, c.stream, c.n, L.Dcur.p, L.D0.p, L.sc.p,
+                     L.w1b.p, L.b1.p, L.w2b.p, L.b2.p, L.w3b.p, L.Yi.p);
   LAUNCHCHECK();
-  L.Y.resize(size_t(c.n) * kOutPad);
-  forward(c, L, c.n, L.X.p, L.Y.p, false);
-  hipLaunchKernelGGL(k_slow_estimate, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, L.f0.p, L.Y.p, L.Yz.p, c.d_charge.p,
+  kt_end(c);
+  hipLaunchKernelGGL(k_slow_estimate, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, L.f0.p, L.Yi.p, L.Yz.p, c.d_charge.p,
                      float(L.e_scale), L.fest.p);
   LAUNCHCHECK();
 }
@@ -728,8 +824,8 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
       train_on_pair(c, L, L.D0.p, L.f0.p);
     }
     ++L.frames;
-    // the reference point of the next interval
-    HIPCHECK(hipMemcpyAsync(L.D0.p, L.Dcur.p, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
+    // the reference point of the next interval (the current descriptors become D0; Dcur is rewritten every step)
+    std::swap(L.D0, L.Dcur);
     HIPCHECK(hipMemcpyAsync(L.f0.p, L.fslow.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     const bool was = L.learned_mode;
     L.learned_mode = L.frames >= L.train_frames && L.scaled && (training || L.rmse <= L.tol);
@@ -777,11 +873,22 @@ void learned_forward_host(capsgpu_ctx& c, const float* params, const float* x, i
   L.Xs.resize(xp.size());
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(xp.size(), 256)), dim3(256), 0, s, int64_t(xp.size()), xf.p, L.Xs.p);
   LAUNCHCHECK();
+  // the production inference path: the fused kernel on D = x, D0 = 0 and unit scales
+  DevBuf<float> zero, ones;
+  zero.resize(xp.size());
+  zero.zero(s);
+  std::vector<float> one(kIn, 1.0f);
+  ones.upload(one, s);
+  L.Yi.resize(size_t(M) * kOut);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(M, 32 * kInferWaves)), dim3(64 * kInferWaves), 0, s, M, xf.p, zero.p, ones.p, L.w1b.p,
+                     L.b1.p, L.w2b.p, L.b2.p, L.w3b.p, L.Yi.p);
+  LAUNCHCHECK();
+  std::vector<float> yh = L.Yi.to_host(s);
+  for (int i = 0; i < m; ++i)
+    for (int k = 0; k < kOut; ++k) y[size_t(i) * kOut + k] = yh[size_t(i) * kOut + k] + b3[size_t(k)];
+  // the training path's forward on the same rows (kept in L.Y) for the gradient check
   L.Y.resize(size_t(M) * kOutPad);
   forward(c, L, M, L.Xs.p, L.Y.p, false);
-  std::vector<float> yh = L.Y.to_host(s);
-  for (int i = 0; i < m; ++i)
-    for (int k = 0; k < kOut; ++k) y[size_t(i) * kOut + k] = yh[size_t(i) * kOutPad + k] + b3[size_t(k)];
 }
 
 // gradients of L = ½ Σ (y − t)² / m at the given parameters (the layout of the forward call), into grad (same layout)
