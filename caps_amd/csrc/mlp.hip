@@ -44,8 +44,8 @@ struct DescGeom {
   int nc[3];
 };
 
-__global__ void k_cell_bin(DescGeom G, int n, const double* __restrict__ x, float4* __restrict__ xw, int* __restrict__ cell,
-                           uint32_t* __restrict__ ccount) {
+__global__ void k_cell_bin(DescGeom G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ xw,
+                           int* __restrict__ cell, uint32_t* __restrict__ ccount) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double f[3];
@@ -61,7 +61,7 @@ __global__ void k_cell_bin(DescGeom G, int n, const double* __restrict__ x, floa
   w.x = float(G.o[0] + G.a[0] * f[0] + G.b[0] * f[1] + G.c[0] * f[2]);
   w.y = float(G.o[1] + G.a[1] * f[0] + G.b[1] * f[1] + G.c[1] * f[2]);
   w.z = float(G.o[2] + G.a[2] * f[0] + G.b[2] * f[1] + G.c[2] * f[2]);
-  w.w = 0.0f;
+  w.w = float(q[i]);
   xw[i] = w;
   const int c = (ci[2] * G.nc[1] + ci[1]) * G.nc[0] + ci[0];
   cell[i] = c;
@@ -77,9 +77,9 @@ __global__ void k_cell_fill(int n, const int* __restrict__ cell, const int64_t* 
 }
 
 // atoms of each cell in index order, so the descriptor sums have a fixed order: each atom's rank among its cell's
-// atoms (filled unordered by k_cell_fill into tmp) gives its place
+// atoms (filled unordered by k_cell_fill into tmp) gives its place; the positions (q in .w) are copied in that order
 __global__ void k_cell_rank(int n, const int* __restrict__ cell, const int64_t* __restrict__ cstart, const uint32_t* __restrict__ tmp,
-                            uint32_t* __restrict__ catoms) {
+                            const float4* __restrict__ xw, uint32_t* __restrict__ catoms, float4* __restrict__ xs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = cell[i];
@@ -87,56 +87,162 @@ __global__ void k_cell_rank(int n, const int* __restrict__ cell, const int64_t* 
   int64_t r = 0;
   for (int64_t k = b; k < e; ++k) r += tmp[k] < uint32_t(i) ? 1 : 0;
   catoms[b + r] = uint32_t(i);
+  xs[b + r] = xw[i];
 }
 
-// 64 atoms per workgroup, each lane's 96 sums in its own LDS row (odd stride: no bank conflicts)
-constexpr int kDescRow = kIn + 1;
-__global__ void __launch_bounds__(64) k_desc(DescGeom G, int n, const float4* __restrict__ xw, const double* __restrict__ q,
-                                             const int* __restrict__ cell, const int64_t* __restrict__ cstart,
-                                             const uint32_t* __restrict__ catoms, float* __restrict__ out) {
-  __shared__ float acc[64 * kDescRow];
+// the smooth cut-off ½(1 + cos(π r/8 Å)) as ½(1 + sin z), z = π(½ − r/8 Å) ∈ (−π/2, π/2], by the sine's Taylor series
+// to z¹¹ (truncation below 6e-8 on that interval)
+__device__ inline float desc_cutoff(float r) {
+  const float z = 3.14159265f * (0.5f - r * (1.0f / kRcut)), z2 = z * z;
+  float p = -1.0f / 39916800.0f;
+  p = __builtin_fmaf(p, z2, 1.0f / 362880.0f);
+  p = __builtin_fmaf(p, z2, -1.0f / 5040.0f);
+  p = __builtin_fmaf(p, z2, 1.0f / 120.0f);
+  p = __builtin_fmaf(p, z2, -1.0f / 6.0f);
+  p = __builtin_fmaf(p, z2, 1.0f);
+  return __builtin_fmaf(0.5f * z, p, 0.5f);
+}
+
+// 64 atoms (in cell order) per workgroup, one per lane, each lane's 96 sums in its own LDS row (odd stride: no bank
+// conflicts). A lane walks the 27 cells around its own, reading the cell-ordered positions; the neighbours within 8 Å
+// go into the lane's queue in LDS (cell-ordered index | image code << 27), and whenever a lane's queue may not hold
+// another batch of candidates the wave's active lanes empty theirs together, so the bin updates run only for pairs
+// within range. Queued pairs are
+// summed in the order they were found: a fixed order.
+constexpr int kDescRow = kIn + 1, kDescQueue = 32, kDescBatch = 8;
+template <bool kRegAcc>
+__global__ void __launch_bounds__(64) k_desc(DescGeom G, int n, const float4* __restrict__ xs, const int* __restrict__ cell,
+                                             const int64_t* __restrict__ cstart, const uint32_t* __restrict__ catoms, float* __restrict__ out) {
+  __shared__ float acc[kRegAcc ? 1 : 64 * kDescRow];
+  __shared__ uint32_t que[kDescQueue * 64];
+  __shared__ float4 shift[27];
   const int lane = threadIdx.x;
-  // lanes take the atoms in cell order, so a wavefront's lanes walk the same neighbour cells and read the same atoms
+  if (lane < 27) {
+    // the lattice vector of image code w = (sx + 1) + 3 (sy + 1) + 9 (sz + 1)
+    const int sx = lane % 3 - 1, sy = (lane / 3) % 3 - 1, sz = lane / 9 - 1;
+    shift[lane] = make_float4(float(sx * G.a[0] + sy * G.b[0] + sz * G.c[0]), float(sx * G.a[1] + sy * G.b[1] + sz * G.c[1]),
+                              float(sx * G.a[2] + sy * G.b[2] + sz * G.c[2]), 0.0f);
+  }
   const int slot = blockIdx.x * 64 + lane;
-  const int i = slot < n ? int(catoms[slot]) : n;
-  float* A = acc + lane * kDescRow;
-  for (int k = 0; k < kIn; ++k) A[k] = 0.0f;
-  if (i < n) {
-    const float4 xi = xw[i];
-    const int c = cell[i];
-    const int cx = c % G.nc[0], cy = (c / G.nc[0]) % G.nc[1], cz = c / (G.nc[0] * G.nc[1]);
-    const float dlt = kRcut / float(kBins - 1);
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          int jx = cx + dx, jy = cy + dy, jz = cz + dz;
-          // the lattice vector of the image the wrapped cell stands for
-          const int sx = jx < 0 ? -1 : jx >= G.nc[0] ? 1 : 0, sy = jy < 0 ? -1 : jy >= G.nc[1] ? 1 : 0, sz = jz < 0 ? -1 : jz >= G.nc[2] ? 1 : 0;
-          jx -= sx * G.nc[0], jy -= sy * G.nc[1], jz -= sz * G.nc[2];
-          const float shx = float(sx * G.a[0] + sy * G.b[0] + sz * G.c[0]);
-          const float shy = float(sx * G.a[1] + sy * G.b[1] + sz * G.c[1]);
-          const float shz = float(sx * G.a[2] + sy * G.b[2] + sz * G.c[2]);
-          const int h = (jz * G.nc[1] + jy) * G.nc[0] + jx;
-          for (int64_t k = cstart[h]; k < cstart[h + 1]; ++k) {
-            const uint32_t j = catoms[k];
-            const float4 xj = xw[j];
-            const float ex = xj.x + shx - xi.x, ey = xj.y + shy - xi.y, ez = xj.z + shz - xi.z;
-            const float r2 = ex * ex + ey * ey + ez * ez;
-            if (r2 >= kRcut * kRcut || r2 < 1e-6f) continue;
-            const float r = sqrtf(r2), ir = 1.0f / r;
-            const float fc = 0.5f * (1.0f + cosf(3.14159265f * r / kRcut));
-            const float u = r / dlt;
-            const int b0 = min(int(u), kBins - 2);
-            const float w1 = u - float(b0), w0 = 1.0f - w1;
-            const float qf = float(q[j]) * fc;
-            const float v0 = qf * w0, v1 = qf * w1;
-            A[b0] += v0, A[b0 + 1] += v1;
-            A[kBins + b0] += v0 * ex * ir, A[kBins + b0 + 1] += v1 * ex * ir;
-            A[2 * kBins + b0] += v0 * ey * ir, A[2 * kBins + b0 + 1] += v1 * ey * ir;
-            A[3 * kBins + b0] += v0 * ez * ir, A[3 * kBins + b0 + 1] += v1 * ez * ir;
-          }
-        }
-    for (int k = 0; k < kIn; ++k) out[size_t(i) * kIn + k] = A[k];
+  const bool live = slot < n;
+  float* A = acc + (kRegAcc ? 0 : lane * kDescRow);
+  float R[kRegAcc ? kIn : 1];
+  if (kRegAcc) {
+#pragma unroll
+    for (int k = 0; k < (kRegAcc ? kIn : 1); ++k) R[k] = 0.0f;
+  } else {
+    for (int k = 0; k < kIn; ++k) A[k] = 0.0f;
+  }
+  __syncthreads();
+  const float4 xi = live ? xs[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const int c = live ? cell[catoms[slot]] : 0;
+  const int cx = c % G.nc[0], cy = (c / G.nc[0]) % G.nc[1], cz = c / (G.nc[0] * G.nc[1]);
+  const float idlt = float(kBins - 1) / kRcut;
+  int cnt = 0;
+  // one queued pair into the lane's sums
+  auto accumulate = [&](const float4& xj, const float4& sh) __attribute__((always_inline)) {
+    const float ex = xj.x + sh.x - xi.x, ey = xj.y + sh.y - xi.y, ez = xj.z + sh.z - xi.z;
+    const float r2 = ex * ex + ey * ey + ez * ez;
+    const float r = sqrtf(r2), ir = 1.0f / r;
+    const float u = r * idlt;
+    const float qf = xj.w * desc_cutoff(r);
+    const float hx = ex * ir, hy = ey * ir, hz = ez * ir;
+    if (kRegAcc) {
+      // every bin's hat weight max(0, 1 − |u − b|): two are non-zero
+      const float g1 = qf * hx, g2 = qf * hy, g3 = qf * hz;
+#pragma unroll
+      for (int b = 0; b < (kRegAcc ? kBins : 0); ++b) {
+        const float wb = fmaxf(0.0f, 1.0f - fabsf(u - float(b)));
+        R[b] = __builtin_fmaf(wb, qf, R[b]);
+        R[kBins + b] = __builtin_fmaf(wb, g1, R[kBins + b]);
+        R[2 * kBins + b] = __builtin_fmaf(wb, g2, R[2 * kBins + b]);
+        R[3 * kBins + b] = __builtin_fmaf(wb, g3, R[3 * kBins + b]);
+      }
+    } else {
+      const int b0 = min(int(u), kBins - 2);
+      const float w1 = u - float(b0), w0 = 1.0f - w1;
+      const float v0 = qf * w0, v1 = qf * w1;
+      A[b0] += v0, A[b0 + 1] += v1;
+      A[kBins + b0] += v0 * hx, A[kBins + b0 + 1] += v1 * hx;
+      A[2 * kBins + b0] += v0 * hy, A[2 * kBins + b0 + 1] += v1 * hy;
+      A[3 * kBins + b0] += v0 * hz, A[3 * kBins + b0 + 1] += v1 * hz;
+    }
+  };
+  // the queued pairs of the active lanes into their sums, in queue order; four positions loaded ahead at a time
+  auto drain = [&]() __attribute__((always_inline)) {
+    for (int t = 0; t < cnt; t += 4) {
+      float4 xj[4], sh[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t e = que[min(t + u, cnt - 1) * 64 + lane];
+        xj[u] = xs[e & 0x7ffffffu];
+        sh[u] = shift[e >> 27];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (t + u < cnt) accumulate(xj[u], sh[u]);
+    }
+    cnt = 0;
+  };
+  // the candidates [kb, ke) of image code w, a batch of positions loaded while the previous one is tested (indexes
+  // past the range's end repeat its last atom and are not taken)
+  auto scan = [&](int64_t kb, int64_t ke, uint32_t w) __attribute__((always_inline)) {
+    if (kb >= ke) return;
+    const float4 sh = shift[w];
+    float4 cur[kDescBatch], nxt[kDescBatch];
+#pragma unroll
+    for (int u = 0; u < kDescBatch; ++u) cur[u] = xs[kb + u < ke ? kb + u : ke - 1];
+    for (int64_t k0 = kb; k0 < ke; k0 += kDescBatch) {
+#pragma unroll
+      for (int u = 0; u < kDescBatch; ++u) nxt[u] = xs[k0 + kDescBatch + u < ke ? k0 + kDescBatch + u : ke - 1];
+      if (__any(cnt > kDescQueue - kDescBatch)) drain();
+#pragma unroll
+      for (int u = 0; u < kDescBatch; ++u) {
+        const float ex = cur[u].x + sh.x - xi.x, ey = cur[u].y + sh.y - xi.y, ez = cur[u].z + sh.z - xi.z;
+        const float r2 = ex * ex + ey * ey + ez * ez;
+        if (k0 + u < ke && r2 < kRcut * kRcut && r2 >= 1e-6f) que[(cnt++) * 64 + lane] = uint32_t(k0 + u) | (w << 27);
+      }
+#pragma unroll
+      for (int u = 0; u < kDescBatch; ++u) cur[u] = nxt[u];
+    }
+  };
+  // the 27 cells around the lane's own as 9 rows along a; a row's cells are consecutive in the cell order (one range)
+  // unless the row wraps at the cell's a faces (two ranges with different images)
+  if (live)
+    for (int row = 0; row < 9; ++row) {
+      int jy = cy + row % 3 - 1, jz = cz + row / 3 - 1;
+      const int sy = jy < 0 ? -1 : jy >= G.nc[1] ? 1 : 0, sz = jz < 0 ? -1 : jz >= G.nc[2] ? 1 : 0;
+      jy -= sy * G.nc[1], jz -= sz * G.nc[2];
+      const int base = (jz * G.nc[1] + jy) * G.nc[0], nx = G.nc[0];
+      const uint32_t wyz = uint32_t(3 * (sy + 1) + 9 * (sz + 1));
+      for (int part = 0; part < 2; ++part) {
+        // cx = 0: the last cell of the row (image −a), then cells 0, 1; cx = nx − 1: cells cx − 1, cx, then cell 0
+        // (image +a); otherwise cells cx − 1 … cx + 1
+        int c0, c1;
+        uint32_t w = wyz + 1u;
+        if (cx == 0) c0 = part ? 0 : nx - 1, c1 = part ? 2 : nx, w = part ? wyz + 1u : wyz;
+        else if (cx == nx - 1) c0 = part ? 0 : cx - 1, c1 = part ? 1 : nx, w = part ? wyz + 2u : wyz + 1u;
+        else c0 = cx - 1, c1 = part ? cx - 1 : cx + 2;
+        scan(cstart[base + c0], cstart[base + c1], w);
+      }
+    }
+  drain();
+  if (kRegAcc) {
+    // each lane's row straight out, 16 bytes at a time
+    if (live) {
+      float4* o = reinterpret_cast<float4*>(out + size_t(catoms[slot]) * kIn);
+#pragma unroll
+      for (int k = 0; k < (kRegAcc ? kIn / 4 : 0); ++k) o[k] = make_float4(R[4 * k], R[4 * k + 1], R[4 * k + 2], R[4 * k + 3]);
+    }
+  } else {
+    // the rows out, one atom at a time across the wave (coalesced)
+    __syncthreads();
+    for (int l = 0; l < 64; ++l) {
+      const int sl = blockIdx.x * 64 + l;
+      if (sl >= n) break;
+      const size_t i = catoms[sl];
+      for (int k = lane; k < kIn; k += 64) out[i * kIn + k] = acc[l * kDescRow + k];
+    }
   }
 }
 
@@ -255,13 +361,19 @@ __global__ void k_mul_dsilu(int64_t n, const float* __restrict__ dH, const float
   dZf[e] = v;
 }
 
-// column sums of an M x N FP32 matrix (bias gradients), one thread per column, rows in order
-__global__ void k_colsum_f(int M, int N, const float* __restrict__ X, float* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  double s = 0;
-  for (int i = 0; i < M; ++i) s += X[size_t(i) * N + j];
-  out[j] = float(s);
+// column sums of an M x N FP32 matrix (bias gradients): one workgroup per column, a fixed-order tree over the rows
+__global__ void __launch_bounds__(256) k_colsum_f(int M, int N, const float* __restrict__ X, float* __restrict__ out) {
+  __shared__ float red[256];
+  const int j = blockIdx.x;
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < M; i += blockDim.x) s += X[size_t(i) * N + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights; the BF16 copy for the MFMA tiles
@@ -454,7 +566,7 @@ struct Learned {
   DevBuf<float> w1, b1, w2, b2, w3, b3, m1, m2, grad;
   DevBuf<__bf16> w1b, w2b, w3b;
   // descriptors and cells
-  DevBuf<float4> xw;
+  DevBuf<float4> xw, xs;
   DevBuf<int> cell;
   DevBuf<uint32_t> ccount, cfill, catoms, ctmp;
   DevBuf<int64_t> cstart;
@@ -507,10 +619,14 @@ void gemm(capsgpu_ctx& c, Learned& L, int M, int N, int K, const __bf16* A, int6
   LAUNCHCHECK();
   kt_end(c);
   if (splits > 1) {
+    kt_begin(c, "k_reduce_splits");
     hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(size_t(M) * N, 256)), dim3(256), 0, c.stream, M, N, splits, L.part.p, C, ldc);
     LAUNCHCHECK();
+    kt_end(c);
   }
 }
+
+constexpr bool kDescRegAcc = true;
 
 // raw descriptors of every atom at the device positions into out (n x 96, FP32)
 void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
@@ -518,12 +634,13 @@ void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
   hipStream_t s = c.stream;
   const DescGeom G = desc_geom(c);
   const int ncells = G.nc[0] * G.nc[1] * G.nc[2];
+  if (n >= (1 << 27)) throw Error(CAPSGPU_EINVAL, "learned r-RESPA takes fewer than 2^27 atoms");
   L.xw.resize(size_t(n));
   L.cell.resize(size_t(n));
   L.ccount.resize(size_t(ncells));
   L.ccount.zero(s);
   kt_begin(c, "k_cell_bin");
-  hipLaunchKernelGGL(k_cell_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, L.xw.p, L.cell.p, L.ccount.p);
+  hipLaunchKernelGGL(k_cell_bin, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, L.xw.p, L.cell.p, L.ccount.p);
   LAUNCHCHECK();
   kt_end(c);
   L.cstart.resize(size_t(ncells) + 1);
@@ -532,14 +649,18 @@ void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
   L.cfill.zero(s);
   L.catoms.resize(size_t(n));
   L.ctmp.resize(size_t(n));
+  L.xs.resize(size_t(n));
+  kt_begin(c, "k_cell_fill");
   hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.cfill.p, L.ctmp.p);
   LAUNCHCHECK();
+  kt_end(c);
   kt_begin(c, "k_cell_rank");
-  hipLaunchKernelGGL(k_cell_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.ctmp.p, L.catoms.p);
+  hipLaunchKernelGGL(k_cell_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.ctmp.p, L.xw.p, L.catoms.p, L.xs.p);
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_desc");
-  hipLaunchKernelGGL(k_desc, dim3(blocks_for(n, 64)), dim3(64), 0, s, G, n, L.xw.p, c.d_charge.p, L.cell.p, L.cstart.p, L.catoms.p, out);
+  if (kDescRegAcc) hipLaunchKernelGGL(k_desc<true>, dim3(blocks_for(n, 64)), dim3(64), 0, s, G, n, L.xs.p, L.cell.p, L.cstart.p, L.catoms.p, out);
+  else hipLaunchKernelGGL(k_desc<false>, dim3(blocks_for(n, 64)), dim3(64), 0, s, G, n, L.xs.p, L.cell.p, L.cstart.p, L.catoms.p, out);
   LAUNCHCHECK();
   kt_end(c);
 }
@@ -575,8 +696,10 @@ void forward(capsgpu_ctx& c, Learned& L, int M, const __bf16* X, float* Y, bool 
 // the network's output at zero input, subtracted from every prediction (an increment of zero gives zero)
 void zero_output(capsgpu_ctx& c, Learned& L) {
   L.X.resize(size_t(32) * kIn);
+  kt_begin(c, "k_delta_in");
   hipLaunchKernelGGL(k_delta_in, dim3(blocks_for(32 * kIn, 256)), dim3(256), 0, c.stream, 32, nullptr, nullptr, nullptr, nullptr, L.X.p);
   LAUNCHCHECK();
+  kt_end(c);
   L.Yz.resize(size_t(32) * kOutPad);
   forward(c, L, 32, L.X.p, L.Yz.p, false);
 }
@@ -590,8 +713,10 @@ void train_step(capsgpu_ctx& c, Learned& L, int M) {
   L.dYb.resize(size_t(M) * kOutPad);
   L.dYf.resize(size_t(M) * kOutPad);
   L.epart.resize(nb);
+  kt_begin(c, "k_out_grad");
   hipLaunchKernelGGL(k_out_grad, dim3(nb), dim3(256), 0, s, M, L.Y.p, L.b3.p, L.Tt.p, 1.0f / float(M), L.dYb.p, L.dYf.p, L.epart.p);
   LAUNCHCHECK();
+  kt_end(c);
   // gradient layout in L.grad: w1 (128·96), b1 (128), w2 (128·128), b2 (128), w3 (32·128), b3 (32)
   float* gw1 = L.grad.p;
   float* gb1 = gw1 + kHid * kIn;
@@ -601,29 +726,39 @@ void train_step(capsgpu_ctx& c, Learned& L, int M) {
   float* gb3 = gw3 + kOutPad * kHid;
   // dW3 = dYᵀ H2 (32 x 128, K = M); db3 = Σ dY
   gemm(c, L, kOutPad, kHid, M, L.dYb.p, 1, kOutPad, L.H2.p, kHid, 1, gw3, kHid);
-  hipLaunchKernelGGL(k_colsum_f, dim3(1), dim3(64), 0, s, M, kOutPad, L.dYf.p, gb3);
+  kt_begin(c, "k_colsum_f");
+  hipLaunchKernelGGL(k_colsum_f, dim3(kOutPad), dim3(256), 0, s, M, kOutPad, L.dYf.p, gb3);
   LAUNCHCHECK();
+  kt_end(c);
   // dH2 = dY W3 (M x 128, K = 32)
   L.dH.resize(size_t(M) * kHid);
   L.dZb.resize(size_t(M) * kHid);
   L.dZf.resize(size_t(M) * kHid);
   gemm(c, L, M, kHid, kOutPad, L.dYb.p, kOutPad, 1, L.w3b.p, kHid, 1, L.dH.p, kHid);
+  kt_begin(c, "k_mul_dsilu");
   hipLaunchKernelGGL(k_mul_dsilu, dim3(blocks_for(size_t(M) * kHid, 256)), dim3(256), 0, s, int64_t(M) * kHid, L.dH.p, L.dsilu2.p, L.dZb.p,
                      L.dZf.p);
   LAUNCHCHECK();
+  kt_end(c);
   // dW2 = dZ2ᵀ H1 (128 x 128, K = M); db2 = Σ dZ2
   gemm(c, L, kHid, kHid, M, L.dZb.p, 1, kHid, L.H1.p, kHid, 1, gw2, kHid);
-  hipLaunchKernelGGL(k_colsum_f, dim3(blocks_for(kHid, 64)), dim3(64), 0, s, M, kHid, L.dZf.p, gb2);
+  kt_begin(c, "k_colsum_f");
+  hipLaunchKernelGGL(k_colsum_f, dim3(kHid), dim3(256), 0, s, M, kHid, L.dZf.p, gb2);
   LAUNCHCHECK();
+  kt_end(c);
   // dH1 = dZ2 W2 (M x 128, K = 128)
   gemm(c, L, M, kHid, kHid, L.dZb.p, kHid, 1, L.w2b.p, kHid, 1, L.dH.p, kHid);
+  kt_begin(c, "k_mul_dsilu");
   hipLaunchKernelGGL(k_mul_dsilu, dim3(blocks_for(size_t(M) * kHid, 256)), dim3(256), 0, s, int64_t(M) * kHid, L.dH.p, L.dsilu1.p, L.dZb.p,
                      L.dZf.p);
   LAUNCHCHECK();
+  kt_end(c);
   // dW1 = dZ1ᵀ X (128 x 96, K = M); db1 = Σ dZ1
   gemm(c, L, kHid, kIn, M, L.dZb.p, 1, kHid, L.Xs.p, kIn, 1, gw1, kIn);
-  hipLaunchKernelGGL(k_colsum_f, dim3(blocks_for(kHid, 64)), dim3(64), 0, s, M, kHid, L.dZf.p, gb1);
+  kt_begin(c, "k_colsum_f");
+  hipLaunchKernelGGL(k_colsum_f, dim3(kHid), dim3(256), 0, s, M, kHid, L.dZf.p, gb1);
   LAUNCHCHECK();
+  kt_end(c);
   // Adam on every parameter block
   ++L.adam_t;
   const float b1 = 0.9f, b2 = 0.999f, c1 = 1.0f - std::pow(b1, float(L.adam_t)), c2 = 1.0f - std::pow(b2, float(L.adam_t));
@@ -661,8 +796,10 @@ void train_on_pair(capsgpu_ctx& c, Learned& L, const float* Dold, const double* 
   if (!L.scaled) {
     // the feature scales and the target scale from the first pair, so inputs and targets are O(1)
     L.sc.resize(kIn);
+    kt_begin(c, "k_feature_scale");
     hipLaunchKernelGGL(k_feature_scale, dim3(kIn), dim3(256), 0, s, c.n, L.Dcur.p, Dold, L.sc.p);
     LAUNCHCHECK();
+    kt_end(c);
     std::vector<double> f = L.fslow.to_host(s), g(3 * size_t(c.n));
     HIPCHECK(hipMemcpyAsync(g.data(), Fold, g.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -687,11 +824,15 @@ void train_on_pair(capsgpu_ctx& c, Learned& L, const float* Dold, const double* 
   L.sidx.upload(pick, s);
   L.Xs.resize(size_t(M) * kIn);
   L.Tt.resize(size_t(M) * kOut);
+  kt_begin(c, "k_delta_in");
   hipLaunchKernelGGL(k_delta_in, dim3(blocks_for(size_t(M) * kIn, 256)), dim3(256), 0, s, M, L.sidx.p, L.Dcur.p, Dold, L.sc.p, L.Xs.p);
   LAUNCHCHECK();
+  kt_end(c);
+  kt_begin(c, "k_gather_targets");
   hipLaunchKernelGGL(k_gather_targets, dim3(blocks_for(M, 256)), dim3(256), 0, s, M, L.sidx.p, L.fslow.p, Fold, c.d_charge.p,
                      float(1.0 / L.e_scale), L.Tt.p);
   LAUNCHCHECK();
+  kt_end(c);
   for (int it = 0; it < L.iters; ++it) train_step(c, L, M);
   zero_output(c, L);
 }
@@ -699,8 +840,10 @@ void train_on_pair(capsgpu_ctx& c, Learned& L, const float* Dold, const double* 
 double rel_rmse(capsgpu_ctx& c, Learned& L, const double* pred, const double* truth, const double* full) {
   const unsigned nb = blocks_for(c.n, 256);
   L.epart.resize(2 * size_t(nb));
+  kt_begin(c, "k_err_parts");
   hipLaunchKernelGGL(k_err_parts, dim3(nb), dim3(256), 0, c.stream, c.n, pred, truth, full, L.epart.p);
   LAUNCHCHECK();
+  kt_end(c);
   std::vector<double> h = L.epart.to_host(c.stream);
   double e = 0, t = 0;
   for (unsigned b = 0; b < nb; ++b) e += h[2 * b], t += h[2 * b + 1];
@@ -715,9 +858,11 @@ void estimate(capsgpu_ctx& c, Learned& L) {
                      L.w1b.p, L.b1.p, L.w2b.p, L.b2.p, L.w3b.p, L.Yi.p);
   LAUNCHCHECK();
   kt_end(c);
+  kt_begin(c, "k_slow_estimate");
   hipLaunchKernelGGL(k_slow_estimate, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, L.f0.p, L.Yi.p, L.Yz.p, c.d_charge.p,
                      float(L.e_scale), L.fest.p);
   LAUNCHCHECK();
+  kt_end(c);
 }
 
 void init_weights(capsgpu_ctx& c, Learned& L) {
@@ -808,8 +953,10 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
     if (L.last_full >= 0 && !training) {
       estimate(c, L);
       HIPCHECK(hipMemcpyAsync(L.ftot.p, c.f.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      kt_begin(c, "k_axpy");
       hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fslow.p, L.ftot.p);
       LAUNCHCHECK();
+      kt_end(c);
       L.rmse = rel_rmse(c, L, L.fest.p, L.fslow.p, L.ftot.p);
       L.rmse_stale = rel_rmse(c, L, L.f0.p, L.fslow.p, L.ftot.p);
     }
@@ -831,12 +978,16 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
     L.learned_mode = L.frames >= L.train_frames && L.scaled && (training || L.rmse <= L.tol);
     L.last_full = step;
     result = (was && !L.learned_mode) ? 2 : 0;
+    kt_begin(c, "k_axpy");
     hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fslow.p, c.f.p);
     LAUNCHCHECK();
+    kt_end(c);
   } else {
     estimate(c, L);
+    kt_begin(c, "k_axpy");
     hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fest.p, c.f.p);
     LAUNCHCHECK();
+    kt_end(c);
     result = 1;
   }
   if (want)
