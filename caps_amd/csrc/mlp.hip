@@ -5,7 +5,7 @@
 // themselves: pairs of full steps k apart give (ΔD, ΔF/q). At every full step the prediction is judged against the
 // fresh PME force: above the tolerance (relative force RMSE against the full force) the next interval runs plain PME.
 //
-//  k_cell_*      atoms binned into cells of at least 8 Å (fractional coordinates), sorted by atom in each cell
+//  k_cell_*      atoms binned into cells of at least 4 Å (fractional coordinates), sorted by atom in each cell
 //  k_desc        per atom: 24 hat-function radial bins × {Σ q_j h, Σ q_j h d̂_x, d̂_y, d̂_z} over neighbours within
 //                8 Å (smooth cosine cut-off), accumulated in LDS, FP32
 //  k_delta_in    the network input: (D − D0)·s per feature, BF16
@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -103,18 +104,18 @@ __device__ inline float desc_cutoff(float r) {
   return __builtin_fmaf(0.5f * z, p, 0.5f);
 }
 
-// 64 atoms (in cell order) per workgroup, one per lane, each lane's 96 sums in its own LDS row (odd stride: no bank
-// conflicts). A lane walks the 27 cells around its own, reading the cell-ordered positions; the neighbours within 8 Å
-// go into the lane's queue in LDS (cell-ordered index | image code << 27), and whenever a lane's queue may not hold
-// another batch of candidates the wave's active lanes empty theirs together, so the bin updates run only for pairs
-// within range. Queued pairs are
-// summed in the order they were found: a fixed order.
-constexpr int kDescRow = kIn + 1, kDescQueue = 32, kDescBatch = 8;
-template <bool kRegAcc>
+// 64 atoms (in cell order) per workgroup, one per lane, the 96 sums in the lane's registers. A lane walks the 125 cells
+// within two cells of its own (cells are at least 4 Å wide) as 25 rows along a, each row's cells consecutive in the
+// cell order (one range of the cell-ordered positions, or two where the row wraps at the a faces); rows go from the
+// centre out, each followed by its opposite, so the lanes' pair counts grow evenly. The kernel is one loop: each turn
+// either tests the next batch of every lane's candidates, the neighbours within 8 Å going onto the lane's stack of 64
+// slots in LDS (cell-ordered index | image code << 27), or takes queued pairs into the sums in lockstep across the
+// wave — a few from every lane once every live lane holds that many, a batch's worth when some lane's stack may not
+// hold another batch, and the rest at the end. The sums' order follows from the cell order alone.
+constexpr int kDescSlots = 64, kDescBatch = 4, kDescStep = 4;   // a turn takes kDescStep ≥ kDescBatch pairs
 __global__ void __launch_bounds__(64) k_desc(DescGeom G, int n, const float4* __restrict__ xs, const int* __restrict__ cell,
                                              const int64_t* __restrict__ cstart, const uint32_t* __restrict__ catoms, float* __restrict__ out) {
-  __shared__ float acc[kRegAcc ? 1 : 64 * kDescRow];
-  __shared__ uint32_t que[kDescQueue * 64];
+  __shared__ uint32_t que[kDescSlots * 64];
   __shared__ float4 shift[27];
   const int lane = threadIdx.x;
   if (lane < 27) {
@@ -125,124 +126,114 @@ __global__ void __launch_bounds__(64) k_desc(DescGeom G, int n, const float4* __
   }
   const int slot = blockIdx.x * 64 + lane;
   const bool live = slot < n;
-  float* A = acc + (kRegAcc ? 0 : lane * kDescRow);
-  float R[kRegAcc ? kIn : 1];
-  if (kRegAcc) {
+  float R[kIn];
 #pragma unroll
-    for (int k = 0; k < (kRegAcc ? kIn : 1); ++k) R[k] = 0.0f;
-  } else {
-    for (int k = 0; k < kIn; ++k) A[k] = 0.0f;
-  }
+  for (int k = 0; k < kIn; ++k) R[k] = 0.0f;
   __syncthreads();
   const float4 xi = live ? xs[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const int c = live ? cell[catoms[slot]] : 0;
   const int cx = c % G.nc[0], cy = (c / G.nc[0]) % G.nc[1], cz = c / (G.nc[0] * G.nc[1]);
   const float idlt = float(kBins - 1) / kRcut;
-  int cnt = 0;
-  // one queued pair into the lane's sums
-  auto accumulate = [&](const float4& xj, const float4& sh) __attribute__((always_inline)) {
-    const float ex = xj.x + sh.x - xi.x, ey = xj.y + sh.y - xi.y, ez = xj.z + sh.z - xi.z;
-    const float r2 = ex * ex + ey * ey + ez * ez;
-    const float r = sqrtf(r2), ir = 1.0f / r;
-    const float u = r * idlt;
-    const float qf = xj.w * desc_cutoff(r);
-    const float hx = ex * ir, hy = ey * ir, hz = ez * ir;
-    if (kRegAcc) {
-      // every bin's hat weight max(0, 1 − |u − b|): two are non-zero
-      const float g1 = qf * hx, g2 = qf * hy, g3 = qf * hz;
-#pragma unroll
-      for (int b = 0; b < (kRegAcc ? kBins : 0); ++b) {
-        const float wb = fmaxf(0.0f, 1.0f - fabsf(u - float(b)));
-        R[b] = __builtin_fmaf(wb, qf, R[b]);
-        R[kBins + b] = __builtin_fmaf(wb, g1, R[kBins + b]);
-        R[2 * kBins + b] = __builtin_fmaf(wb, g2, R[2 * kBins + b]);
-        R[3 * kBins + b] = __builtin_fmaf(wb, g3, R[3 * kBins + b]);
-      }
-    } else {
-      const int b0 = min(int(u), kBins - 2);
-      const float w1 = u - float(b0), w0 = 1.0f - w1;
-      const float v0 = qf * w0, v1 = qf * w1;
-      A[b0] += v0, A[b0 + 1] += v1;
-      A[kBins + b0] += v0 * hx, A[kBins + b0 + 1] += v1 * hx;
-      A[2 * kBins + b0] += v0 * hy, A[2 * kBins + b0 + 1] += v1 * hy;
-      A[3 * kBins + b0] += v0 * hz, A[3 * kBins + b0 + 1] += v1 * hz;
-    }
+  // the lane's cursor: row step, part of the row, next candidate, end of the range, image code and origin
+  int step = 0, part = 0, k = 0, ke = 0, cnt = 0;
+  uint32_t wc = 0;
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  bool done = !live;
+  // the range of the cursor's (step, part) (empty ranges stay: they cost one turn)
+  auto locate = [&]() __attribute__((always_inline)) {
+    // row 12 (dy = dz = 0) first, then 13, 11, 14, 10, …: each row followed by its opposite
+    const int row = 12 + ((step + 1) >> 1) * ((step & 1) ? 1 : -1);
+    int jy = cy + row % 5 - 2, jz = cz + row / 5 - 2;
+    const int sy = jy < 0 ? -1 : jy >= G.nc[1] ? 1 : 0, sz = jz < 0 ? -1 : jz >= G.nc[2] ? 1 : 0;
+    jy -= sy * G.nc[1], jz -= sz * G.nc[2];
+    const int base = (jz * G.nc[1] + jy) * G.nc[0], nx = G.nc[0];
+    const uint32_t wyz = uint32_t(3 * (sy + 1) + 9 * (sz + 1));
+    // cx < 2: cells nx + cx − 2 … nx − 1 (image −a), then 0 … cx + 2; cx > nx − 3: cells cx − 2 … nx − 1, then
+    // 0 … cx + 2 − nx (image +a); otherwise cells cx − 2 … cx + 2 (and an empty second part)
+    int c0, c1;
+    uint32_t w = wyz + 1u;
+    if (cx < 2) c0 = part ? 0 : nx + cx - 2, c1 = part ? cx + 3 : nx, w = part ? wyz + 1u : wyz;
+    else if (cx > nx - 3) c0 = part ? 0 : cx - 2, c1 = part ? cx + 3 - nx : nx, w = part ? wyz + 2u : wyz + 1u;
+    else c0 = cx - 2, c1 = part ? cx - 2 : cx + 3;
+    k = int(cstart[base + c0]), ke = int(cstart[base + c1]);
+    const float4 sh = shift[w];
+    ox = xi.x - sh.x, oy = xi.y - sh.y, oz = xi.z - sh.z;
+    wc = w << 27;
   };
-  // the queued pairs of the active lanes into their sums, in queue order; four positions loaded ahead at a time
-  auto drain = [&]() __attribute__((always_inline)) {
-    for (int t = 0; t < cnt; t += 4) {
-      float4 xj[4], sh[4];
+  // on to the next range: the row's second part where the row wraps, else the next row (or done)
+  auto advance = [&]() __attribute__((always_inline)) {
+    const bool wraps = cx < 2 || cx > G.nc[0] - 3;
+    if (part == 0 && wraps) part = 1;
+    else part = 0, ++step;
+    done = step == 25;
+    if (!done) locate();
+  };
+  locate();
+  for (;;) {
+    // test batches while every active lane's stack has room for one and not every live lane holds a turn's pairs
+    // (the choices are uniform across the wave). A batch of each active lane's candidates goes onto its stack: the
+    // positions array holds a batch of padding past its end, so reads past ke stay inside it (those candidates are
+    // not taken), and the slot a candidate would take is written whether or not it is taken.
+    for (;;) {
+      if (__any(!done && cnt > kDescSlots - kDescBatch) || __all(done) || __all(!live || cnt >= kDescStep)) break;
+      if (!done) {
+        float4 b[kDescBatch];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t e = que[min(t + u, cnt - 1) * 64 + lane];
+        for (int u = 0; u < kDescBatch; ++u) b[u] = xs[k + u];
+        const int nv = ke - k;
+#pragma unroll
+        for (int u = 0; u < kDescBatch; ++u) {
+          const float ex = b[u].x - ox, ey = b[u].y - oy, ez = b[u].z - oz;
+          const float r2 = ex * ex + ey * ey + ez * ez;
+          que[cnt * 64 + lane] = uint32_t(k + u) | wc;
+          cnt += (u < nv && r2 < kRcut * kRcut && r2 >= 1e-6f) ? 1 : 0;
+        }
+        k += kDescBatch;
+        if (k >= ke) advance();
+      }
+    }
+    if (__all(done) && !__any(cnt > 0)) break;
+    // kDescStep pairs from the top of each lane's stack (fewer where it holds fewer), two positions loaded ahead; the
+    // separation is computed exactly as the test computed it; every bin's hat weight max(0, 1 − |u − b|), two of
+    // them non-zero, goes into the sums. A position past the lane's pairs reads atom 0 and adds zeros.
+#pragma unroll
+    for (int t = 0; t < kDescStep; t += 2) {
+      float4 xj[2], sh[2];
+      bool ok[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        ok[u] = t + u < cnt;
+        const uint32_t e = ok[u] ? que[(cnt - 1 - (t + u)) * 64 + lane] : 0u;
         xj[u] = xs[e & 0x7ffffffu];
         sh[u] = shift[e >> 27];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (t + u < cnt) accumulate(xj[u], sh[u]);
-    }
-    cnt = 0;
-  };
-  // the candidates [kb, ke) of image code w, a batch of positions loaded while the previous one is tested (indexes
-  // past the range's end repeat its last atom and are not taken)
-  auto scan = [&](int64_t kb, int64_t ke, uint32_t w) __attribute__((always_inline)) {
-    if (kb >= ke) return;
-    const float4 sh = shift[w];
-    float4 cur[kDescBatch], nxt[kDescBatch];
+      for (int u = 0; u < 2; ++u) {
+        const float px = xi.x - sh[u].x, py = xi.y - sh[u].y, pz = xi.z - sh[u].z;
+        const float ex = xj[u].x - px, ey = xj[u].y - py, ez = xj[u].z - pz;
+        const float r2 = ok[u] ? ex * ex + ey * ey + ez * ez : 1.0f;
+        const float ir = __builtin_amdgcn_rsqf(r2), r = r2 * ir;
+        // u clamped to its range (no change within 8 Å): u − b then takes one subtraction with b as a literal
+        const float uu = __builtin_amdgcn_fmed3f(r * idlt, 0.0f, float(kBins));
+        const float q0 = ok[u] ? xj[u].w * desc_cutoff(r) : 0.0f;
+        const float q1 = q0 * (ex * ir), q2 = q0 * (ey * ir), q3 = q0 * (ez * ir);
 #pragma unroll
-    for (int u = 0; u < kDescBatch; ++u) cur[u] = xs[kb + u < ke ? kb + u : ke - 1];
-    for (int64_t k0 = kb; k0 < ke; k0 += kDescBatch) {
-#pragma unroll
-      for (int u = 0; u < kDescBatch; ++u) nxt[u] = xs[k0 + kDescBatch + u < ke ? k0 + kDescBatch + u : ke - 1];
-      if (__any(cnt > kDescQueue - kDescBatch)) drain();
-#pragma unroll
-      for (int u = 0; u < kDescBatch; ++u) {
-        const float ex = cur[u].x + sh.x - xi.x, ey = cur[u].y + sh.y - xi.y, ez = cur[u].z + sh.z - xi.z;
-        const float r2 = ex * ex + ey * ey + ez * ez;
-        if (k0 + u < ke && r2 < kRcut * kRcut && r2 >= 1e-6f) que[(cnt++) * 64 + lane] = uint32_t(k0 + u) | (w << 27);
-      }
-#pragma unroll
-      for (int u = 0; u < kDescBatch; ++u) cur[u] = nxt[u];
-    }
-  };
-  // the 27 cells around the lane's own as 9 rows along a; a row's cells are consecutive in the cell order (one range)
-  // unless the row wraps at the cell's a faces (two ranges with different images)
-  if (live)
-    for (int row = 0; row < 9; ++row) {
-      int jy = cy + row % 3 - 1, jz = cz + row / 3 - 1;
-      const int sy = jy < 0 ? -1 : jy >= G.nc[1] ? 1 : 0, sz = jz < 0 ? -1 : jz >= G.nc[2] ? 1 : 0;
-      jy -= sy * G.nc[1], jz -= sz * G.nc[2];
-      const int base = (jz * G.nc[1] + jy) * G.nc[0], nx = G.nc[0];
-      const uint32_t wyz = uint32_t(3 * (sy + 1) + 9 * (sz + 1));
-      for (int part = 0; part < 2; ++part) {
-        // cx = 0: the last cell of the row (image −a), then cells 0, 1; cx = nx − 1: cells cx − 1, cx, then cell 0
-        // (image +a); otherwise cells cx − 1 … cx + 1
-        int c0, c1;
-        uint32_t w = wyz + 1u;
-        if (cx == 0) c0 = part ? 0 : nx - 1, c1 = part ? 2 : nx, w = part ? wyz + 1u : wyz;
-        else if (cx == nx - 1) c0 = part ? 0 : cx - 1, c1 = part ? 1 : nx, w = part ? wyz + 2u : wyz + 1u;
-        else c0 = cx - 1, c1 = part ? cx - 1 : cx + 2;
-        scan(cstart[base + c0], cstart[base + c1], w);
+        for (int bb = 0; bb < kBins; ++bb) {
+          const float wb = __builtin_amdgcn_fmed3f(1.0f - fabsf(uu - float(bb)), 0.0f, 1.0f);
+          R[bb] = __builtin_fmaf(wb, q0, R[bb]);
+          R[kBins + bb] = __builtin_fmaf(wb, q1, R[kBins + bb]);
+          R[2 * kBins + bb] = __builtin_fmaf(wb, q2, R[2 * kBins + bb]);
+          R[3 * kBins + bb] = __builtin_fmaf(wb, q3, R[3 * kBins + bb]);
+        }
       }
     }
-  drain();
-  if (kRegAcc) {
-    // each lane's row straight out, 16 bytes at a time
-    if (live) {
-      float4* o = reinterpret_cast<float4*>(out + size_t(catoms[slot]) * kIn);
+    cnt = max(cnt - kDescStep, 0);
+  }
+  // each lane's row straight out, 16 bytes at a time
+  if (live) {
+    float4* o = reinterpret_cast<float4*>(out + size_t(catoms[slot]) * kIn);
 #pragma unroll
-      for (int k = 0; k < (kRegAcc ? kIn / 4 : 0); ++k) o[k] = make_float4(R[4 * k], R[4 * k + 1], R[4 * k + 2], R[4 * k + 3]);
-    }
-  } else {
-    // the rows out, one atom at a time across the wave (coalesced)
-    __syncthreads();
-    for (int l = 0; l < 64; ++l) {
-      const int sl = blockIdx.x * 64 + l;
-      if (sl >= n) break;
-      const size_t i = catoms[sl];
-      for (int k = lane; k < kIn; k += 64) out[i * kIn + k] = acc[l * kDescRow + k];
-    }
+    for (int q = 0; q < kIn / 4; ++q) o[q] = make_float4(R[4 * q], R[4 * q + 1], R[4 * q + 2], R[4 * q + 3]);
   }
 }
 
@@ -596,8 +587,8 @@ DescGeom desc_geom(const capsgpu_ctx& c) {
   for (int d = 0; d < 3; ++d) {
     for (int k = 0; k < 3; ++k) G.hi[d][k] = rows[d][k] / v;
     const double w = std::fabs(v) / std::sqrt(rows[d][0] * rows[d][0] + rows[d][1] * rows[d][1] + rows[d][2] * rows[d][2]);
-    G.nc[d] = int(w / double(kRcut));
-    if (G.nc[d] < 3) throw Error(CAPSGPU_EINVAL, "learned r-RESPA needs a cell at least 3 x 8 A wide along each axis");
+    G.nc[d] = int(w / (0.5 * double(kRcut)));
+    if (G.nc[d] < 5) throw Error(CAPSGPU_EINVAL, "learned r-RESPA needs a cell at least 5 x 4 A wide along each axis");
   }
   return G;
 }
@@ -626,8 +617,6 @@ void gemm(capsgpu_ctx& c, Learned& L, int M, int N, int K, const __bf16* A, int6
   }
 }
 
-constexpr bool kDescRegAcc = true;
-
 // raw descriptors of every atom at the device positions into out (n x 96, FP32)
 void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
   const int n = c.n;
@@ -649,7 +638,7 @@ void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
   L.cfill.zero(s);
   L.catoms.resize(size_t(n));
   L.ctmp.resize(size_t(n));
-  L.xs.resize(size_t(n));
+  L.xs.resize(size_t(n) + kDescBatch);   // padding: the scan reads up to a batch less one past a range's end
   kt_begin(c, "k_cell_fill");
   hipLaunchKernelGGL(k_cell_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, L.cell.p, L.cstart.p, L.cfill.p, L.ctmp.p);
   LAUNCHCHECK();
@@ -659,8 +648,7 @@ void descriptors(capsgpu_ctx& c, Learned& L, float* out) {
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_desc");
-  if (kDescRegAcc) hipLaunchKernelGGL(k_desc<true>, dim3(blocks_for(n, 64)), dim3(64), 0, s, G, n, L.xs.p, L.cell.p, L.cstart.p, L.catoms.p, out);
-  else hipLaunchKernelGGL(k_desc<false>, dim3(blocks_for(n, 64)), dim3(64), 0, s, G, n, L.xs.p, L.cell.p, L.cstart.p, L.catoms.p, out);
+  hipLaunchKernelGGL(k_desc, dim3(blocks_for(n, 64)), dim3(64), 0, s, G, n, L.xs.p, L.cell.p, L.cstart.p, L.catoms.p, out);
   LAUNCHCHECK();
   kt_end(c);
 }
@@ -919,7 +907,7 @@ Learned* learned_begin(capsgpu_ctx& c, const capsgpu_dyn_opts& o) {
   L.train_frames = o.learned_train >= 0 ? o.learned_train : 2000;
   L.last_full = -1;
   L.ring_n = 0;
-  (void)desc_geom(c);   // the cell must hold 3 cells of 8 Å per axis
+  (void)desc_geom(c);   // the cell must hold 5 cells of 4 Å per axis
   const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
   L.f0.resize(n3);
   L.fslow.resize(n3);

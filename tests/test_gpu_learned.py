@@ -87,6 +87,25 @@ def test_descriptors_match_the_oracle():
     assert err < 1e-5
 
 
+def test_descriptors_triclinic_partial_wave():
+    # a sheared cell five 4 Å cells wide, rows that wrap at every face, and 1100 atoms: the last workgroup is partly
+    # empty
+    print("case descriptors triclinic", flush=True)
+    s = H.triclinic_case(n=1100)
+    ev = ca.Evaluator(s.ff, ca.EnergyOptions(cutoff=8.0, coulomb=False, parts=1, tail=False))
+    ev.set_state(s.x, None, s.cell)
+    d = ev.learned_descriptors()
+    do = np.zeros(s.ff.n * 96)
+    O.orc().orc_descriptors(np.ascontiguousarray(s.x, np.float64).reshape(-1), np.ascontiguousarray(s.ff.charge), s.ff.n,
+                            np.ascontiguousarray(s.cell, np.float64), do)
+    do = do.reshape(-1, 96)
+    err = np.sqrt(((d - do) ** 2).sum() / (do ** 2).sum())
+    print("descriptor rel RMS", err, flush=True)
+    assert err < 1e-5
+    # the same positions give the same bits
+    assert np.array_equal(d, ev.learned_descriptors())
+
+
 def test_learned_respa_runs_and_judges_itself():
     # 200 frames of plain PME train the network on (ΔD, ΔF/q) pairs four steps apart; then the slow force comes from the
     # network between full PME steps every 4 steps, each full step judging the prediction against PME and against the
