@@ -600,7 +600,10 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       c->v.upload(v, s);
     }
     const double dt = o->dt, hk = 0.5 * dt * kAcc, bskin = std::min(0.8, c->opt.skin);
-    const double lim_b = 0.25 * c->opt.skin * c->opt.skin, lim_p = 0.25 * bskin * bskin;
+    // move limits for the pair list's rebuild and prune checks; with no pair terms there is no list (negative: only the
+    // positions' finiteness is checked)
+    const bool listed = (c->opt.parts & 2) != 0;
+    const double lim_b = listed ? 0.25 * c->opt.skin * c->opt.skin : -1.0, lim_p = listed ? 0.25 * bskin * bskin : -1.0;
     const double c1 = std::exp(-dt / std::max(o->tau_t, 1e-9)), c2 = std::sqrt(1 - c1 * c1);
     const double kt_target = 0.5 * ndof * kBoltz * o->temperature;
     DevBuf<double> sdv, kpart, bpart, scal;   // scal: [0] α, [1] bath

@@ -92,8 +92,8 @@ struct Pme {
   DevBuf<double> bmod;                  // |b(m)|² per axis, K1 + K2 + K3
   DevBuf<float> theta;                  // per atom: the B-spline offsets w of the three axes and the charge (float4)
   DevBuf<int32_t> base;                 // per atom: the first grid index per axis and the home tile (int4)
-  DevBuf<float> wts;                    // per atom: θ and dθ of the three axes (6p floats)
-  DevBuf<uint32_t> tile, tcount, tfill, tatoms;
+  DevBuf<int32_t> home;                 // per axis and tile: the home tiles that reach it (count + 4)
+  DevBuf<uint32_t> tile, tcount, tfill, tatoms;   // tile: the atoms filled unordered per tile; tatoms: in index order
   DevBuf<int64_t> tstart;
   DevBuf<float> fpme;                   // 3 per atom
   DevBuf<float2> tw;                    // twiddles of the three axes

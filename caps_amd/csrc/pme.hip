@@ -4,8 +4,9 @@
 //
 //  k_splines      per atom: fractional grid coordinate, first grid point, B-spline weights and derivatives; home tile
 //  k_tile_*       atoms binned by the 4x4x4 grid tile of their first point (sorted by atom in each tile)
-//  k_spread       one 64-lane workgroup per tile, one grid point per lane: each point sums q·θa·θb·θc over the atoms
-//                 whose support covers it, staged in LDS (every point written once, no atomics, a fixed order)
+//  k_spread       one 64-lane workgroup per tile: each point sums q·θa·θb·θc over the atoms whose support covers it,
+//                 staged in LDS, a lane taking a row of 4 points over a quarter of the atoms (every point written
+//                 once, no atomics, a fixed order)
 //  k_fft_x_r2c    two real lines per complex transform along a (as the reference), the half spectrum written
 //  k_fft_y        complex lines along b, 16 lines of consecutive a-frequencies per workgroup (coalesced)
 //  k_fft_z_conv   lines along c: forward transform, × influence function (energy and virial on energy steps),
@@ -28,33 +29,40 @@ constexpr int kMaxFft = 512;
 
 __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
-// B-spline weights θ[j] = M_p(w + p − 1 − j) … in the reference's order (caps_kspace.f90:568-598), FP32
-__device__ inline void bspline(float w, int p, float* th, float* dth) {
-  for (int j = 0; j < kMaxOrder; ++j) th[j] = 0.0f;
+// B-spline weights θ[j] = M_p(w + p − 1 − j) … in the reference's order (caps_kspace.f90:568-598), FP32; the order
+// P a compile-time constant, so every loop unrolls and the arrays stay in registers
+template <int P>
+__device__ inline void bspline(float w, float* th, float* dth) {
+#pragma unroll
+  for (int j = 0; j < P; ++j) th[j] = 0.0f;
   th[1] = w;
   th[0] = 1.0f - w;
-  for (int k = 3; k <= p - 1; ++k) {
+#pragma unroll
+  for (int k = 3; k <= P - 1; ++k) {
     const float div = 1.0f / float(k - 1);
     th[k - 1] = div * w * th[k - 2];
+#pragma unroll
     for (int j = 1; j <= k - 2; ++j) th[k - j - 1] = div * ((w + float(j)) * th[k - j - 2] + (float(k - j) - w) * th[k - j - 1]);
     th[0] = div * (1.0f - w) * th[0];
   }
   dth[0] = -th[0];
-  for (int j = 2; j <= p; ++j) dth[j - 1] = th[j - 2] - th[j - 1];
-  const float div = 1.0f / float(p - 1);
-  th[p - 1] = div * w * th[p - 2];
-  for (int j = 1; j <= p - 2; ++j) th[p - j - 1] = div * ((w + float(j)) * th[p - j - 2] + (float(p - j) - w) * th[p - j - 1]);
+#pragma unroll
+  for (int j = 2; j <= P; ++j) dth[j - 1] = th[j - 2] - th[j - 1];
+  const float div = 1.0f / float(P - 1);
+  th[P - 1] = div * w * th[P - 2];
+#pragma unroll
+  for (int j = 1; j <= P - 2; ++j) th[P - j - 1] = div * ((w + float(j)) * th[P - j - 2] + (float(P - j) - w) * th[P - j - 1]);
   th[0] = div * (1.0f - w) * th[0];
 }
 
 struct GridArgs {
-  int k[3], nt[3], p, h1;
+  int k[3], nt[3], p, h1, ntm;   // ntm: the largest of nt (the home table's stride)
   double hi[3][3];
 };
 
 // fractional coordinate s = h⁻¹ x, u = (s − ⌊s⌋) K, the first point ⌊u⌋ − p + 1 (mod K) and w = u − ⌊u⌋
 __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
-                          int4* __restrict__ base, uint32_t* __restrict__ tcount, float* __restrict__ wts) {
+                          int4* __restrict__ base, uint32_t* __restrict__ tcount) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float w[3];
@@ -72,87 +80,74 @@ __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const
   wq[i] = make_float4(w[0], w[1], w[2], float(q[i]));
   base[i] = make_int4(b[0], b[1], b[2], tile);
   atomicAdd(&tcount[tile], 1u);
-  // the weights θ and derivatives dθ of the three axes, 6p floats per atom, for the spread and the gather
-  float* o = wts + size_t(i) * 6 * G.p;
-  for (int d = 0; d < 3; ++d) {
-    float th[kMaxOrder], dth[kMaxOrder];
-    bspline(w[d], G.p, th, dth);
-    for (int j = 0; j < G.p; ++j) o[d * G.p + j] = th[j], o[3 * G.p + d * G.p + j] = dth[j];
-  }
 }
 
 __global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
+                            uint32_t* __restrict__ tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = base[i].w;
+  tmp[tstart[t] + atomicAdd(&tfill[t], 1u)] = uint32_t(i);
+}
+
+// the atoms of each tile in index order: an atom's place is its rank among its tile's atoms (filled unordered in tmp)
+__global__ void k_tile_rank(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tmp,
                             uint32_t* __restrict__ tatoms) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int t = base[i].w;
-  tatoms[tstart[t] + atomicAdd(&tfill[t], 1u)] = uint32_t(i);
-}
-
-__global__ void k_tile_sort(int ntiles, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tatoms) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
   const int64_t b = tstart[t], e = tstart[t + 1];
-  for (int64_t k = b + 1; k < e; ++k) {
-    const uint32_t v = tatoms[k];
-    int64_t j = k;
-    while (j > b && tatoms[j - 1] > v) tatoms[j] = tatoms[j - 1], --j;
-    tatoms[j] = v;
-  }
+  int64_t r = 0;
+  for (int64_t k = b; k < e; ++k) r += tmp[k] < uint32_t(i) ? 1 : 0;
+  tatoms[b + r] = uint32_t(i);
 }
 
-// distinct home tiles (along one axis) of the first points that reach points [4T, 4T + 3]
-__device__ inline int home_tiles(int T, int K, int p, int* out) {
-  int m = 0;
-  for (int bb = 4 * T - p + 1; bb <= 4 * T + 3; ++bb) {
-    int b = bb % K;
-    if (b < 0) b += K;
-    const int h = b >> 2;
-    bool seen = false;
-    for (int q = 0; q < m; ++q) seen |= out[q] == h;
-    if (!seen && m < 8) out[m++] = h;
-  }
-  return m;
-}
-
-// one 64-lane workgroup per 4x4x4 tile, one grid point per lane; the atoms whose first point lies in a tile that can
-// reach this one are staged in LDS 64 at a time (their base points, charges and weights), then every lane sums over
-// them in the same fixed order
+// one 64-lane workgroup per 4x4x4 tile. The atoms whose first point lies in a tile that can reach this one (the home
+// table lists those tiles per axis) are staged in LDS 64 at a time: per atom q·θb[dy]·θc[dz] for every (dy, dz), θa
+// with three zeros before and five after it, and the offsets r_d = 4 t_d − b_d of its first point from the tile's
+// corner (folded into [−3, p]), so a point (i, j, k) of the tile sees θa[i + r_a], θb[j + r_b], θc[k + r_c]. Lane =
+// (row (j, k), quarter of the atoms): it sums its quarter's terms for the row's 4 points; the quarters are then added
+// in a fixed order (every point written once, no atomics). Dynamic LDS: kSpreadChunk · (p² + p + 8) floats.
 constexpr int kSpreadChunk = 64, kMaxHome = 64;   // at most 4 home tiles per axis reach a tile (p <= 8, partial last tile)
-__global__ void __launch_bounds__(64) k_spread(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
-                                               const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ wts,
-                                               float* __restrict__ grid) {
-  __shared__ float s_w[kSpreadChunk][3 * kMaxOrder];
-  __shared__ int s_b[kSpreadChunk][3];
-  __shared__ float s_q[kSpreadChunk];
+template <int P>
+__global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
+                                               const uint32_t* __restrict__ tatoms, const float4* __restrict__ wq,
+                                               const int4* __restrict__ base, float* __restrict__ grid) {
+  extern __shared__ float2 lds[];
+  __shared__ int s_r[kSpreadChunk][3];
+  __shared__ float s_part[4][64];
   __shared__ int64_t s_hb[kMaxHome], s_pre[kMaxHome + 1];
-  __shared__ int s_nh;
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
-  const int p = G.p;
+  constexpr int p = P, pp = P * P, px = P + 8;
+  float* s_yz = reinterpret_cast<float*>(lds);          // [chunk][p²]
+  float* s_x = s_yz + kSpreadChunk * pp;                 // [chunk][p + 8]: 0 0 0 θa[0 … p−1] 0 0 0 0 0
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
-  const int gx = 4 * tx + (lane & 3), gy = 4 * ty + ((lane >> 2) & 3), gz = 4 * tz + (lane >> 4);
-  // the candidate tiles in a fixed order and the running count of their atoms
-  if (lane == 0) {
-    int hx[8], hy[8], hz[8];
-    const int mx = home_tiles(tx, G.k[0], p, hx), my = home_tiles(ty, G.k[1], p, hy), mz = home_tiles(tz, G.k[2], p, hz);
-    int nh = 0;
-    s_pre[0] = 0;
-    for (int iz = 0; iz < mz; ++iz)
-      for (int iy = 0; iy < my; ++iy)
-        for (int ix = 0; ix < mx && nh < kMaxHome; ++ix) {
-          const int h = (hz[iz] * G.nt[1] + hy[iy]) * G.nt[0] + hx[ix];
-          s_hb[nh] = tstart[h];
-          s_pre[nh + 1] = s_pre[nh] + (tstart[h + 1] - tstart[h]);
-          ++nh;
-        }
-    s_nh = nh;
+  // the candidate tiles in a fixed order (c, b, a) from the home table, their atom counts in parallel, a running sum
+  const int* hx = home + (0 * G.ntm + tx) * 5;
+  const int* hy = home + (1 * G.ntm + ty) * 5;
+  const int* hz = home + (2 * G.ntm + tz) * 5;
+  const int mx = hx[0], my = hy[0], mz = hz[0], nh = mx * my * mz;
+  int64_t cnt = 0;
+  if (lane < nh) {
+    const int ix = lane % mx, iy = (lane / mx) % my, iz = lane / (mx * my);
+    const int h = (hz[1 + iz] * G.nt[1] + hy[1 + iy]) * G.nt[0] + hx[1 + ix];
+    s_hb[lane] = tstart[h];
+    cnt = tstart[h + 1] - tstart[h];
   }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t o = __shfl_up(cnt, d, 64);
+    if (lane >= d) cnt += o;
+  }
+  if (lane < nh) s_pre[lane + 1] = cnt;
+  if (lane == 0) s_pre[0] = 0;
   __syncthreads();
-  const int64_t m = s_pre[s_nh];
-  float acc = 0.0f;
+  const int64_t m = s_pre[nh];
+  const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4;
+  float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
   for (int64_t c0 = 0; c0 < m; c0 += kSpreadChunk) {
     const int64_t f = c0 + lane;
     if (f < m) {
@@ -160,23 +155,51 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int64_t* __rest
       while (s_pre[t + 1] <= f) ++t;
       const uint32_t a = tatoms[s_hb[t] + (f - s_pre[t])];
       const int4 b = base[a];
-      s_b[lane][0] = b.x, s_b[lane][1] = b.y, s_b[lane][2] = b.z;
-      s_q[lane] = wq[a].w;
-      const float* w = wts + size_t(a) * 6 * p;
-      for (int k = 0; k < 3 * p; ++k) s_w[lane][k] = w[k];
+      const float4 wv = wq[a];
+      const int bb[3] = {b.x, b.y, b.z}, tt[3] = {tx, ty, tz};
+      for (int d = 0; d < 3; ++d) {
+        int r = (4 * tt[d] - bb[d]) % G.k[d];
+        r += r < 0 ? G.k[d] : 0;
+        r -= r > G.k[d] - 4 ? G.k[d] : 0;
+        s_r[lane][d] = r < p ? r : p;
+      }
+      float ta[P], tb[P], tc[P], dd[P];
+      bspline<P>(wv.x, ta, dd);
+      bspline<P>(wv.y, tb, dd);
+      bspline<P>(wv.z, tc, dd);
+      float* sx = s_x + lane * px;
+#pragma unroll
+      for (int k = 0; k < px; ++k) sx[k] = (k >= 3 && k < 3 + P) ? ta[(k >= 3 && k < 3 + P) ? k - 3 : 0] : 0.0f;
+      float* syz = s_yz + lane * pp;
+#pragma unroll
+      for (int dy = 0; dy < P; ++dy) {
+        const float qb = wv.w * tb[dy];
+#pragma unroll
+        for (int dz = 0; dz < P; ++dz) syz[dy * P + dz] = qb * tc[dz];
+      }
     }
     __syncthreads();
-    const int cnt = int(m - c0 < kSpreadChunk ? m - c0 : kSpreadChunk);
-    for (int jj = 0; jj < cnt; ++jj) {
-      int dx = gx - s_b[jj][0], dy = gy - s_b[jj][1], dz = gz - s_b[jj][2];
-      dx += dx < 0 ? G.k[0] : 0;
-      dy += dy < 0 ? G.k[1] : 0;
-      dz += dz < 0 ? G.k[2] : 0;
-      if (dx < p && dy < p && dz < p) acc += s_q[jj] * s_w[jj][p + dy] * s_w[jj][2 * p + dz] * s_w[jj][dx];
+    const int n_here = int(m - c0 < kSpreadChunk ? m - c0 : kSpreadChunk);
+    for (int j0 = 0; j0 < n_here; j0 += 4) {
+      const int j = j0 + quarter;
+      if (j < n_here) {
+        const int dy = gy + s_r[j][1], dz = gz + s_r[j][2];
+        if (unsigned(dy) < unsigned(p) && unsigned(dz) < unsigned(p)) {
+          const float v = s_yz[j * pp + dy * p + dz];
+          const float* xr = s_x + j * px + s_r[j][0] + 3;
+          acc0 += v * xr[0], acc1 += v * xr[1], acc2 += v * xr[2], acc3 += v * xr[3];
+        }
+      }
     }
     __syncthreads();
   }
-  if (gx < G.k[0] && gy < G.k[1] && gz < G.k[2]) grid[(size_t(gz) * G.k[1] + gy) * G.k[0] + gx] = acc;
+  // the quarters' sums by point (i + 4 j + 16 k), then each lane adds its point's four in order
+  const int row = 4 * gy + 16 * gz;
+  s_part[quarter][row] = acc0, s_part[quarter][row + 1] = acc1, s_part[quarter][row + 2] = acc2, s_part[quarter][row + 3] = acc3;
+  __syncthreads();
+  const float sum = ((s_part[0][lane] + s_part[1][lane]) + s_part[2][lane]) + s_part[3][lane];
+  const int gx = 4 * tx + (lane & 3), gy_ = 4 * ty + ((lane >> 2) & 3), gz_ = 4 * tz + (lane >> 4);
+  if (gx < G.k[0] && gy_ < G.k[1] && gz_ < G.k[2]) grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
 }
 
 // ---- FFT ------------------------------------------------------------------------------------------------------------
@@ -457,50 +480,101 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
   }
 }
 
-__global__ void k_gather(GridArgs G, int n, const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ wts,
-                         const float* __restrict__ grid, float* __restrict__ f) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 w = wq[i];
-  const int4 b = base[i];
-  const int p = G.p;
-  const float* o = wts + size_t(i) * 6 * p;
-  float tx[kMaxOrder], dx[kMaxOrder], ty[kMaxOrder], dy[kMaxOrder], tz[kMaxOrder], dz[kMaxOrder];
-  for (int j = 0; j < kMaxOrder; ++j) {
-    const bool in = j < p;
-    tx[j] = in ? o[j] : 0.0f, ty[j] = in ? o[p + j] : 0.0f, tz[j] = in ? o[2 * p + j] : 0.0f;
-    dx[j] = in ? o[3 * p + j] : 0.0f, dy[j] = in ? o[4 * p + j] : 0.0f, dz[j] = in ? o[5 * p + j] : 0.0f;
-  }
-  float g0 = 0, g1 = 0, g2 = 0;
-  for (int c = 0; c < G.p; ++c) {
-    int iz = b.z + c;
-    iz -= iz >= G.k[2] ? G.k[2] : 0;
-    for (int bb = 0; bb < G.p; ++bb) {
-      int iy = b.y + bb;
-      iy -= iy >= G.k[1] ? G.k[1] : 0;
-      const float* row = grid + (size_t(iz) * G.k[1] + iy) * G.k[0];
-      float s0 = 0, s1 = 0;
-      for (int a = 0; a < G.p; ++a) {
-        int ix = b.x + a;
-        ix -= ix >= G.k[0] ? G.k[0] : 0;
-        const float v = row[ix];
-        s0 += dx[a] * v;
-        s1 += tx[a] * v;
-      }
-      g0 += s0 * ty[bb] * tz[c];
-      g1 += s1 * dy[bb] * tz[c];
-      g2 += s1 * ty[bb] * dz[c];
+// one 64-lane workgroup per 4x4x4 tile: the tile's box of (p + 3)³ mesh points in LDS (loaded row by row, wrapped),
+// then its atoms eight at a time, eight lanes per atom, each lane summing a share of the atom's p² rows; the eight
+// lanes' sums are added in a fixed order (butterfly) and the first lane writes the atom's force. Dynamic LDS: the box
+// (p + 3)³ floats, then per atom slot 6p weights.
+template <int P>
+__global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
+                                               const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ grid,
+                                               float* __restrict__ f) {
+  extern __shared__ float2 lds[];
+  const int lane = threadIdx.x;
+  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const int64_t a0 = tstart[tile], na = tstart[tile + 1] - a0;
+  if (na == 0) return;
+  constexpr int p = P, W = P + 3;
+  float* box = reinterpret_cast<float*>(lds);            // [W][W][W] (c, b, a)
+  float* s_w = box + W * W * W;                          // [8][6 P]: θa θb θc dθa dθb dθc
+  const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
+  for (int r = lane; r < W * W; r += 64) {
+    int yy = 4 * ty + r % W, zz = 4 * tz + r / W;
+    yy -= yy >= G.k[1] ? G.k[1] : 0;
+    yy -= yy >= G.k[1] ? G.k[1] : 0;
+    zz -= zz >= G.k[2] ? G.k[2] : 0;
+    zz -= zz >= G.k[2] ? G.k[2] : 0;
+    const float* src = grid + (size_t(zz) * G.k[1] + yy) * G.k[0];
+    for (int xx = 0; xx < W; ++xx) {
+      int gx = 4 * tx + xx;
+      gx -= gx >= G.k[0] ? G.k[0] : 0;
+      gx -= gx >= G.k[0] ? G.k[0] : 0;
+      box[r * W + xx] = src[gx];
     }
   }
-  for (int m = 0; m < 3; ++m) {
-    const double grad = G.k[0] * double(g0) * G.hi[0][m] + G.k[1] * double(g1) * G.hi[1][m] + G.k[2] * double(g2) * G.hi[2][m];
-    f[3 * i + m] = float(-double(w.w) * grad);
+  const int slot = lane >> 3, sub = lane & 7;
+  for (int64_t g0 = 0; g0 < na; g0 += 8) {
+    const int64_t ia = g0 + slot;
+    const bool has = ia < na;
+    const uint32_t i = has ? tatoms[a0 + ia] : 0u;
+    const float4 w = wq[i];
+    const int4 b = base[i];
+    float* ws = s_w + slot * 6 * P;
+    if (sub == 0) {
+      float th[P], dth[P];
+      bspline<P>(w.x, th, dth);
+#pragma unroll
+      for (int j = 0; j < P; ++j) ws[j] = th[j], ws[3 * P + j] = dth[j];
+      bspline<P>(w.y, th, dth);
+#pragma unroll
+      for (int j = 0; j < P; ++j) ws[P + j] = th[j], ws[4 * P + j] = dth[j];
+      bspline<P>(w.z, th, dth);
+#pragma unroll
+      for (int j = 0; j < P; ++j) ws[2 * P + j] = th[j], ws[5 * P + j] = dth[j];
+    }
+    __syncthreads();
+    // the atom's first point relative to the tile's corner (its home tile: 0 … 3 per axis)
+    const int ox = b.x - 4 * tx, oy = b.y - 4 * ty, oz = b.z - 4 * tz;
+    float ta[P], da[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) ta[j] = ws[j], da[j] = ws[3 * P + j];
+    float g0s = 0.0f, g1s = 0.0f, g2s = 0.0f;
+    if (has)
+      for (int r = sub; r < p * p; r += 8) {
+        const int dy = r % p, dz = r / p;
+        const float* row = box + ((oz + dz) * W + (oy + dy)) * W + ox;
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int a = 0; a < P; ++a) {
+          const float v = row[a];
+          s0 += da[a] * v;
+          s1 += ta[a] * v;
+        }
+        const float tyv = ws[P + dy], dyv = ws[4 * P + dy], tzv = ws[2 * P + dz], dzv = ws[5 * P + dz];
+        g0s += s0 * tyv * tzv;
+        g1s += s1 * dyv * tzv;
+        g2s += s1 * tyv * dzv;
+      }
+#pragma unroll
+    for (int d = 4; d >= 1; d >>= 1) {
+      g0s += __shfl_xor(g0s, d, 64);
+      g1s += __shfl_xor(g1s, d, 64);
+      g2s += __shfl_xor(g2s, d, 64);
+    }
+    if (has && sub == 0)
+      for (int m = 0; m < 3; ++m) {
+        const double grad = G.k[0] * double(g0s) * G.hi[0][m] + G.k[1] * double(g1s) * G.hi[1][m] + G.k[2] * double(g2s) * G.hi[2][m];
+        f[3 * size_t(i) + m] = float(-double(w.w) * grad);
+      }
+    __syncthreads();
   }
 }
 
 GridArgs grid_args(const Pme& P) {
   GridArgs G{};
   for (int d = 0; d < 3; ++d) G.k[d] = P.k[d], G.nt[d] = (P.k[d] + 3) / 4;
+  G.ntm = std::max(G.nt[0], std::max(G.nt[1], G.nt[2]));
   G.p = P.order;
   G.h1 = P.h1;
   for (int r = 0; r < 3; ++r)
@@ -615,6 +689,23 @@ void pme_setup(capsgpu_ctx& c) {
   hipLaunchKernelGGL(k_green, dim3(blocks_for(nh, 256)), dim3(256), 0, s, G, fac, kCoulomb, P.vol, P.bmod.p, P.gtab.p);
   LAUNCHCHECK();
   kt_end(c);
+  // per axis and tile T: the distinct home tiles of the first points that reach points 4T … 4T + 3 (count, then up to
+  // four tiles in the order of those first points)
+  std::vector<int32_t> home(size_t(3) * G.ntm * 5, 0);
+  for (int d = 0; d < 3; ++d)
+    for (int T = 0; T < G.nt[d]; ++T) {
+      int32_t* h = home.data() + (size_t(d) * G.ntm + T) * 5;
+      for (int bb = 4 * T - P.order + 1; bb <= 4 * T + 3; ++bb) {
+        const int t = (((bb % P.k[d]) + P.k[d]) % P.k[d]) >> 2;
+        bool seen = false;
+        for (int q = 0; q < h[0]; ++q) seen |= h[1 + q] == t;
+        if (!seen) {
+          if (h[0] == 4) throw Error(-1, "PME: more than four home tiles along an axis");
+          h[1 + h[0]++] = t;
+        }
+      }
+    }
+  P.home.upload(home, s);
   P.grid.resize(size_t(P.k[0]) * P.k[1] * P.k[2]);
   P.half.resize(nh);
   for (int q = 0; q < 12; ++q) P.cell[q] = cell[q];
@@ -636,25 +727,31 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   float4* wq = reinterpret_cast<float4*>(P.theta.p);
   int4* base = reinterpret_cast<int4*>(P.base.p);
   kt_begin(c, "k_splines");
-  P.wts.resize(6 * size_t(G.p) * size_t(n));
-  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p, P.wts.p);
+  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p);
   LAUNCHCHECK();
   kt_end(c);
   P.tstart.resize(size_t(ntiles) + 1);
   c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
   P.tfill.resize(size_t(ntiles));
   P.tfill.zero(s);
+  P.tile.resize(size_t(n));
   P.tatoms.resize(size_t(n));
   kt_begin(c, "k_tile_fill");
-  hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tfill.p, P.tatoms.p);
+  hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tfill.p, P.tile.p);
   LAUNCHCHECK();
   kt_end(c);
-  kt_begin(c, "k_tile_sort");
-  hipLaunchKernelGGL(k_tile_sort, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, ntiles, P.tstart.p, P.tatoms.p);
+  kt_begin(c, "k_tile_rank");
+  hipLaunchKernelGGL(k_tile_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tile.p, P.tatoms.p);
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_spread");
-  hipLaunchKernelGGL(k_spread, dim3(unsigned(ntiles)), dim3(64), 0, s, G, P.tstart.p, P.tatoms.p, wq, base, P.wts.p, P.grid.p);
+  const size_t lds_spread = size_t(kSpreadChunk) * size_t(G.p * G.p + G.p + 8) * sizeof(float);
+  switch (G.p) {
+#define CAPSGPU_SPREAD(Q) \
+  case Q: hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), lds_spread, s, G, P.home.p, P.tstart.p, P.tatoms.p, wq, base, P.grid.p); break;
+    CAPSGPU_SPREAD(3) CAPSGPU_SPREAD(4) CAPSGPU_SPREAD(5) CAPSGPU_SPREAD(6) CAPSGPU_SPREAD(7) CAPSGPU_SPREAD(8)
+#undef CAPSGPU_SPREAD
+  }
   LAUNCHCHECK();
   kt_end(c);
   const int npairs = (P.k[1] * P.k[2] + 1) / 2;
@@ -692,7 +789,13 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   kt_end(c);
   P.fpme.resize(3 * size_t(n));
   kt_begin(c, "k_gather");
-  hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, wq, base, P.wts.p, P.grid.p, P.fpme.p);
+  const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 3) + 8 * 6 * size_t(G.p)) * sizeof(float);
+  switch (G.p) {
+#define CAPSGPU_GATHER(Q) \
+  case Q: hipLaunchKernelGGL(k_gather<Q>, dim3(unsigned(ntiles)), dim3(64), lds_gather, s, G, P.tstart.p, P.tatoms.p, wq, base, P.grid.p, P.fpme.p); break;
+    CAPSGPU_GATHER(3) CAPSGPU_GATHER(4) CAPSGPU_GATHER(5) CAPSGPU_GATHER(6) CAPSGPU_GATHER(7) CAPSGPU_GATHER(8)
+#undef CAPSGPU_GATHER
+  }
   LAUNCHCHECK();
   kt_end(c);
 }

@@ -203,23 +203,30 @@ __global__ void __launch_bounds__(256) k_colsum(const double* __restrict__ part,
 
 // largest squared displacement against the positions of the last build (flag 0) and the last prune (flag 1), in
 // float-ordered bits (non-negative floats compare as their bit patterns)
+// flags[0] / flags[1]: some atom moved more than √lim_b / √lim_p since the list's build / prune (no check when the
+// limit is negative: no list); flags[3]: a position is not a number. One atomic per wave that finds something, and
+// none once the flag reads as set.
 __device__ inline void move_flags(bool valid, const double* x, const double* x0, const double* x0p, int i, double lim_b, double lim_p,
                                   uint32_t* flags) {
   bool mb = false, mp = false, bad = false;
   if (valid) {
-    double d0 = 0, d1 = 0;
-    for (int m = 0; m < 3; ++m) {
-      const double a = x[3 * i + m] - x0[3 * i + m], b = x[3 * i + m] - x0p[3 * i + m];
-      d0 += a * a, d1 += b * b;
+    if (lim_b >= 0.0) {
+      double d0 = 0, d1 = 0;
+      for (int m = 0; m < 3; ++m) {
+        const double a = x[3 * i + m] - x0[3 * i + m], b = x[3 * i + m] - x0p[3 * i + m];
+        d0 += a * a, d1 += b * b;
+      }
+      mb = d0 > lim_b, mp = d1 > lim_p;
+      bad = !isfinite(d0);
+    } else {
+      bad = !isfinite(x[3 * i] + x[3 * i + 1] + x[3 * i + 2]);
     }
-    mb = d0 > lim_b, mp = d1 > lim_p;
-    bad = !isfinite(d0);
   }
   const uint64_t bm = __ballot(mb), pm = __ballot(mp), nm = __ballot(bad);
   if ((threadIdx.x & 63) == 0) {
-    if (bm) atomicOr(&flags[0], 1u);
-    if (pm) atomicOr(&flags[1], 1u);
-    if (nm) atomicOr(&flags[3], 1u);
+    if (bm && !flags[0]) atomicOr(&flags[0], 1u);
+    if (pm && !flags[1]) atomicOr(&flags[1], 1u);
+    if (nm && !flags[3]) atomicOr(&flags[3], 1u);
   }
 }
 
