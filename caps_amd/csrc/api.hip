@@ -241,7 +241,7 @@ void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts,
       const int mblocks = (c.pme.h1 + 15) / 16;
       colsum(c, c.pme.epart.p, int64_t(mblocks) * c.pme.k[1], 7, out + 20);
     }
-    if (bonded) colsum(c, c.bonded.epart.p, blocks_for(c.n, 256), 9, out + 27);
+    if (bonded) colsum(c, c.bonded.epart.p, c.bonded.eblocks, 9, out + 27);
     HIPCHECK(hipMemcpyAsync(c.h_terms, out, 36 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
     std::memcpy(S.pair, c.h_terms, sizeof S.pair);

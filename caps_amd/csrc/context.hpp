@@ -104,10 +104,12 @@ struct Bonded {
   int nb = 0, na = 0, nf = 0;
   DevBuf<int32_t> bij, aijk, fij;
   DevBuf<double> bkr, akt, fp;
-  DevBuf<int32_t> aoff;                 // per atom: its terms (CSR)
-  DevBuf<int32_t> aterm;                // term id | kind << 28 (0 bond, 1 angle, 2 fene)
+  DevBuf<int32_t> aoff;                 // per atom: its slots (CSR)
+  DevBuf<int32_t> aslot;                // a member's slot: bond t 2t + k, angle t 2nb + 3t + k, FENE t 2nb + 3na + 2t + k
+  DevBuf<double> slot;                  // 3 per slot: the term's force on that member
   DevBuf<double> f;                     // 3 per atom
-  DevBuf<double> epart;                 // per block: bond, angle, virial, w[6]
+  DevBuf<double> epart;                 // per block of terms: bond, angle, virial, w[6]
+  int eblocks = 0;                      // rows of epart
 };
 
 

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
                                                const uint32_t* __restrict__ tatoms, const float4* __restrict__ wq,
                                                const int4* __restrict__ base, float* __restrict__ grid) {
   extern __shared__ float2 lds[];
-  __shared__ int s_r[kSpreadChunk][3];
+  __shared__ int s_r[kSpreadChunk];   // r_a + 3 | (r_b + 3) << 8 | (r_c + 3) << 16
   __shared__ float s_part[4][64];
   __shared__ int64_t s_hb[kMaxHome], s_pre[kMaxHome + 1];
   const int lane = threadIdx.x;
@@ -156,13 +156,19 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
       const uint32_t a = tatoms[s_hb[t] + (f - s_pre[t])];
       const int4 b = base[a];
       const float4 wv = wq[a];
+      // r = 4t − b lies in (−K, K + 3): to [0, K) and then into [−3, K − 4], and at most p (beyond, no point is reached)
       const int bb[3] = {b.x, b.y, b.z}, tt[3] = {tx, ty, tz};
+      int rp = 0;
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
-        int r = (4 * tt[d] - bb[d]) % G.k[d];
+        int r = 4 * tt[d] - bb[d];
         r += r < 0 ? G.k[d] : 0;
+        r -= r >= G.k[d] ? G.k[d] : 0;
         r -= r > G.k[d] - 4 ? G.k[d] : 0;
-        s_r[lane][d] = r < p ? r : p;
+        r = r < p ? r : p;
+        rp |= (r + 3) << (8 * d);
       }
+      s_r[lane] = rp;
       float ta[P], tb[P], tc[P], dd[P];
       bspline<P>(wv.x, ta, dd);
       bspline<P>(wv.y, tb, dd);
@@ -183,10 +189,11 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
     for (int j0 = 0; j0 < n_here; j0 += 4) {
       const int j = j0 + quarter;
       if (j < n_here) {
-        const int dy = gy + s_r[j][1], dz = gz + s_r[j][2];
+        const int rp = s_r[j];
+        const int dy = gy + ((rp >> 8) & 255) - 3, dz = gz + ((rp >> 16) & 255) - 3;
         if (unsigned(dy) < unsigned(p) && unsigned(dz) < unsigned(p)) {
           const float v = s_yz[j * pp + dy * p + dz];
-          const float* xr = s_x + j * px + s_r[j][0] + 3;
+          const float* xr = s_x + j * px + (rp & 255);
           acc0 += v * xr[0], acc1 += v * xr[1], acc2 += v * xr[2], acc3 += v * xr[3];
         }
       }
@@ -227,50 +234,111 @@ Fft1 make_fft(int n, int twoff) {
   return f;
 }
 
+// R-point DFTs, o[u] = Σ_q v[q] exp(sign 2πi qu/R), in place; radix 4 and 8 with their exact ±i and (±1 ± i)/√2
+// factors, the odd radices with cos/sin(2πm/R) as literals
+__device__ inline float2 mul_si(float2 a, int sign) { return sign > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x); }   // × sign·i
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline void dft4(float2& v0, float2& v1, float2& v2, float2& v3, int sign) {
+  const float2 a = cadd(v0, v2), b = csub(v0, v2), c = cadd(v1, v3), d = mul_si(csub(v1, v3), sign);
+  v0 = cadd(a, c), v2 = csub(a, c), v1 = cadd(b, d), v3 = csub(b, d);
+}
+template <int R>
+__device__ inline void dft(float2* v, int sign) {
+  if (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b), v[1] = csub(a, b);
+  } else if (R == 4) {
+    dft4(v[0], v[1], v[2], v[3], sign);
+  } else if (R == 8) {
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3, sign);
+    dft4(o0, o1, o2, o3, sign);
+    const float h = 0.70710678118654752f;   // √2 / 2
+    // o[u] × exp(sign 2πi u/8), u = 1, 2, 3
+    const float2 t1 = make_float2(h * (o1.x - sign * o1.y), h * (o1.y + sign * o1.x));
+    const float2 t2 = mul_si(o2, sign);
+    const float2 t3 = make_float2(-h * (o3.x + sign * o3.y), h * (sign * o3.x - o3.y));
+    v[0] = cadd(e0, o0), v[4] = csub(e0, o0);
+    v[1] = cadd(e1, t1), v[5] = csub(e1, t1);
+    v[2] = cadd(e2, t2), v[6] = csub(e2, t2);
+    v[3] = cadd(e3, t3), v[7] = csub(e3, t3);
+  } else {
+    // R = 3, 5, 7: cos and sin of 2πm/R
+    constexpr float C3[3] = {1.0f, -0.5f, -0.5f}, S3[3] = {0.0f, 0.86602540378443865f, -0.86602540378443865f};
+    constexpr float C5[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f, -0.80901699437494742f, 0.30901699437494742f};
+    constexpr float S5[5] = {0.0f, 0.95105651629515357f, 0.58778525229247313f, -0.58778525229247313f, -0.95105651629515357f};
+    constexpr float C7[7] = {1.0f, 0.62348980185873353f, -0.22252093395631440f, -0.90096886790241913f, -0.90096886790241913f,
+                             -0.22252093395631440f, 0.62348980185873353f};
+    constexpr float S7[7] = {0.0f, 0.78183148246802981f, 0.97492791218182361f, 0.43388373911755812f, -0.43388373911755812f,
+                             -0.97492791218182361f, -0.78183148246802981f};
+    const float* Cs = R == 3 ? C3 : R == 5 ? C5 : C7;
+    const float* Ss = R == 3 ? S3 : R == 5 ? S5 : S7;
+    float2 o[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        const int m = (q * u) % R;
+        const float c = Cs[m], sn = sign * Ss[m];
+        acc.x += v[q].x * c - v[q].y * sn;
+        acc.y += v[q].x * sn + v[q].y * c;
+      }
+      o[u] = acc;
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) v[u] = o[u];
+  }
+}
+
+// one Stockham stage of radix R over nl lines of length n: butterfly i of a line takes x_q = src[i + q n/R] × w^(q k),
+// w = exp(sign 2πi / (pR)), k = i mod p, and writes its DFT to dst[(i − k) R + k + q p]; twl holds exp(2πi m/n)
+template <int R>
+__device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, int p, const float2* twl, int sign) {
+  const int nb = n / R, step = n / (p * R);
+  for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
+    const int line = t / nb, i = t - line * nb;
+    const int k = i % p;
+    float2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const float2 x = src[line * n + i + q * nb];
+      if (q == 0) {
+        v[q] = x;
+      } else {
+        float2 w = twl[q * k * step];   // q k step < n
+        if (sign < 0) w.y = -w.y;
+        v[q] = cmul(x, w);
+      }
+    }
+    dft<R>(v, sign);
+    const int j = (i - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) dst[line * n + j + q * p] = v[q];
+  }
+}
+
+// the axis's twiddles exp(2πi m/n) into LDS (the caller synchronises before the transform)
+__device__ inline void stage_twiddles(float2* twl, const float2* __restrict__ tw, const Fft1& F) {
+  for (int t = threadIdx.x; t < F.n; t += blockDim.x) twl[t] = tw[F.tw + t];
+}
+
 // nl lines of length n in A (line-major), transformed by the whole workgroup; returns the buffer with the result
-__device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const float2* __restrict__ tw, int sign) {
+__device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const float2* twl, int sign) {
   float2* src = A;
   float2* dst = B;
   const int n = F.n;
   int p = 1;
   for (int s = 0; s < F.nr; ++s) {
-    const int R = F.rad[s], nb = n / R, step = n / (p * R), rstep = n / R;
-    for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
-      const int line = t / nb, i = t - line * nb;
-      const int k = i % p;
-      float2 v[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (q < R) {
-          float2 x = src[line * n + i + q * nb];
-          const int idx = (q * k * step) % n;
-          float2 w = tw[F.tw + idx];
-          if (sign < 0) w.y = -w.y;
-          v[q] = cmul(x, w);
-        }
-      }
-      float2 o[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (u < R) {
-          float2 acc = v[0];
-#pragma unroll
-          for (int q = 1; q < 8; ++q) {
-            if (q < R) {
-              const int idx = ((q * u) % R) * rstep;
-              float2 w = tw[F.tw + idx];
-              if (sign < 0) w.y = -w.y;
-              const float2 c = cmul(v[q], w);
-              acc.x += c.x, acc.y += c.y;
-            }
-          }
-          o[u] = acc;
-        }
-      }
-      const int j = (i - k) * R + k;
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < R) dst[line * n + j + q * p] = o[q];
+    const int R = F.rad[s];
+    switch (R) {
+      case 2: fft_stage<2>(src, dst, nl, n, p, twl, sign); break;
+      case 3: fft_stage<3>(src, dst, nl, n, p, twl, sign); break;
+      case 4: fft_stage<4>(src, dst, nl, n, p, twl, sign); break;
+      case 5: fft_stage<5>(src, dst, nl, n, p, twl, sign); break;
+      case 7: fft_stage<7>(src, dst, nl, n, p, twl, sign); break;
+      default: fft_stage<8>(src, dst, nl, n, p, twl, sign); break;
     }
     __syncthreads();
     float2* tmp = src;
@@ -280,14 +348,18 @@ __device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const 
   return src;
 }
 
+constexpr int kLinePairsX = 8;   // pairs of real lines per workgroup along a
+
 // real lines along a (index j + K2·k, pairs 2l, 2l+1) -> the half spectrum, as caps_f_r2c_lines (f90:392-423)
 __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
                                                    float2* __restrict__ half) {
   extern __shared__ float2 lds[];
   const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2;
-  const int LP = 256 / 64;   // line pairs per workgroup
+  const int LP = kLinePairsX;
   float2* A = lds;
   float2* B = lds + LP * K1;
+  float2* twl = B + LP * K1;
+  stage_twiddles(twl, tw, FP.ax[0]);
   const int pb = blockIdx.x * LP;
   for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
     const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
@@ -300,7 +372,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const
     A[pl * K1 + m] = z;
   }
   __syncthreads();
-  const float2* Z = fft_lines(A, B, LP, FP.ax[0], tw, +1);
+  const float2* Z = fft_lines(A, B, LP, FP.ax[0], twl, +1);
   for (int t = threadIdx.x; t < LP * h1; t += blockDim.x) {
     const int pl = t / h1, m = t - pl * h1, pr = pb + pl;
     if (pr >= npairs) continue;
@@ -320,9 +392,11 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
                                                    float* __restrict__ grid) {
   extern __shared__ float2 lds[];
   const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2;
-  const int LP = 256 / 64;
+  const int LP = kLinePairsX;
   float2* A = lds;
   float2* B = lds + LP * K1;
+  float2* twl = B + LP * K1;
+  stage_twiddles(twl, tw, FP.ax[0]);
   const int pb = blockIdx.x * LP;
   for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
     const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
@@ -339,7 +413,7 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
     A[pl * K1 + m] = z;
   }
   __syncthreads();
-  const float2* Z = fft_lines(A, B, LP, FP.ax[0], tw, -1);
+  const float2* Z = fft_lines(A, B, LP, FP.ax[0], twl, -1);
   for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
     const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
     if (pr >= npairs) continue;
@@ -360,12 +434,14 @@ __global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const flo
   const int nl = min(kLinesYZ, h1 - m0);
   float2* A = lds;
   float2* B = lds + kLinesYZ * K2;
+  float2* twl = B + kLinesYZ * K2;
+  stage_twiddles(twl, tw, FP.ax[1]);
   for (int t = threadIdx.x; t < kLinesYZ * K2; t += blockDim.x) {
     const int j = t / kLinesYZ, li = t % kLinesYZ;
     if (li < nl) A[li * K2 + j] = half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)];
   }
   __syncthreads();
-  const float2* Z = fft_lines(A, B, nl, FP.ax[1], tw, sign);
+  const float2* Z = fft_lines(A, B, nl, FP.ax[1], twl, sign);
   for (int t = threadIdx.x; t < kLinesYZ * K2; t += blockDim.x) {
     const int j = t / kLinesYZ, li = t % kLinesYZ;
     if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * K2 + j];
@@ -435,13 +511,15 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
   const int nl = min(kLinesYZ, h1 - m0);
   float2* A = lds;
   float2* B = lds + kLinesYZ * K3;
+  float2* twl = B + kLinesYZ * K3;
+  stage_twiddles(twl, tw, FP.ax[2]);
   const size_t zs = size_t(h1) * K2;
   for (int t = threadIdx.x; t < kLinesYZ * K3; t += blockDim.x) {
     const int k = t / kLinesYZ, li = t % kLinesYZ;
     if (li < nl) A[li * K3 + k] = half[size_t(m0 + li) + size_t(h1) * jy + zs * k];
   }
   __syncthreads();
-  float2* Z = fft_lines(A, B, nl, FP.ax[2], tw, +1);
+  float2* Z = fft_lines(A, B, nl, FP.ax[2], twl, +1);
   double e = 0, w[6] = {0, 0, 0, 0, 0, 0};
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int li = t / K3, k = t - li * K3, a = m0 + li;
@@ -460,7 +538,7 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
   }
   __syncthreads();
   float2* other = Z == A ? B : A;
-  const float2* R = fft_lines(Z, other, nl, FP.ax[2], tw, -1);
+  const float2* R = fft_lines(Z, other, nl, FP.ax[2], twl, -1);
   for (int t = threadIdx.x; t < kLinesYZ * K3; t += blockDim.x) {
     const int k = t / kLinesYZ, li = t % kLinesYZ;
     if (li < nl) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * K3 + k];
@@ -755,13 +833,14 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   LAUNCHCHECK();
   kt_end(c);
   const int npairs = (P.k[1] * P.k[2] + 1) / 2;
-  const size_t lds_x = 2 * 4 * size_t(P.k[0]) * sizeof(float2);
+  const size_t lds_x = (2 * kLinePairsX + 1) * size_t(P.k[0]) * sizeof(float2);
   kt_begin(c, "k_fft_x_r2c");
-  hipLaunchKernelGGL(k_fft_x_r2c, dim3(unsigned((npairs + 3) / 4)), dim3(256), lds_x, s, G, F, P.tw.p, P.grid.p, P.half.p);
+  hipLaunchKernelGGL(k_fft_x_r2c, dim3(unsigned((npairs + kLinePairsX - 1) / kLinePairsX)), dim3(256), lds_x, s, G, F, P.tw.p, P.grid.p,
+                     P.half.p);
   LAUNCHCHECK();
   kt_end(c);
   const int mblocks = (P.h1 + kLinesYZ - 1) / kLinesYZ;
-  const size_t lds_y = 2 * kLinesYZ * size_t(P.k[1]) * sizeof(float2), lds_z = 2 * kLinesYZ * size_t(P.k[2]) * sizeof(float2);
+  const size_t lds_y = (2 * kLinesYZ + 1) * size_t(P.k[1]) * sizeof(float2), lds_z = (2 * kLinesYZ + 1) * size_t(P.k[2]) * sizeof(float2);
   kt_begin(c, "k_fft_y");
   hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, +1);
   LAUNCHCHECK();
@@ -784,7 +863,8 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_fft_x_c2r");
-  hipLaunchKernelGGL(k_fft_x_c2r, dim3(unsigned((npairs + 3) / 4)), dim3(256), lds_x, s, G, F, P.tw.p, P.half.p, P.grid.p);
+  hipLaunchKernelGGL(k_fft_x_c2r, dim3(unsigned((npairs + kLinePairsX - 1) / kLinePairsX)), dim3(256), lds_x, s, G, F, P.tw.p, P.half.p,
+                     P.grid.p);
   LAUNCHCHECK();
   kt_end(c);
   P.fpme.resize(3 * size_t(n));

@@ -1,14 +1,16 @@
 // Bonded terms, force summation, integrator steps and deterministic reductions on gfx950, all in FP64.
 //
-//  k_bonded     one thread per atom: every term the atom is in is evaluated and the atom takes its own share, so no
-//               two threads write one force (no atomics); a term's energy and virial are counted by its first atom.
-//               Harmonic bond and angle (core/src/field.cpp:1980-2003), FENE + WCA (:2150-2180).
+//  k_bonded     one thread per term: the term's force on each member atom into that member's slot, energy and
+//               virial into per-block sums. Harmonic bond and angle (core/src/field.cpp:1980-2003), FENE + WCA
+//               (:2150-2180).
+//  k_bonded_sum per atom: its slots added in the order of its terms (no atomics, a fixed order)
 //  k_sum_forces the total force per atom: pair forces (by slot, float) + reciprocal (float) + bonded (double)
 //  k_vv_open    velocity Verlet's opening half kick and drift (core/src/dynamics.cpp:405-431, 657-666), Bussi's
 //               scaling of the previous step folded in
 //  k_baoab_open Langevin BAOAB's B, A, O, A (dynamics.cpp:613-621), O with counter-based normal noise
 //  k_kick       the closing half kick, with kinetic-energy partial sums on thermo steps
 //  k_colsum     column sums of per-block partials in a fixed order (bit-reproducible energies and virials)
+#include <algorithm>
 #include <cmath>
 
 #include "context.hpp"
@@ -55,83 +57,79 @@ __device__ inline void vir_add(double* A, const double* d, const double* f) {
 
 constexpr int kEB = 9;   // bond, angle, virial, w[6]
 
+// one thread per term (bonds, then angles, then FENE bonds): the term's force on each member atom into that member's
+// slot (bond t: slots 2t, 2t + 1; angle t: 2nb + 3t + 0…2; FENE t: 2nb + 3na + 2t + 0…1), energies and virial into
+// per-block sums. Every term is computed once; k_bonded_sum adds each atom's slots in a fixed order.
 template <bool kE>
-__global__ void __launch_bounds__(256) k_bonded(int n, CellArgs C, const double* __restrict__ x, const int32_t* __restrict__ aoff,
-                                                const int32_t* __restrict__ aterm, const int32_t* __restrict__ bij,
-                                                const double* __restrict__ bkr, const int32_t* __restrict__ aijk,
-                                                const double* __restrict__ akt, const int32_t* __restrict__ fij,
-                                                const double* __restrict__ fp, double* __restrict__ fout, double* __restrict__ epart,
-                                                uint32_t* __restrict__ flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs C, const double* __restrict__ x,
+                                                const int32_t* __restrict__ bij, const double* __restrict__ bkr,
+                                                const int32_t* __restrict__ aijk, const double* __restrict__ akt,
+                                                const int32_t* __restrict__ fij, const double* __restrict__ fp, double* __restrict__ slot,
+                                                double* __restrict__ epart, uint32_t* __restrict__ flags) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (i < n) {
-    double F[3] = {0, 0, 0};
-    for (int32_t e = aoff[i]; e < aoff[i + 1]; ++e) {
-      const int32_t code = aterm[e], kind = code >> 28, t = code & 0x0fffffff;
-      if (kind == 0 || kind == 2) {
-        const int32_t a = kind == 0 ? bij[2 * t] : fij[2 * t], b = kind == 0 ? bij[2 * t + 1] : fij[2 * t + 1];
-        double d[3] = {x[3 * b] - x[3 * a], x[3 * b + 1] - x[3 * a + 1], x[3 * b + 2] - x[3 * a + 2]};
-        min_image(C, d);
-        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        double en, dEdr;
-        if (kind == 0) {
-          const double k = bkr[2 * t], dr = r - bkr[2 * t + 1];
-          en = k * dr * dr;
-          dEdr = 2 * k * dr;
-        } else {
-          const double K = fp[4 * t], R0 = fp[4 * t + 1], eps = fp[4 * t + 2], sg = fp[4 * t + 3], q = r / R0;
-          if (q >= 1) {
-            atomicOr(&flags[2], 1u);
-            continue;
-          }
-          en = -0.5 * K * R0 * R0 * log(1 - q * q);
-          dEdr = K * r / (1 - q * q);
-          if (eps > 0 && r < 1.122462048309373 * sg) {
-            const double s2 = (sg / r) * (sg / r), s6 = s2 * s2 * s2;
-            en += 4 * eps * (s6 * s6 - s6) + eps;
-            dEdr += -24 * eps * (2 * s6 * s6 - s6) / r;
-          }
-        }
-        double fj[3] = {0, 0, 0};
-        if (r > 1e-12)
-          for (int m = 0; m < 3; ++m) fj[m] = d[m] * (-dEdr / r);
-        const double sgn = i == b ? 1.0 : -1.0;
-        for (int m = 0; m < 3; ++m) F[m] += sgn * fj[m];
-        if (kE && i == a) {
-          A[0] += en;
-          vir_add(A, d, fj);
-        }
+  if (g < nb || (g >= nb + na && g < nb + na + nf)) {
+    const bool fene = g >= nb;
+    const int t = fene ? g - nb - na : g;
+    const int32_t a = fene ? fij[2 * t] : bij[2 * t], b = fene ? fij[2 * t + 1] : bij[2 * t + 1];
+    double d[3] = {x[3 * b] - x[3 * a], x[3 * b + 1] - x[3 * a + 1], x[3 * b + 2] - x[3 * a + 2]};
+    min_image(C, d);
+    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double en = 0, dEdr = 0;
+    bool ok = true;
+    if (!fene) {
+      const double k = bkr[2 * t], dr = r - bkr[2 * t + 1];
+      en = k * dr * dr;
+      dEdr = 2 * k * dr;
+    } else {
+      const double K = fp[4 * t], R0 = fp[4 * t + 1], eps = fp[4 * t + 2], sg = fp[4 * t + 3], q = r / R0;
+      if (q >= 1) {
+        atomicOr(&flags[2], 1u);
+        ok = false;
       } else {
-        const int32_t ia = aijk[3 * t], ja = aijk[3 * t + 1], ka = aijk[3 * t + 2];
-        double u[3] = {x[3 * ia] - x[3 * ja], x[3 * ia + 1] - x[3 * ja + 1], x[3 * ia + 2] - x[3 * ja + 2]};
-        double v[3] = {x[3 * ka] - x[3 * ja], x[3 * ka + 1] - x[3 * ja + 1], x[3 * ka + 2] - x[3 * ja + 2]};
-        min_image(C, u);
-        min_image(C, v);
-        const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (lu * lv);
-        cs = cs < -1 ? -1 : cs > 1 ? 1 : cs;
-        const double th = acos(cs), sn = fmax(sqrt(1 - cs * cs), 1e-8);
-        const double dth = th - akt[2 * t + 1], kt = akt[2 * t];
-        const double g = 2 * kt * dth / sn;
-        double fi[3], fk[3];
-        for (int m = 0; m < 3; ++m) {
-          fi[m] = (v[m] * (1 / (lu * lv)) - u[m] * (cs / (lu * lu))) * g;
-          fk[m] = (u[m] * (1 / (lu * lv)) - v[m] * (cs / (lv * lv))) * g;
-        }
-        if (i == ia)
-          for (int m = 0; m < 3; ++m) F[m] += fi[m];
-        else if (i == ka)
-          for (int m = 0; m < 3; ++m) F[m] += fk[m];
-        else
-          for (int m = 0; m < 3; ++m) F[m] -= fi[m] + fk[m];
-        if (kE && i == ia) {
-          A[1] += kt * dth * dth;
-          vir_add(A, u, fi);
-          vir_add(A, v, fk);
+        en = -0.5 * K * R0 * R0 * log(1 - q * q);
+        dEdr = K * r / (1 - q * q);
+        if (eps > 0 && r < 1.122462048309373 * sg) {
+          const double s2 = (sg / r) * (sg / r), s6 = s2 * s2 * s2;
+          en += 4 * eps * (s6 * s6 - s6) + eps;
+          dEdr += -24 * eps * (2 * s6 * s6 - s6) / r;
         }
       }
     }
-    for (int m = 0; m < 3; ++m) fout[3 * i + m] = F[m];
+    double fj[3] = {0, 0, 0};
+    if (ok && r > 1e-12)
+      for (int m = 0; m < 3; ++m) fj[m] = d[m] * (-dEdr / r);
+    double* o = slot + 3 * (fene ? 2 * size_t(nb) + 3 * size_t(na) + 2 * size_t(t) : 2 * size_t(t));
+    for (int m = 0; m < 3; ++m) o[m] = -fj[m], o[3 + m] = fj[m];
+    if (kE && ok) {
+      A[0] += en;
+      vir_add(A, d, fj);
+    }
+  } else if (g >= nb && g < nb + na) {
+    const int t = g - nb;
+    const int32_t ia = aijk[3 * t], ja = aijk[3 * t + 1], ka = aijk[3 * t + 2];
+    double u[3] = {x[3 * ia] - x[3 * ja], x[3 * ia + 1] - x[3 * ja + 1], x[3 * ia + 2] - x[3 * ja + 2]};
+    double v[3] = {x[3 * ka] - x[3 * ja], x[3 * ka + 1] - x[3 * ja + 1], x[3 * ka + 2] - x[3 * ja + 2]};
+    min_image(C, u);
+    min_image(C, v);
+    const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (lu * lv);
+    cs = cs < -1 ? -1 : cs > 1 ? 1 : cs;
+    const double th = acos(cs), sn = fmax(sqrt(1 - cs * cs), 1e-8);
+    const double dth = th - akt[2 * t + 1], kt = akt[2 * t];
+    const double gg = 2 * kt * dth / sn;
+    double fi[3], fk[3];
+    for (int m = 0; m < 3; ++m) {
+      fi[m] = (v[m] * (1 / (lu * lv)) - u[m] * (cs / (lu * lu))) * gg;
+      fk[m] = (u[m] * (1 / (lu * lv)) - v[m] * (cs / (lv * lv))) * gg;
+    }
+    double* o = slot + 3 * (2 * size_t(nb) + 3 * size_t(t));
+    for (int m = 0; m < 3; ++m) o[m] = fi[m], o[3 + m] = -(fi[m] + fk[m]), o[6 + m] = fk[m];
+    if (kE) {
+      A[1] += kt * dth * dth;
+      vir_add(A, u, fi);
+      vir_add(A, v, fk);
+    }
   }
   if (kE) {
     __shared__ double red[kEB][256];
@@ -144,6 +142,32 @@ __global__ void __launch_bounds__(256) k_bonded(int n, CellArgs C, const double*
     }
     if (threadIdx.x < kEB) epart[kEB * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
   }
+}
+
+// each atom's bonded force: its slots summed in the order of its terms
+__global__ void __launch_bounds__(256) k_bonded_sum(int n, const int32_t* __restrict__ aoff, const int32_t* __restrict__ aslot,
+                                                    const double* __restrict__ slot, double* __restrict__ fout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double F[3] = {0, 0, 0};
+  const int32_t e0 = aoff[i], ne = aoff[i + 1] - e0;
+  // the first eight slots' indexes, then their forces, loaded together; more than eight in a loop
+  int32_t sl[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sl[k] = k < ne ? aslot[e0 + k] : -1;
+  double v[8][3];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    for (int m = 0; m < 3; ++m) v[k][m] = sl[k] >= 0 ? slot[3 * size_t(sl[k]) + m] : 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k < ne)
+      for (int m = 0; m < 3; ++m) F[m] += v[k][m];
+  for (int32_t e = e0 + 8; e < e0 + ne; ++e) {
+    const double* o = slot + 3 * size_t(aslot[e]);
+    for (int m = 0; m < 3; ++m) F[m] += o[m];
+  }
+  for (int m = 0; m < 3; ++m) fout[3 * i + m] = F[m];
 }
 
 __global__ void k_sum_forces(int n, const uint32_t* __restrict__ iperm, const float4* __restrict__ fnb, const float* __restrict__ frec,
@@ -407,12 +431,14 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
   auto chk = [&](int32_t a) {
     if (a < 0 || a >= n) throw Error(-1, "a bonded term names an atom out of range");
   };
+  // per atom: its slots (bond t, member k: 2t + k; angle t: 2nb + 3t + k; FENE t: 2nb + 3na + 2t + k), terms in order
   for (int t = 0; t < B.nb; ++t)
-    for (int k = 0; k < 2; ++k) chk(ff.bond_atoms[2 * t + k]), per[size_t(ff.bond_atoms[2 * t + k])].push_back(t);
+    for (int k = 0; k < 2; ++k) chk(ff.bond_atoms[2 * t + k]), per[size_t(ff.bond_atoms[2 * t + k])].push_back(2 * t + k);
   for (int t = 0; t < B.na; ++t)
-    for (int k = 0; k < 3; ++k) chk(ff.angle_atoms[3 * t + k]), per[size_t(ff.angle_atoms[3 * t + k])].push_back(t | (1 << 28));
+    for (int k = 0; k < 3; ++k) chk(ff.angle_atoms[3 * t + k]), per[size_t(ff.angle_atoms[3 * t + k])].push_back(2 * B.nb + 3 * t + k);
   for (int t = 0; t < B.nf; ++t)
-    for (int k = 0; k < 2; ++k) chk(ff.fene_atoms[2 * t + k]), per[size_t(ff.fene_atoms[2 * t + k])].push_back(t | (2 << 28));
+    for (int k = 0; k < 2; ++k)
+      chk(ff.fene_atoms[2 * t + k]), per[size_t(ff.fene_atoms[2 * t + k])].push_back(2 * B.nb + 3 * B.na + 2 * t + k);
   std::vector<int32_t> off(size_t(n) + 1, 0), idx;
   for (int i = 0; i < n; ++i) {
     off[size_t(i) + 1] = off[size_t(i)] + int32_t(per[size_t(i)].size());
@@ -420,7 +446,8 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
   }
   if (idx.empty()) idx.push_back(0);
   B.aoff.upload(off, s);
-  B.aterm.upload(idx, s);
+  B.aslot.upload(idx, s);
+  B.slot.resize(3 * (2 * size_t(B.nb) + 3 * size_t(B.na) + 2 * size_t(B.nf)) + 3);
   auto up_i = [&](DevBuf<int32_t>& d, const int32_t* p, size_t m) {
     if (m) d.upload(p, m, s);
     else d.resize(1);
@@ -440,18 +467,23 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
 
 void bonded_forces(capsgpu_ctx& c, bool want) {
   Bonded& B = c.bonded;
-  const int n = c.n;
+  const int n = c.n, nt = B.nb + B.na + B.nf;
   const CellArgs C = cell_args(c);
-  const unsigned nb = blocks_for(n, 256);
+  const unsigned tb = blocks_for(size_t(std::max(nt, 1)), 256);
+  B.eblocks = int(tb);
   kt_begin(c, "k_bonded");
   if (want) {
-    B.epart.resize(kEB * size_t(nb));
-    hipLaunchKernelGGL(k_bonded<true>, dim3(nb), dim3(256), 0, c.stream, n, C, c.x.p, B.aoff.p, B.aterm.p, B.bij.p, B.bkr.p, B.aijk.p,
-                       B.akt.p, B.fij.p, B.fp.p, B.f.p, B.epart.p, c.flags.p);
+    B.epart.resize(kEB * size_t(tb));
+    hipLaunchKernelGGL(k_bonded<true>, dim3(tb), dim3(256), 0, c.stream, B.nb, B.na, B.nf, C, c.x.p, B.bij.p, B.bkr.p, B.aijk.p, B.akt.p,
+                       B.fij.p, B.fp.p, B.slot.p, B.epart.p, c.flags.p);
   } else {
-    hipLaunchKernelGGL(k_bonded<false>, dim3(nb), dim3(256), 0, c.stream, n, C, c.x.p, B.aoff.p, B.aterm.p, B.bij.p, B.bkr.p, B.aijk.p,
-                       B.akt.p, B.fij.p, B.fp.p, B.f.p, nullptr, c.flags.p);
+    hipLaunchKernelGGL(k_bonded<false>, dim3(tb), dim3(256), 0, c.stream, B.nb, B.na, B.nf, C, c.x.p, B.bij.p, B.bkr.p, B.aijk.p, B.akt.p,
+                       B.fij.p, B.fp.p, B.slot.p, nullptr, c.flags.p);
   }
+  LAUNCHCHECK();
+  kt_end(c);
+  kt_begin(c, "k_bonded_sum");
+  hipLaunchKernelGGL(k_bonded_sum, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, B.aoff.p, B.aslot.p, B.slot.p, B.f.p);
   LAUNCHCHECK();
   kt_end(c);
 }

@@ -29,18 +29,21 @@ def _oracle_recip(s, rc, rtol, spacing, order):
     return e, f, v, grid
 
 
-CASES = [("melt order 4", "melt", 4), ("melt order 5", "melt", 5), ("triclinic order 5", "tri", 5), ("triclinic order 6", "tri", 6)]
+# the grids: 27 (radix 3), 24 (8 · 3); spacing 0.7 on the triclinic cell gives 35 × 32 × 35 (radices 5, 7, 8, 4) and
+# 0.45 on the melt 56 (8 · 7)
+CASES = [("melt order 4", "melt", 4, 1.0), ("melt order 5", "melt", 5, 1.0), ("triclinic order 5", "tri", 5, 1.0),
+         ("triclinic order 6", "tri", 6, 1.0), ("triclinic order 5 radix 5 and 7", "tri", 5, 0.7), ("melt order 8 radix 7", "melt", 8, 0.45)]
 
 
-@pytest.mark.parametrize("name,which,order", CASES, ids=[c[0] for c in CASES])
-def test_reciprocal_part_matches_the_oracle(name, which, order):
+@pytest.mark.parametrize("name,which,order,spacing", CASES, ids=[c[0] for c in CASES])
+def test_reciprocal_part_matches_the_oracle(name, which, order, spacing):
     s = H.melt_case(chains=16) if which == "melt" else H.triclinic_case(n=1100)
     print("case", name, s.ff.n, flush=True)
     rc = 10.0
-    op = ca.EnergyOptions(cutoff=rc, coulomb=True, electrostatics="pme", ewald_rtol=1e-5, pme_order=order, pme_spacing=1.0,
+    op = ca.EnergyOptions(cutoff=rc, coulomb=True, electrostatics="pme", ewald_rtol=1e-5, pme_order=order, pme_spacing=spacing,
                           tail=False, parts=4)
     t, f = ca.Evaluator(s.ff, op).compute(s.x, s.cell)
-    e, fo, vo, grid = _oracle_recip(s, rc, 1e-5, 1.0, order)
+    e, fo, vo, grid = _oracle_recip(s, rc, 1e-5, spacing, order)
     print("grid", grid, "E", t.coulomb, e, flush=True)
     assert t.coulomb == pytest.approx(e, rel=1e-5)
     assert np.sqrt(((f - fo) ** 2).sum() / (fo ** 2).sum()) < 1e-5
