@@ -114,6 +114,7 @@ def lib():
         L.capsgpu_profile.argtypes = [C.c_void_p, C.c_int32]
         L.capsgpu_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.capsgpu_profile_kernels.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        L.capsgpu_profile_only.argtypes = [C.c_void_p, C.c_char_p]
         L.capsgpu_learned_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         L.capsgpu_learned_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                                C.POINTER(C.c_double)]
@@ -303,6 +304,10 @@ class Evaluator:
     def profile(self, on=True):
         """start (zeroing the sums) or stop the stage profiling of every later evaluation, dynamics included"""
         _check(lib().capsgpu_profile(self._h, int(on)))
+
+    def profile_only(self, kernel=None):
+        """time only the launches of one kernel (None: every kernel) and no stages"""
+        _check(lib().capsgpu_profile_only(self._h, kernel.encode() if kernel else None))
 
     def profile_read(self):
         """{stage: (summed device ms, evaluations)} since profiling started"""

@@ -155,7 +155,7 @@ void prof_harvest(capsgpu_ctx& c) {
 
 // a set of events for the next profiled evaluation
 void prof_begin(capsgpu_ctx& c) {
-  if (!c.profile || c.timing) return;
+  if (!c.profile || c.timing || !c.kfilter.empty()) return;
   if (c.pev_used == c.pev.size()) {
     if (c.pev.size() >= 1024) {
       prof_harvest(c);
@@ -170,7 +170,7 @@ void prof_begin(capsgpu_ctx& c) {
 
 void ev_record(capsgpu_ctx& c, int k) {
   if (c.timing) HIPCHECK(hipEventRecord(c.ev[k], c.stream));
-  else if (c.profile) HIPCHECK(hipEventRecord(c.pev[c.pslot][size_t(k)], c.stream));
+  else if (c.profile && c.kfilter.empty()) HIPCHECK(hipEventRecord(c.pev[c.pslot][size_t(k)], c.stream));
 }
 
 // forces at the device positions c.x into c.f; the energy terms into t when want. The list follows the reference's
@@ -472,6 +472,14 @@ int32_t capsgpu_profile(capsgpu_ctx* c, int32_t on) {
     c->profile = on != 0;
     for (int k = 0; k < 6; ++k) c->prof_ms[k] = 0, c->prof_n[k] = 0;
     c->ktot.clear();
+  });
+}
+
+int32_t capsgpu_profile_only(capsgpu_ctx* c, const char* kernel) {
+  return guard([&] {
+    if (!c) throw Error(CAPSGPU_EINVAL, "null context");
+    prof_harvest(*c);
+    c->kfilter = kernel ? kernel : "";
   });
 }
 

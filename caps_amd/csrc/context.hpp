@@ -151,6 +151,8 @@ struct capsgpu_ctx {
   hipEvent_t ev[8] = {};
   // stage profiling of every evaluation (dynamics included): event sets recorded on the stream, summed when harvested
   bool profile = false;
+  std::string kfilter;                  // when not empty: only launches of this kernel are timed, and no stages
+  bool kt_open = false;                 // a launch timer is open (kt_begin recorded its first event)
   std::vector<std::array<hipEvent_t, 7>> pev;
   size_t pev_used = 0, pslot = 0;
   double prof_ms[6] = {0, 0, 0, 0, 0, 0};

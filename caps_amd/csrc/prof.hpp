@@ -27,7 +27,7 @@ inline void kt_harvest(capsgpu_ctx& c) {
 
 // before a launch: an event pair for it when profiling
 inline void kt_begin(capsgpu_ctx& c, const char* name) {
-  if (!c.profile) return;
+  if (!c.profile || (!c.kfilter.empty() && c.kfilter != name)) return;
   if (c.kev_used == c.kev.size()) {
     if (c.kev.size() >= 8192) {
       kt_harvest(c);
@@ -40,13 +40,15 @@ inline void kt_begin(capsgpu_ctx& c, const char* name) {
   }
   c.kev[c.kev_used].name = name;
   HIPCHECK(hipEventRecord(c.kev[c.kev_used].a, c.stream));
+  c.kt_open = true;
 }
 
 // after the launch
 inline void kt_end(capsgpu_ctx& c) {
-  if (!c.profile) return;
+  if (!c.profile || !c.kt_open) return;
   HIPCHECK(hipEventRecord(c.kev[c.kev_used].b, c.stream));
   ++c.kev_used;
+  c.kt_open = false;
 }
 
 }  // namespace capsgpu

@@ -174,6 +174,9 @@ int32_t capsgpu_profile_read(capsgpu_ctx* ctx, double ms[6], int64_t counts[6]);
 /* per-kernel launch times since profiling started, as text: one "name summed_ms launches" line per kernel, truncated
  * to cap − 1 characters and NUL-terminated */
 int32_t capsgpu_profile_kernels(capsgpu_ctx* ctx, char* buf, int32_t cap);
+/* time only the launches of one kernel (by name, e.g. "k_spread"; NULL or "" for every kernel) and no stages, so a
+ * profiled run keeps two events per launch of that kernel and nothing else (no counterpart in the reference) */
+int32_t capsgpu_profile_only(capsgpu_ctx* ctx, const char* kernel);
 
 /* ---- learned r-RESPA inspection, for the parity checks (no counterpart in the reference) --------------------------- */
 /* The network (96-128-128-3, SiLU; BF16 MFMA, FP32 accumulation) at the given FP32 parameters, laid out w1 [128][96],
