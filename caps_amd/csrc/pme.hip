@@ -103,27 +103,28 @@ __global__ void k_tile_rank(int n, const int4* __restrict__ base, const int64_t*
 }
 
 // one 64-lane workgroup per 4x4x4 tile. The atoms whose first point lies in a tile that can reach this one (the home
-// table lists those tiles per axis) are staged in LDS 64 at a time: per atom q·θb[dy]·θc[dz] for every (dy, dz), θa
-// with three zeros before and five after it, and the offsets r_d = 4 t_d − b_d of its first point from the tile's
-// corner (folded into [−3, p]), so a point (i, j, k) of the tile sees θa[i + r_a], θb[j + r_b], θc[k + r_c]. Lane =
-// (row (j, k), quarter of the atoms): it sums its quarter's terms for the row's 4 points; the quarters are then added
-// in a fixed order (every point written once, no atomics). Dynamic LDS: kSpreadChunk · (p² + p + 8) floats.
+// table lists those tiles per axis) are staged in LDS 64 at a time. With r_d = 4 t_d − b_d the offset of an atom's
+// first point from the tile's corner (folded into [−3, p]), a point (i, j, k) of the tile sees θa[i + r_a],
+// θb[j + r_b], θc[k + r_c]; so per atom: q·θb[dy]·θc[dz] for every (dy, dz), the four θa[i + r_a] of the tile's points
+// along a, and a 16-bit mask of the tile's rows (j, k) it reaches. Lane = (row (j, k), quarter of the atoms): it sums
+// its quarter's terms for the row's 4 points; the quarters are then added in a fixed order (every point written once,
+// no atomics). Dynamic LDS: kSpreadChunk · p² floats.
 constexpr int kSpreadChunk = 64, kMaxHome = 64;   // at most 4 home tiles per axis reach a tile (p <= 8, partial last tile)
 template <int P>
 __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
                                                const uint32_t* __restrict__ tatoms, const float4* __restrict__ wq,
                                                const int4* __restrict__ base, float* __restrict__ grid) {
   extern __shared__ float2 lds[];
-  __shared__ int s_r[kSpreadChunk];   // r_a + 3 | (r_b + 3) << 8 | (r_c + 3) << 16
+  __shared__ float4 s_wx[kSpreadChunk];   // θa[r_a + i] for the tile's four points i along a (zero outside the support)
+  __shared__ int2 s_ent[kSpreadChunk];    // {j p² + r_b p + r_c + kYzBias, the rows (j, k) of the tile the atom reaches}
   __shared__ float s_part[4][64];
   __shared__ int64_t s_hb[kMaxHome], s_pre[kMaxHome + 1];
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
-  constexpr int p = P, pp = P * P, px = P + 8;
-  float* s_yz = reinterpret_cast<float*>(lds);          // [chunk][p²]
-  float* s_x = s_yz + kSpreadChunk * pp;                 // [chunk][p + 8]: 0 0 0 θa[0 … p−1] 0 0 0 0 0
+  constexpr int p = P, pp = P * P, kYzBias = 4 * P + 4;   // −(r_b p + r_c) ≤ 3p + 3 < kYzBias
+  float* s_yz = reinterpret_cast<float*>(lds);          // [chunk][p²]: q θb[dy] θc[dz]
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
   // the candidate tiles in a fixed order (c, b, a) from the home table, their atom counts in parallel, a running sum
   const int* hx = home + (0 * G.ntm + tx) * 5;
@@ -146,7 +147,9 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
   if (lane == 0) s_pre[0] = 0;
   __syncthreads();
   const int64_t m = s_pre[nh];
-  const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4;
+  // lane = (row (j, k) of the tile, quarter of the atoms); its row's bit in the reach mask is j + 4 k = lane & 15
+  const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4, rowbit = 1 << (lane & 15);
+  const int lane_yz = gy * P + gz - kYzBias;
   float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
   for (int64_t c0 = 0; c0 < m; c0 += kSpreadChunk) {
     const int64_t f = c0 + lane;
@@ -158,25 +161,37 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
       const float4 wv = wq[a];
       // r = 4t − b lies in (−K, K + 3): to [0, K) and then into [−3, K − 4], and at most p (beyond, no point is reached)
       const int bb[3] = {b.x, b.y, b.z}, tt[3] = {tx, ty, tz};
-      int rp = 0;
+      int r[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        int r = 4 * tt[d] - bb[d];
-        r += r < 0 ? G.k[d] : 0;
-        r -= r >= G.k[d] ? G.k[d] : 0;
-        r -= r > G.k[d] - 4 ? G.k[d] : 0;
-        r = r < p ? r : p;
-        rp |= (r + 3) << (8 * d);
+        int v = 4 * tt[d] - bb[d];
+        v += v < 0 ? G.k[d] : 0;
+        v -= v >= G.k[d] ? G.k[d] : 0;
+        v -= v > G.k[d] - 4 ? G.k[d] : 0;
+        r[d] = v < p ? v : p;
       }
-      s_r[lane] = rp;
       float ta[P], tb[P], tc[P], dd[P];
       bspline<P>(wv.x, ta, dd);
       bspline<P>(wv.y, tb, dd);
       bspline<P>(wv.z, tc, dd);
-      float* sx = s_x + lane * px;
+      float wx[4];
 #pragma unroll
-      for (int k = 0; k < px; ++k) sx[k] = (k >= 3 && k < 3 + P) ? ta[(k >= 3 && k < 3 + P) ? k - 3 : 0] : 0.0f;
-      float* syz = s_yz + lane * pp;
+      for (int i = 0; i < 4; ++i) {
+        float w = 0.0f;
+#pragma unroll
+        for (int k = 0; k < P; ++k) w = (i + r[0] == k) ? ta[k] : w;
+        wx[i] = w;
+      }
+      s_wx[lane] = make_float4(wx[0], wx[1], wx[2], wx[3]);
+      // the rows (j, k) with 0 ≤ j + r_b < p and 0 ≤ k + r_c < p
+      int mb = 0, mc = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) mb |= (unsigned(i + r[1]) < unsigned(P) ? 1 : 0) << i, mc |= (unsigned(i + r[2]) < unsigned(P) ? 1 : 0) << i;
+      int reach = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) reach |= ((mc >> k) & 1) ? mb << (4 * k) : 0;
+      s_ent[lane] = make_int2(int(f - c0) * pp + r[1] * P + r[2] + kYzBias, reach);
+      float* syz = s_yz + (f - c0) * pp;
 #pragma unroll
       for (int dy = 0; dy < P; ++dy) {
         const float qb = wv.w * tb[dy];
@@ -189,12 +204,11 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
     for (int j0 = 0; j0 < n_here; j0 += 4) {
       const int j = j0 + quarter;
       if (j < n_here) {
-        const int rp = s_r[j];
-        const int dy = gy + ((rp >> 8) & 255) - 3, dz = gz + ((rp >> 16) & 255) - 3;
-        if (unsigned(dy) < unsigned(p) && unsigned(dz) < unsigned(p)) {
-          const float v = s_yz[j * pp + dy * p + dz];
-          const float* xr = s_x + j * px + (rp & 255);
-          acc0 += v * xr[0], acc1 += v * xr[1], acc2 += v * xr[2], acc3 += v * xr[3];
+        const int2 e = s_ent[j];
+        if (e.y & rowbit) {
+          const float v = s_yz[e.x + lane_yz];
+          const float4 w = s_wx[j];
+          acc0 += v * w.x, acc1 += v * w.y, acc2 += v * w.z, acc3 += v * w.w;
         }
       }
     }
@@ -823,7 +837,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_spread");
-  const size_t lds_spread = size_t(kSpreadChunk) * size_t(G.p * G.p + G.p + 8) * sizeof(float);
+  const size_t lds_spread = size_t(kSpreadChunk) * size_t(G.p * G.p) * sizeof(float);
   switch (G.p) {
 #define CAPSGPU_SPREAD(Q) \
   case Q: hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), lds_spread, s, G, P.home.p, P.tstart.p, P.tatoms.p, wq, base, P.grid.p); break;

@@ -41,3 +41,18 @@ def test_fene_past_r0_is_a_field_error():
         ev.compute(s.x, s.cell)
     t, f = ev.compute(s2.x, s2.cell)
     assert np.isfinite(f).all()
+
+
+@pytest.mark.parametrize("which", ["262k melt", "kg 10k"])
+def test_bonded_terms_at_full_size(which):
+    # the 262,144-atom melt (bonds and angles) and the 10,000-bead Kremer-Grest melt (FENE + WCA) of the benchmark
+    # configurations
+    s = ca.build_melt(2048, seed=1) if which == "262k melt" else ca.kremer_grest(chains=200, beads=50, density=0.85, seed=1)
+    print("case full-size", which, s.ff.n, flush=True)
+    t, f = _bonded_only(s)
+    e, fo, wo = H.oracle_bonded(s)
+    # both FP64; the sums over ~2.6e5 terms run in different orders, so the energies agree to about N·ε ≈ 3e-11
+    assert t.bond == pytest.approx(e[0], rel=1e-10, abs=1e-9)
+    assert t.angle == pytest.approx(e[1], rel=1e-10, abs=1e-9)
+    assert t.virial == pytest.approx(e[2], rel=1e-10, abs=1e-8)
+    assert np.abs(f - fo).max() <= 1e-10 * np.abs(fo).max() + 1e-10

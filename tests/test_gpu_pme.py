@@ -60,3 +60,20 @@ def test_reciprocal_forces_follow_the_atoms():
     t1, f1 = ev.compute(s.x + s.cell[3:6], s.cell)
     assert t1.coulomb == pytest.approx(t0.coulomb, rel=1e-6)
     assert np.sqrt(((f1 - f0) ** 2).sum() / (f0 ** 2).sum()) < 1e-5
+
+
+@pytest.mark.parametrize("chains,order", [(256, 4), (2048, 5)], ids=["32k order 4 on 64^3", "262k order 5 on 128^3"])
+def test_reciprocal_part_at_full_size(chains, order):
+    # the 32,768- and 262,144-atom melts of the benchmark workloads, against the oracle's FP64 PME (seconds on the CPU)
+    s = ca.build_melt(chains, seed=1)
+    print("case full-size melt", s.ff.n, "order", order, flush=True)
+    rc = 12.0
+    op = ca.EnergyOptions(cutoff=rc, coulomb=True, electrostatics="pme", ewald_rtol=1e-5, pme_order=order, pme_spacing=1.0,
+                          tail=False, parts=4)
+    t, f = ca.Evaluator(s.ff, op).compute(s.x, s.cell)
+    e, fo, vo, grid = _oracle_recip(s, rc, 1e-5, 1.0, order)
+    err = np.sqrt(((f - fo) ** 2).sum() / (fo ** 2).sum())
+    print("grid", grid, "E", t.coulomb, e, "force rel RMS", err, flush=True)
+    assert t.coulomb == pytest.approx(e, rel=1e-5)
+    assert err < 1e-5
+    assert np.abs(np.array(t.w) - vo).max() < 1e-5 * np.abs(vo).max()
