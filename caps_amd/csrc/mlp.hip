@@ -452,93 +452,107 @@ __global__ void k_feature_scale(int n, const float* __restrict__ D, const float*
 
 // the network on every atom's input (D − D0)·s, fused: one workgroup of four wavefronts per 128 atoms, the weights in
 // LDS, the hidden layers kept on chip (BF16 through a per-wavefront LDS tile between layers); Y [n][3] FP32 without b3
-constexpr int kInferWaves = 4, kW1Row = kIn + 8, kWRow = kHid + 8;
+constexpr int kInferWaves = 8, kW1Row = kIn + 8, kWRow = kHid + 8;
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-__global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* __restrict__ D, const float* __restrict__ D0, const float* __restrict__ sc,
-                                                   const __bf16* __restrict__ W1, const float* __restrict__ b1, const __bf16* __restrict__ W2,
-                                                   const float* __restrict__ b2, const __bf16* __restrict__ W3, float* __restrict__ Y) {
-  __shared__ __bf16 sW1[kHid * kW1Row];
-  __shared__ __bf16 sW2[kHid * kWRow];
-  __shared__ __bf16 sW3[kOutPad * kWRow];
-  __shared__ __bf16 sH[kInferWaves][32 * kWRow];
+__global__ void __launch_bounds__(64 * kInferWaves) k_mlp_infer(int n, const float* __restrict__ D, const float* __restrict__ D0,
+                                                                const float* __restrict__ sc, const __bf16* __restrict__ W1,
+                                                                const float* __restrict__ b1, const __bf16* __restrict__ W2,
+                                                                const float* __restrict__ b2, const __bf16* __restrict__ W3,
+                                                                float* __restrict__ Y) {
+  __shared__ __attribute__((aligned(16))) __bf16 sW1[kHid * kW1Row];
+  __shared__ __attribute__((aligned(16))) __bf16 sW2[kHid * kWRow];
+  __shared__ __attribute__((aligned(16))) __bf16 sW3[kOutPad * kWRow];
+  __shared__ __attribute__((aligned(16))) __bf16 sH[kInferWaves][32 * kWRow];
   __shared__ float sb1[kHid], sb2[kHid], ssc[kIn];
-  for (int e = threadIdx.x; e < kHid * kIn; e += blockDim.x) sW1[(e / kIn) * kW1Row + e % kIn] = W1[e];
-  for (int e = threadIdx.x; e < kHid * kHid; e += blockDim.x) sW2[(e / kHid) * kWRow + e % kHid] = W2[e];
-  for (int e = threadIdx.x; e < kOutPad * kHid; e += blockDim.x) sW3[(e / kHid) * kWRow + e % kHid] = W3[e];
+  // the weights once per workgroup, 8 BF16 (16 bytes) at a time into rows padded against bank conflicts
+  for (int e = threadIdx.x; e < kHid * kIn / 8; e += blockDim.x)
+    *reinterpret_cast<uint4*>(sW1 + (e / (kIn / 8)) * kW1Row + (e % (kIn / 8)) * 8) = reinterpret_cast<const uint4*>(W1)[e];
+  for (int e = threadIdx.x; e < kHid * kHid / 8; e += blockDim.x)
+    *reinterpret_cast<uint4*>(sW2 + (e / (kHid / 8)) * kWRow + (e % (kHid / 8)) * 8) = reinterpret_cast<const uint4*>(W2)[e];
+  for (int e = threadIdx.x; e < kOutPad * kHid / 8; e += blockDim.x)
+    *reinterpret_cast<uint4*>(sW3 + (e / (kHid / 8)) * kWRow + (e % (kHid / 8)) * 8) = reinterpret_cast<const uint4*>(W3)[e];
   for (int e = threadIdx.x; e < kHid; e += blockDim.x) sb1[e] = b1[e], sb2[e] = b2[e];
   for (int e = threadIdx.x; e < kIn; e += blockDim.x) ssc[e] = sc[e];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r0 = (blockIdx.x * kInferWaves + wave) * 32;
-  if (r0 >= n) return;
   const int lr = lane & 31, kh = (lane >> 5) * 8;
-  const int row = r0 + lr;
   __bf16* H = sH[wave];
-  f32x16 acc[4];
-  // layer 1: Z1 = X W1ᵀ with X = (D − D0)·s formed on the fly
-  for (int jt = 0; jt < 4; ++jt)
-    for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
-  for (int k0 = 0; k0 < kIn; k0 += 16) {
-    bf16x8 a;
-    for (int t = 0; t < 8; ++t) {
-      const int k = k0 + kh + t;
-      a[t] = row < n ? to_bf16((D[size_t(row) * kIn + k] - D0[size_t(row) * kIn + k]) * ssc[k]) : to_bf16(0.0f);
-    }
-    for (int jt = 0; jt < 4; ++jt) {
-      bf16x8 b;
-      const __bf16* w = sW1 + (jt * 32 + lr) * kW1Row + k0 + kh;
-      for (int t = 0; t < 8; ++t) b[t] = w[t];
-      acc[jt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[jt], 0, 0, 0);
-    }
-  }
-  for (int jt = 0; jt < 4; ++jt)
-    for (int q = 0; q < 4; ++q)
-      for (int t = 0; t < 4; ++t) {
-        const int rr = q * 8 + (lane >> 5) * 4 + t, col = jt * 32 + lr;
-        const float z = acc[jt][q * 4 + t] + sb1[col];
-        H[rr * kWRow + col] = to_bf16(z / (1.0f + __expf(-z)));
+  // 32-row blocks, one per wave, the workgroups striding over them (the grid holds about one workgroup per CU)
+  for (int r0 = (blockIdx.x * kInferWaves + wave) * 32; r0 < n; r0 += gridDim.x * kInferWaves * 32) {
+    const int row = r0 + lr;
+    f32x16 acc[4];
+    // layer 1: Z1 = X W1ᵀ with X = (D − D0)·s formed on the fly, 8 features per lane and step (two float4 loads each)
+    for (int jt = 0; jt < 4; ++jt)
+      for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+    for (int k0 = 0; k0 < kIn; k0 += 16) {
+      bf16x8 a;
+      if (row < n) {
+        const float4* dp = reinterpret_cast<const float4*>(D + size_t(row) * kIn + k0 + kh);
+        const float4* d0p = reinterpret_cast<const float4*>(D0 + size_t(row) * kIn + k0 + kh);
+        const float4 u0 = dp[0], u1 = dp[1], v0 = d0p[0], v1 = d0p[1];
+        const float dd[8] = {u0.x - v0.x, u0.y - v0.y, u0.z - v0.z, u0.w - v0.w, u1.x - v1.x, u1.y - v1.y, u1.z - v1.z, u1.w - v1.w};
+        for (int t = 0; t < 8; ++t) a[t] = to_bf16(dd[t] * ssc[k0 + kh + t]);
+      } else {
+        for (int t = 0; t < 8; ++t) a[t] = to_bf16(0.0f);
       }
-  wave_lds_sync();
-  // layer 2: Z2 = H1 W2ᵀ
-  for (int jt = 0; jt < 4; ++jt)
-    for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
-  for (int k0 = 0; k0 < kHid; k0 += 16) {
-    bf16x8 a;
-    for (int t = 0; t < 8; ++t) a[t] = H[lr * kWRow + k0 + kh + t];
-    for (int jt = 0; jt < 4; ++jt) {
-      bf16x8 b;
-      const __bf16* w = sW2 + (jt * 32 + lr) * kWRow + k0 + kh;
-      for (int t = 0; t < 8; ++t) b[t] = w[t];
-      acc[jt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[jt], 0, 0, 0);
+      for (int jt = 0; jt < 4; ++jt) {
+        bf16x8 b;
+        const __bf16* w = sW1 + (jt * 32 + lr) * kW1Row + k0 + kh;
+        for (int t = 0; t < 8; ++t) b[t] = w[t];
+        acc[jt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[jt], 0, 0, 0);
+      }
     }
-  }
-  wave_lds_sync();
-  for (int jt = 0; jt < 4; ++jt)
-    for (int q = 0; q < 4; ++q)
-      for (int t = 0; t < 4; ++t) {
-        const int rr = q * 8 + (lane >> 5) * 4 + t, col = jt * 32 + lr;
-        const float z = acc[jt][q * 4 + t] + sb2[col];
-        H[rr * kWRow + col] = to_bf16(z / (1.0f + __expf(-z)));
+    for (int jt = 0; jt < 4; ++jt)
+      for (int q = 0; q < 4; ++q)
+        for (int t = 0; t < 4; ++t) {
+          const int rr = q * 8 + (lane >> 5) * 4 + t, col = jt * 32 + lr;
+          const float z = acc[jt][q * 4 + t] + sb1[col];
+          H[rr * kWRow + col] = to_bf16(z / (1.0f + __expf(-z)));
+        }
+    wave_lds_sync();
+    // layer 2: Z2 = H1 W2ᵀ
+    for (int jt = 0; jt < 4; ++jt)
+      for (int r = 0; r < 16; ++r) acc[jt][r] = 0.0f;
+    for (int k0 = 0; k0 < kHid; k0 += 16) {
+      bf16x8 a;
+      for (int t = 0; t < 8; ++t) a[t] = H[lr * kWRow + k0 + kh + t];
+      for (int jt = 0; jt < 4; ++jt) {
+        bf16x8 b;
+        const __bf16* w = sW2 + (jt * 32 + lr) * kWRow + k0 + kh;
+        for (int t = 0; t < 8; ++t) b[t] = w[t];
+        acc[jt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[jt], 0, 0, 0);
       }
-  wave_lds_sync();
-  // layer 3: Y = H2 W3ᵀ (one 32x32 tile, 3 columns used)
-  f32x16 y;
-  for (int r = 0; r < 16; ++r) y[r] = 0.0f;
-  for (int k0 = 0; k0 < kHid; k0 += 16) {
-    bf16x8 a, b;
-    for (int t = 0; t < 8; ++t) a[t] = H[lr * kWRow + k0 + kh + t], b[t] = sW3[lr * kWRow + k0 + kh + t];
-    y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, y, 0, 0, 0);
+    }
+    wave_lds_sync();
+    for (int jt = 0; jt < 4; ++jt)
+      for (int q = 0; q < 4; ++q)
+        for (int t = 0; t < 4; ++t) {
+          const int rr = q * 8 + (lane >> 5) * 4 + t, col = jt * 32 + lr;
+          const float z = acc[jt][q * 4 + t] + sb2[col];
+          H[rr * kWRow + col] = to_bf16(z / (1.0f + __expf(-z)));
+        }
+    wave_lds_sync();
+    // layer 3: Y = H2 W3ᵀ (one 32x32 tile, 3 columns used)
+    f32x16 y;
+    for (int r = 0; r < 16; ++r) y[r] = 0.0f;
+    for (int k0 = 0; k0 < kHid; k0 += 16) {
+      bf16x8 a, b;
+      for (int t = 0; t < 8; ++t) a[t] = H[lr * kWRow + k0 + kh + t], b[t] = sW3[lr * kWRow + k0 + kh + t];
+      y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, y, 0, 0, 0);
+    }
+    if (lr < kOut)
+      for (int q = 0; q < 4; ++q)
+        for (int t = 0; t < 4; ++t) {
+          const int rr = r0 + q * 8 + (lane >> 5) * 4 + t;
+          if (rr < n) Y[size_t(rr) * kOut + lr] = y[q * 4 + t];
+        }
+    // H is rewritten by the next block's layer 1 only after every lane has read it here
+    wave_lds_sync();
   }
-  if (lr < kOut)
-    for (int q = 0; q < 4; ++q)
-      for (int t = 0; t < 4; ++t) {
-        const int rr = r0 + q * 8 + (lane >> 5) * 4 + t;
-        if (rr < n) Y[size_t(rr) * kOut + lr] = y[q * 4 + t];
-      }
 }
 
 }  // namespace
@@ -552,6 +566,7 @@ struct Learned {
   uint64_t rng = 0x9E3779B97F4A7C15ull;
   float lr = 1e-3f;
   int64_t adam_t = 0;
+  int cus = 0;   // compute units of the device (the inference grid)
   double t_slow[CAPSGPU_NTERMS] = {};
   // weights: FP32 masters, BF16 copies, Adam moments, gradients
   DevBuf<float> w1, b1, w2, b2, w3, b3, m1, m2, grad;
@@ -591,6 +606,17 @@ DescGeom desc_geom(const capsgpu_ctx& c) {
     if (G.nc[d] < 5) throw Error(CAPSGPU_EINVAL, "learned r-RESPA needs a cell at least 5 x 4 A wide along each axis");
   }
   return G;
+}
+
+// the inference grid: one workgroup per CU at most (each loads the weights once and strides over the row blocks)
+unsigned infer_grid(Learned& L, int rows) {
+  if (!L.cus) {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&L.cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (L.cus < 1) L.cus = 1;
+  }
+  return std::max(1u, std::min(blocks_for(size_t(rows), 32 * kInferWaves), unsigned(L.cus)));
 }
 
 void gemm(capsgpu_ctx& c, Learned& L, int M, int N, int K, const __bf16* A, int64_t sai, int64_t sak, const __bf16* B, int64_t sbk,
@@ -842,7 +868,7 @@ double rel_rmse(capsgpu_ctx& c, Learned& L, const double* pred, const double* tr
 void estimate(capsgpu_ctx& c, Learned& L) {
   L.Yi.resize(size_t(c.n) * kOut);
   kt_begin(c, "k_mlp_infer");
-  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(c.n, 32 * kInferWaves)), dim3(64 * kInferWaves), 0, c.stream, c.n, L.Dcur.p, L.D0.p, L.sc.p,
+  hipLaunchKernelGGL(k_mlp_infer, dim3(infer_grid(L, c.n)), dim3(64 * kInferWaves), 0, c.stream, c.n, L.Dcur.p, L.D0.p, L.sc.p,
                      L.w1b.p, L.b1.p, L.w2b.p, L.b2.p, L.w3b.p, L.Yi.p);
   LAUNCHCHECK();
   kt_end(c);
@@ -1019,7 +1045,7 @@ void learned_forward_host(capsgpu_ctx& c, const float* params, const float* x, i
   std::vector<float> one(kIn, 1.0f);
   ones.upload(one, s);
   L.Yi.resize(size_t(M) * kOut);
-  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(M, 32 * kInferWaves)), dim3(64 * kInferWaves), 0, s, M, xf.p, zero.p, ones.p, L.w1b.p,
+  hipLaunchKernelGGL(k_mlp_infer, dim3(infer_grid(L, M)), dim3(64 * kInferWaves), 0, s, M, xf.p, zero.p, ones.p, L.w1b.p,
                      L.b1.p, L.w2b.p, L.b2.p, L.w3b.p, L.Yi.p);
   LAUNCHCHECK();
   std::vector<float> yh = L.Yi.to_host(s);

@@ -38,13 +38,14 @@ def test_network_forward_matches_the_oracle():
     print("case mlp forward", flush=True)
     rng = np.random.default_rng(0)
     P = _params(rng)
-    x = rng.normal(0, 1, (256, 96))
+    m = 700   # three workgroups of 256 rows, the last one partial
+    x = rng.normal(0, 1, (m, 96))
     ev = _evaluator()
     y = ev.learned_forward(_flat(P), x)
     # the oracle at the BF16-rounded weights and inputs: what is left is the BF16 rounding of the activations
     Pb = [_bf16(p).astype(np.float64) if k % 2 == 0 else p for k, p in enumerate(P)]
-    yo = np.zeros((256, 3))
-    O.orc().orc_mlp_forward(*[np.ascontiguousarray(p, np.float64).reshape(-1) for p in Pb], _bf16(x).astype(np.float64).reshape(-1), 256,
+    yo = np.zeros((m, 3))
+    O.orc().orc_mlp_forward(*[np.ascontiguousarray(p, np.float64).reshape(-1) for p in Pb], _bf16(x).astype(np.float64).reshape(-1), m,
                             yo.reshape(-1))
     err = np.sqrt(((y - yo) ** 2).sum() / (yo ** 2).sum())
     print("forward rel RMS", err, flush=True)
