@@ -175,6 +175,13 @@ def main():
         out["roofline"] = {"kernel": name, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS, "achieved": ach,
                            "frac": ach / HBM_PEAK_GBS if ach else None, "traffic": None, "avg_launch_ms": tot / cnt,
                            "launches_timed": cnt, "algorithmic_bytes_per_launch": b}
+        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE measurement of the same kernel and workload, when there
+        # is one (profiles/<kernel>_traffic_262k_melt.json, written by caps_amd/traffic.py)
+        tpath = os.path.join(ROOT, "profiles", "%s_traffic_262k_melt.json" % name)
+        if os.path.exists(tpath):
+            tr = json.load(open(tpath))
+            out["roofline"]["traffic"] = tr["traffic_bytes"]
+            out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
     pme_ms = sum(v[0] / v[1] for k, v in kernels.items() if k.startswith(("k_splines", "k_tile", "k_spread", "k_fft", "k_gather")))
     out["pme"] = {"ms_per_eval": pme_ms, "algorithmic_bytes": pme_bytes(grid, n),
                   "achieved_GBs": pme_bytes(grid, n) / (pme_ms * 1e-3) / 1e9 if pme_ms else None}

@@ -109,4 +109,11 @@ constexpr double kAtm = 68568.415;        // kcal/(mol·Å³) → atm (core/src/
 
 inline unsigned blocks_for(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
 
+// the work item of workgroup b of G: workgroups go round-robin over the 8 XCDs, so XCD x is given the x-th contiguous
+// eighth of the items, and neighbouring items (which share data) meet in one XCD's L2 (a bijection on [0, G))
+__device__ inline int64_t xcd_block(int64_t b, int64_t G) {
+  const int64_t x = b % 8, i = b / 8, q = G / 8, r = G % 8;
+  return x * q + (x < r ? x : r) + i;
+}
+
 }  // namespace capsgpu

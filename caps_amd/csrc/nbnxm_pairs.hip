@@ -61,13 +61,6 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
-// the cluster of workgroup b: consecutive clusters on one XCD (workgroups go round-robin over the 8 XCDs), so
-// neighbouring i clusters, which share most j clusters, share that XCD's L2
-__device__ inline int64_t xcd_block(int64_t b, int64_t G) {
-  const int64_t x = b % 8, i = b / 8, q = G / 8, r = G % 8;
-  return x * q + (x < r ? x : r) + i;
-}
-
 template <int kLj, int kCoul, bool kE>
 __global__ void __launch_bounds__(256) k_pairs(PairArgs P) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;

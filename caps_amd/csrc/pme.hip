@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
   __shared__ int64_t s_hb[kMaxHome], s_pre[kMaxHome + 1];
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
-  const int tile = blockIdx.x;
+  const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
   if (tile >= ntiles) return;
   constexpr int p = P, pp = P * P, kYzBias = 4 * P + 4;   // −(r_b p + r_c) ≤ 3p + 3 < kYzBias
   float* s_yz = reinterpret_cast<float*>(lds);          // [chunk][p²]: q θb[dy] θc[dz]
@@ -583,7 +583,7 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
   extern __shared__ float2 lds[];
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
-  const int tile = blockIdx.x;
+  const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
   if (tile >= ntiles) return;
   const int64_t a0 = tstart[tile], na = tstart[tile + 1] - a0;
   if (na == 0) return;
