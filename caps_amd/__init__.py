@@ -70,6 +70,16 @@ class DReport(C.Structure):
                 ("prunes", C.c_int32), ("learned_steps", C.c_int32), ("full_steps", C.c_int32), ("fallbacks", C.c_int32)]
 
 
+class BOpts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("steps", C.c_int64), ("tau_t", C.c_double), ("seed", C.c_uint64), ("new_velocities", C.c_int32),
+                ("mode", C.c_int32), ("branches", C.c_int32), ("graph_steps", C.c_int32)]
+
+
+class BReport(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("seconds", C.c_double), ("atom_steps_per_s", C.c_double), ("ns_per_day", C.c_double),
+                ("mode", C.c_int32), ("branches", C.c_int32), ("graph_steps", C.c_int32), ("tune_seconds", C.c_double)]
+
+
 class Sys(C.Structure):
     _fields_ = [("n_atoms", C.c_int32), ("n_types", C.c_int32), ("cell", C.c_double * 12), ("x", C.POINTER(C.c_double)),
                 ("type", C.POINTER(C.c_int32)), ("charge", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
@@ -119,6 +129,9 @@ def lib():
         L.capsgpu_learned_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                                C.POINTER(C.c_double)]
         L.capsgpu_learned_descriptors.argtypes = [C.c_void_p, C.c_void_p]
+        L.capsgpu_batch_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        L.capsgpu_batch_run.argtypes = [C.c_void_p, C.POINTER(BOpts), C.c_void_p, C.c_void_p, C.POINTER(BReport)]
+        L.capsgpu_batch_destroy.argtypes = [C.c_void_p]
         L.capsgpu_build_melt.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_build_kg.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_system_free.argtypes = [C.POINTER(Sys)]
@@ -448,6 +461,85 @@ def run_dynamics(system, o, evaluator=None):
     system.x, system.v = x, v
     return DynamicsReport([_row(rows[k]) for k in range(min(nr.value, cap))], rep.steps, rep.seconds, rep.ns_per_day,
                           rep.list_builds, rep.prunes, rep.learned_steps, rep.full_steps, rep.fallbacks)
+
+
+@dataclass
+class BatchOptions:
+    """Langevin BAOAB steps of many replicas together (no reference counterpart: the reference runs a cooling scan's
+    replicas one after another through run_dynamics, core/src/mechanics.cpp:1381-1470). mode: "tuned" (the launch shape
+    timed on the batch), "graph" (branches streams, graph_steps steps per captured graph) or "direct" (no graph)."""
+    dt: float = 1.0
+    steps: int = 1000
+    tau_t: float = 100.0
+    seed: int = 1                     # replica r draws from seed + r
+    new_velocities: bool = True
+    mode: str = "tuned"
+    branches: int = 1
+    graph_steps: int = 1
+
+
+@dataclass
+class BatchReport:
+    steps: int
+    seconds: float
+    atom_steps_per_s: float
+    ns_per_day: float
+    mode: str
+    branches: int
+    graph_steps: int
+    tune_seconds: float
+    temperatures: np.ndarray          # the kinetic temperature of each replica at the end
+
+
+_BATCH_MODES = {"tuned": 0, "graph": 1, "direct": 2}
+
+
+class Batch:
+    """Replicas (one Evaluator each: its own force field, cell, PME grid) stepped together. Each replica ends as
+    run_dynamics(system, DynamicsOptions(thermostat="langevin", seed=seed + r, ...)) of that replica alone would leave it,
+    bit for bit. The evaluators' options must leave the pair path out (parts 1, 4 or 5)."""
+
+    def __init__(self, evaluators):
+        self.evaluators = list(evaluators)
+        n = len(self.evaluators)
+        hs = (C.c_void_p * n)(*[e._h.value for e in self.evaluators])
+        self._h = C.c_void_p()
+        _check(lib().capsgpu_batch_create(C.cast(hs, C.c_void_p), n, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().capsgpu_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, systems, temperatures, o=None):
+        """steps every system from its state at its temperature (K); updates system.x and system.v in place"""
+        o = o or BatchOptions()
+        if o.mode not in _BATCH_MODES:
+            raise ValueError("unknown batch mode '%s' (tuned, graph, direct)" % o.mode)
+        if len(systems) != len(self.evaluators) or len(temperatures) != len(self.evaluators):
+            raise ValueError("one system and one temperature per replica")
+        if not o.new_velocities and any(s.v is None for s in systems):
+            raise ValueError("every replica needs velocities unless new_velocities is set")
+        for ev, s in zip(self.evaluators, systems):
+            ev.set_state(s.x, None if o.new_velocities else s.v, s.cell)
+        b = BOpts()
+        b.dt, b.steps, b.tau_t, b.seed, b.new_velocities = o.dt, o.steps, o.tau_t, o.seed, int(o.new_velocities)
+        b.mode, b.branches, b.graph_steps = _BATCH_MODES[o.mode], o.branches, o.graph_steps
+        T = np.ascontiguousarray(temperatures, dtype=np.float64)
+        Tout = np.zeros(len(self.evaluators))
+        r = BReport()
+        _check(lib().capsgpu_batch_run(self._h, C.byref(b), T.ctypes.data_as(C.c_void_p), Tout.ctypes.data_as(C.c_void_p), C.byref(r)))
+        for ev, s in zip(self.evaluators, systems):
+            x, v, _ = ev.get_state()
+            s.x, s.v = x, v
+        mode = {1: "graph", 2: "direct"}.get(r.mode, "tuned")
+        return BatchReport(r.steps, r.seconds, r.atom_steps_per_s, r.ns_per_day, mode, r.branches, r.graph_steps, r.tune_seconds, Tout)
 
 
 def _row(r):

@@ -9,6 +9,7 @@
 #include <random>
 #include <string>
 
+#include "batch.hpp"
 #include "context.hpp"
 #include "host_rng.hpp"
 #include "mlp.hpp"
@@ -623,6 +624,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     const unsigned nb = blocks_for(n, 256);
     kpart.resize(7 * size_t(nb));
     bpart.resize(size_t(nb));
+    bpart.zero(s);
     scal.resize(2);
     scal.zero(s);
     std::vector<double> one = {1.0, 0.0};
@@ -679,8 +681,16 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
         evaluate(*c, want, tt, flags_known);
       }
     };
+    // without a pair list the flags record only non-finite positions and FENE bonds past R0: they gather over the run and
+    // are read at thermo steps and at the end, so the steps in between need no host synchronisation
+    auto check_flags = [&](int64_t step) {
+      read_flags(*c);
+      if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable by step " + std::to_string(step));
+      if (c->h_flags[2]) throw Error(CAPSGPU_EFIELD, "a FENE bond is stretched to R0 or past; lower the time step or relax the structure first");
+    };
     double t[CAPSGPU_NTERMS];
     c->h_flags[0] = c->h_flags[1] = 0;
+    if (!listed) c->flags.zero(s);
     full_eval(true, t, false);
     // step 0: the kinetic energy of the starting velocities
     kick(*c, 0.0, true, kpart.p);
@@ -690,7 +700,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     bool alpha_pending = false;
     for (int64_t step = 1; step <= o->steps; ++step) {
       const bool thermo = step % every == 0 || step == o->steps;
-      c->flags.zero(s);
+      if (listed) c->flags.zero(s);
       if (R > 1) {
         const double h = dt / R, hkh = 0.5 * h * kAcc;
         kick(*c, hk, false, kpart.p, fslow.p);
@@ -699,8 +709,10 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
           if (j + 1 < R) {
             evaluate(*c, false, t, true, 1, ffast.p);
           } else {
-            read_flags(*c);
-            if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable at step " + std::to_string(step));
+            if (listed) {
+              read_flags(*c);
+              if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable at step " + std::to_string(step));
+            }
             full_eval(thermo, t, true);
           }
           kick(*c, hkh, false, kpart.p, ffast.p);
@@ -713,18 +725,22 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
           scale_v(*c, scal.p);
           if (thermo) kick(*c, 0.0, true, kpart.p);
         }
-        if (thermo) emit(step, t);
+        if (thermo) {
+          if (!listed) check_flags(step);
+          emit(step, t);
+        }
         continue;
       }
       if (o->thermostat == 2) {
-        baoab_open(*c, hk, 0.5 * dt, c1, c2, sdv.p, o->seed, uint64_t(step), lim_b, lim_p, bpart.p);
-        bath_add(*c, bpart.p, scal.p + 1);
+        baoab_open(*c, hk, 0.5 * dt, c1, c2, sdv.p, o->seed, nullptr, uint64_t(step), lim_b, lim_p, bpart.p);
       } else {
         vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p);
         alpha_pending = false;
       }
-      read_flags(*c);
-      if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable at step " + std::to_string(step));
+      if (listed) {
+        read_flags(*c);
+        if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable at step " + std::to_string(step));
+      }
       if (LR) {
         const int r = learned_step(*c, *LR, step, thermo, t);
         last_learned = r == 1;
@@ -745,6 +761,8 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       }
       if (thermo) {
         if (o->thermostat == 1) kick(*c, 0.0, true, kpart.p);
+        if (o->thermostat == 2) bath_add(*c, bpart.p, scal.p + 1);
+        if (!listed) check_flags(step);
         emit(step, t);
       }
     }
@@ -766,6 +784,26 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       rep->fallbacks = int32_t(fallbacks);
     }
   });
+}
+
+int32_t capsgpu_batch_create(capsgpu_ctx* const* replicas, int32_t n, capsgpu_batch** out) {
+  return guard([&] {
+    if (!out) throw Error(CAPSGPU_EINVAL, "null argument");
+    *out = nullptr;
+    *out = batch_create(replicas, n);
+  });
+}
+
+int32_t capsgpu_batch_run(capsgpu_batch* b, const capsgpu_batch_opts* o, const double* temperatures, double* temperature_out,
+                          capsgpu_batch_report* rep) {
+  return guard([&] {
+    if (!b || !o) throw Error(CAPSGPU_EINVAL, "null argument");
+    batch_run(*b, *o, temperatures, temperature_out, rep);
+  });
+}
+
+int32_t capsgpu_batch_destroy(capsgpu_batch* b) {
+  return guard([&] { batch_destroy(b); });
 }
 
 }  // extern "C"

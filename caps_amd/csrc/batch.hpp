@@ -1,0 +1,10 @@
+// Internal: replicas stepped together (batch.hip); the C entry points in api.hip wrap these.
+#pragma once
+
+#include "../../include/capsgpu.h"
+
+namespace capsgpu {
+capsgpu_batch* batch_create(capsgpu_ctx* const* replicas, int n);
+void batch_destroy(capsgpu_batch* b);
+void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temperatures, double* temperature_out, capsgpu_batch_report* rep);
+}  // namespace capsgpu

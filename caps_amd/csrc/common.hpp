@@ -32,6 +32,16 @@ void guard_check(const char* file, int line);
 #define LAUNCHCHECK() ::capsgpu::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
 #endif
 
+namespace {
+// zeros nbytes from p: 16-byte words, then the tail bytes (device allocations are 256-byte aligned). A kernel rather
+// than hipMemsetAsync, so work captured into a HIP graph holds no memset node.
+__global__ void k_zero_bytes(unsigned char* __restrict__ p, size_t nbytes) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x, w = nbytes / 16;
+  if (i < w) reinterpret_cast<uint4*>(p)[i] = make_uint4(0u, 0u, 0u, 0u);
+  else if (i < w + (nbytes & 15)) p[16 * w + (i - w)] = 0;
+}
+}  // namespace
+
 // a device array that owns its memory and grows (never shrinks) on demand
 template <class T>
 struct DevBuf {
@@ -96,7 +106,10 @@ struct DevBuf {
     return h;
   }
   void zero(hipStream_t s) {
-    if (n) HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    if (!n) return;
+    const size_t b = n * sizeof(T), items = b / 16 + (b & 15);
+    hipLaunchKernelGGL(k_zero_bytes, dim3(unsigned((items + 255) / 256)), dim3(256), 0, s, reinterpret_cast<unsigned char*>(p), b);
+    HIPCHECK(hipGetLastError());
   }
 };
 

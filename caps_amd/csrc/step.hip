@@ -283,7 +283,8 @@ __device__ inline void normals3(uint64_t seed, uint64_t step, uint32_t atom, dou
   g[2] = r1 * cos(2 * kPi * u[3]);
 }
 
-template <int kParts>
+// the workgroup's sums of v into its row of part (kAdd: added to what the row holds)
+template <int kParts, bool kAdd = false>
 __device__ inline void block_sum_store(double (&v)[kParts], double* part) {
   __shared__ double red[kParts][256];
   for (int q = 0; q < kParts; ++q) red[q][threadIdx.x] = v[q];
@@ -293,7 +294,10 @@ __device__ inline void block_sum_store(double (&v)[kParts], double* part) {
       for (int q = 0; q < kParts; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x < kParts) part[kParts * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < kParts) {
+    double& d = part[kParts * size_t(blockIdx.x) + threadIdx.x];
+    d = kAdd ? d + red[threadIdx.x][0] : red[threadIdx.x][0];
+  }
 }
 
 // v ← α v (Bussi, the previous step) then v += h·kAcc/m F; x += dt v
@@ -315,11 +319,13 @@ __global__ void __launch_bounds__(256) k_vv_open(int n, double hk, double dt, co
 
 // B (half kick), A (half drift), O, A; partials: kinetic energy before and after O (the bath's share)
 __global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
-                                                    const double* __restrict__ sdv, uint64_t seed, uint64_t step, const double* __restrict__ f,
+                                                    const double* __restrict__ sdv, uint64_t seed, const uint64_t* __restrict__ stepc,
+                                                    uint64_t step_add, const double* __restrict__ f,
                                                     double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
                                                     const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
                                                     double* __restrict__ part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t step = (stepc ? *stepc : 0) + step_add;
   double ke[1] = {0};
   if (i < n) {
     const double a = hk * minv[i], mi = 1.0 / minv[i];
@@ -339,7 +345,7 @@ __global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt
     ke[0] = 0.5 * mi * (k1 - k0) / kAcc;
   }
   move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
-  block_sum_store<1>(ke, part);
+  block_sum_store<1, true>(ke, part);
 }
 
 // the closing half kick; kinetic energy partials (Σ ½ m v², and the tensor m v_a v_b) when asked
@@ -392,10 +398,11 @@ __global__ void k_scale_v(int n, const double* __restrict__ alpha, double* __res
   if (i < 3 * n) v[i] *= *alpha;
 }
 
-__global__ void k_add_scalar(const double* __restrict__ part, int64_t rows, double* __restrict__ acc) {
+// acc −= Σ part, and the rows cleared for the next accumulation (each thread clears the rows it read)
+__global__ void k_add_scalar(double* __restrict__ part, int64_t rows, double* __restrict__ acc) {
   __shared__ double red[256];
   double s = 0;
-  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += part[r];
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += part[r], part[r] = 0.0;
   red[threadIdx.x] = s;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
@@ -403,6 +410,11 @@ __global__ void k_add_scalar(const double* __restrict__ part, int64_t rows, doub
     __syncthreads();
   }
   if (threadIdx.x == 0) *acc -= red[0];
+}
+
+// the step counter of captured dynamics: the steps a replay took
+__global__ void k_step_advance(uint64_t* __restrict__ stepc, uint64_t by) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *stepc += by;
 }
 
 CellArgs cell_args(const capsgpu_ctx& c) {
@@ -527,11 +539,11 @@ void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double l
   kt_end(c);
 }
 
-void baoab_open(capsgpu_ctx& c, double hk, double hdt, double c1, double c2, const double* sdv, uint64_t seed, uint64_t step, double lim_b,
-                double lim_p, double* part) {
+void baoab_open(capsgpu_ctx& c, double hk, double hdt, double c1, double c2, const double* sdv, uint64_t seed, const uint64_t* stepc,
+                uint64_t step_add, double lim_b, double lim_p, double* part) {
   kt_begin(c, "k_baoab_open");
-  hipLaunchKernelGGL(k_baoab_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, hdt, c1, c2, c.d_minv.p, sdv, seed, step,
-                     c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part);
+  hipLaunchKernelGGL(k_baoab_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, hdt, c1, c2, c.d_minv.p, sdv, seed, stepc,
+                     step_add, c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part);
   LAUNCHCHECK();
   kt_end(c);
 }
@@ -562,7 +574,14 @@ void scale_v(capsgpu_ctx& c, const double* alpha) {
   kt_end(c);
 }
 
-void bath_add(capsgpu_ctx& c, const double* part, double* bath) {
+void step_advance(capsgpu_ctx& c, uint64_t* stepc, uint64_t by) {
+  kt_begin(c, "k_step_advance");
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c.stream, stepc, by);
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+void bath_add(capsgpu_ctx& c, double* part, double* bath) {
   kt_begin(c, "k_add_scalar");
   hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(256), 0, c.stream, part, int64_t(blocks_for(c.n, 256)), bath);
   LAUNCHCHECK();

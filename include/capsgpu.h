@@ -153,6 +153,45 @@ typedef struct capsgpu_dyn_report {
 int32_t capsgpu_run_dynamics(capsgpu_ctx* ctx, const capsgpu_dyn_opts* opts, capsgpu_progress_fn progress, void* user,
                              capsgpu_thermo* rows, int32_t cap, int32_t* nrows, capsgpu_dyn_report* report);
 
+/* ---- replicas stepped together ------------------------------------------------------------------------------------
+ * The reference runs the replicas of a cooling scan one after another, each through run_dynamics
+ * (core/src/mechanics.cpp:1381-1470 via caps::run_cooling, core/include/caps/mechanics.hpp:323-353); here many contexts
+ * (each its own force field, cell, PME grid and temperature) take Langevin BAOAB steps together. A chunk of steps of
+ * every replica is captured once into HIP graphs and replayed; the launch shape (how many streams the replicas are
+ * spread over, each replaying its own graph, and how many steps one graph holds) is picked by timing the candidates on
+ * the batch itself, from a snapshot of the state that is restored afterwards. Each replica ends bit-identical to
+ * capsgpu_run_dynamics of that context alone with thermostat 2, seed + r and the same dt, tau_t and temperature.
+ * The replicas' energy options must leave the pair path out (parts 1, 4 or 5): its list rebuilds need the host. */
+typedef struct capsgpu_batch capsgpu_batch;
+
+typedef struct capsgpu_batch_opts {
+  double dt;                    /* fs */
+  int64_t steps;
+  double tau_t;                 /* fs: Langevin 1/γ */
+  uint64_t seed;                /* replica r draws its noise (and new velocities) from seed + r */
+  int32_t new_velocities;       /* 1: Maxwell–Boltzmann at each replica's temperature, else the set state */
+  int32_t mode;                 /* 0: the tuned launch shape; 1: graphs with the shape below; 2: direct launches */
+  int32_t branches;             /* mode 1: streams the replicas are spread over (longest first), each with its graph */
+  int32_t graph_steps;          /* mode 1: steps one graph holds */
+} capsgpu_batch_opts;
+
+typedef struct capsgpu_batch_report {
+  int64_t steps;
+  double seconds;               /* the steps alone: set-up, first forces and tuning left out */
+  double atom_steps_per_s;      /* Σ atoms × steps / seconds */
+  double ns_per_day;            /* simulated time of each replica per day */
+  int32_t mode, branches, graph_steps;   /* the launch shape used */
+  double tune_seconds;          /* the timing of the candidates (0 when the shape was given or already tuned) */
+} capsgpu_batch_report;
+
+/* a batch over n contexts (held, not owned: they must outlive the batch; one context at most once) */
+int32_t capsgpu_batch_create(capsgpu_ctx* const* replicas, int32_t n, capsgpu_batch** out);
+/* steps every replica from its set state at temperatures[r] (K); temperature_out[r] (NULL: not written): the kinetic
+ * temperature at the end. The states stay on the device (capsgpu_get_state per context). */
+int32_t capsgpu_batch_run(capsgpu_batch* batch, const capsgpu_batch_opts* opts, const double* temperatures, double* temperature_out,
+                          capsgpu_batch_report* report);
+int32_t capsgpu_batch_destroy(capsgpu_batch* batch);
+
 /* ---- inspection, for the parity checks (no counterpart in the reference's interfaces) ----------------------------- */
 /* sizes of the current cluster list: atoms, clusters, slots, outer entries, image codes, columns nu, nv, pruned full
  * entries */

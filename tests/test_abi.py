@@ -70,3 +70,20 @@ def test_melt_builder_shapes():
         assert (np.diff(part) > 0).all() and i not in part
     # every carbon has four neighbours, so C(4, 2) = 6 angles with it as the vertex
     assert len(ff.angle_atoms) == 4 * 42 * 6 and len(ff.bond_atoms) == 4 * 127
+
+
+def test_batch_arguments_are_rejected_without_a_device():
+    L = ca.lib()
+    out = C.c_void_p()
+    assert L.capsgpu_batch_create(None, 3, C.byref(out)) == -1
+    assert b"at least one replica" in L.capsgpu_last_error()
+    assert L.capsgpu_batch_create(None, 0, None) == -1
+    assert L.capsgpu_batch_run(None, None, None, None, None) == -1
+    assert b"null" in L.capsgpu_last_error()
+    assert L.capsgpu_batch_destroy(None) == 0
+    # a context named twice is refused before anything runs
+    fake = (C.c_void_p * 2)(C.c_void_p(0x1000), C.c_void_p(0x1000))
+    assert L.capsgpu_batch_create(C.cast(fake, C.c_void_p), 2, C.byref(out)) == -1
+    assert b"twice" in L.capsgpu_last_error()
+    with pytest.raises(ValueError, match="mode"):
+        ca.Batch.run(object.__new__(ca.Batch), [], [], ca.BatchOptions(mode="fast"))
