@@ -87,6 +87,52 @@ def cpu_baseline(system, budget_s=20.0):
                       "list built once outside the timing; bonded terms and the integrator left out" % (len(times), per)}
 
 
+def replica_batch(ca, torch, steps, count=64, seed=SEED):
+    """Many melts of the all-atom chemistry stepped together: `count` replicas with sizes drawn uniformly from 3,000 to
+    40,000 atoms (whole 128-atom chains), each its own cubic box and PME grid (order 4, 1.0 A), Langevin on a 500 -> 200 K
+    ladder; PME reciprocal part + bonded terms (the pair path is held back). Timed three ways over the same steps: one
+    replica after another through run_dynamics (as a cooling scan runs them), the batch with direct launches, and the
+    batch with its tuned launch shape (graphs)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(3000, 40001, count)
+    chains = [max(1, int(round(int(m) / 128))) for m in sizes]
+    temps = np.linspace(500.0, 200.0, count)
+    opts = ca.EnergyOptions(cutoff=RC, skin=SKIN, coulomb=True, electrostatics="pme", ewald_rtol=RTOL, pme_order=4, pme_spacing=SPACING,
+                            tail=False, parts=5)
+    print("bench: building %d replicas" % count, file=sys.stderr, flush=True)
+    systems = [ca.build_melt(c, seed=seed + 100 + k) for k, c in enumerate(chains)]
+    atoms = sum(s.ff.n for s in systems)
+    evs = [ca.Evaluator(s.ff, opts) for s in systems]
+    out = {"workload": "%d all-atom melts of 3,000-40,000 atoms (seeded), PME order 4 at 1.0 A, Langevin 500 -> 200 K ladder, dt 1 fs; "
+                       "PME reciprocal part + bonded terms" % count, "replicas": count, "atoms": atoms, "steps": steps,
+           "box_A": [min(s.cell[3] for s in systems), max(s.cell[3] for s in systems)]}
+    # one replica after another (warmed first)
+    def sequential(nsteps, new_v):
+        total = 0.0
+        for k, (s, ev) in enumerate(zip(systems, evs)):
+            d = ca.DynamicsOptions(dt=DT, steps=nsteps, temperature=float(temps[k]), thermostat="langevin", tau_t=TAU, seed=seed + k,
+                                   thermo_every=10 ** 9, new_velocities=new_v, energy=opts)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ca.run_dynamics(s, d, evaluator=ev)
+            torch.cuda.synchronize()
+            total += time.perf_counter() - t0
+        return total
+    sequential(5, True)
+    secs = sequential(steps, False)
+    out["one_after_another"] = {"seconds": secs, "atom_steps_per_s": atoms * steps / secs, "ns_per_day": steps * DT * 1e-6 * 86400 / secs}
+    b = ca.Batch(evs)
+    for mode in ("direct", "tuned"):
+        o = ca.BatchOptions(dt=DT, steps=steps, tau_t=TAU, seed=seed, new_velocities=False, mode=mode)
+        r = b.run(systems, temps, o)
+        out[mode] = {"seconds": r.seconds, "atom_steps_per_s": r.atom_steps_per_s, "ns_per_day": r.ns_per_day, "mode": r.mode,
+                     "branches": r.branches, "graph_steps": r.graph_steps, "tune_seconds": r.tune_seconds,
+                     "speedup_vs_one_after_another": secs / r.seconds if r.seconds else None}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +140,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline")
     ap.add_argument("--learned-train", type=int, default=2000, help="PME frames the learned r-RESPA network trains on (-1: skip)")
+    ap.add_argument("--replicas", type=int, default=64, help="melts in the replica-batch leg (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d asked)" % a.gpus, file=sys.stderr)
@@ -209,6 +256,8 @@ def main():
             "force_rmse_stale": float(np.mean([r["learned_rmse_stale"] for r in judged])) if judged else None,
             "rmse_definition": "|F_slow predicted - F_slow PME| / |F total| over all atoms at each full step",
             "speedup_vs_plain": (secs / lsecs)}
+    if a.replicas > 0:
+        out["replica_batch"] = replica_batch(ca, torch, a.steps, a.replicas)
     if not a.no_cpu:
         print("bench: the reference CPU path", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(system)

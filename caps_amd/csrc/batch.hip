@@ -200,15 +200,17 @@ double seconds_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// the fastest shape by timing: direct launches and graphs of B ∈ {1, 2, 4, 8} streams × S ∈ {1, 4, 16} steps, each run
-// for about 16 steps after one untimed chunk (which also captures), from the snapshot
+// the fastest shape by timing: direct launches and graphs of B ∈ {1, 2, 4, …, 32} streams × S ∈ {4, 16, 64} steps (at
+// most 1,024 replica steps in one graph), each run for at least 16 steps after one untimed chunk (which also captures),
+// from the snapshot
 Shape tune(capsgpu_batch& b) {
   std::vector<Shape> cand;
   cand.push_back({2, 0, 1});
   const int n = int(b.r.size());
-  for (int B : {1, 2, 4, 8})
+  for (int B : {1, 2, 4, 8, 16, 32})
     if (B <= n)
-      for (int S : {1, 4, 16}) cand.push_back({1, B, S});
+      for (int S : {4, 16, 64})
+        if (int64_t(S) * ((n + B - 1) / B) <= 1024) cand.push_back({1, B, S});
   snapshot(b, false);
   Shape best;
   double best_t = 1e300;

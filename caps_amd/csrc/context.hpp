@@ -94,6 +94,8 @@ struct Pme {
   DevBuf<int32_t> base;                 // per atom: the first grid index per axis and the home tile (int4)
   DevBuf<int32_t> home;                 // per axis and tile: the home tiles that reach it (count + 4)
   DevBuf<uint32_t> tile, tcount, tfill, tatoms;   // tile: the atoms filled unordered per tile; tatoms: in index order
+  DevBuf<float> wsort;                  // theta in tatoms's order (float4 per atom)
+  DevBuf<int32_t> bsort;                // base in tatoms's order (int4 per atom)
   DevBuf<int64_t> tstart;
   DevBuf<float> fpme;                   // 3 per atom
   DevBuf<float2> tw;                    // twiddles of the three axes

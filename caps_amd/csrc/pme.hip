@@ -90,75 +90,79 @@ __global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t*
   tmp[tstart[t] + atomicAdd(&tfill[t], 1u)] = uint32_t(i);
 }
 
-// the atoms of each tile in index order: an atom's place is its rank among its tile's atoms (filled unordered in tmp)
-__global__ void k_tile_rank(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tmp,
-                            uint32_t* __restrict__ tatoms) {
+// the atoms of each tile in index order: an atom's place is its rank among its tile's atoms (filled unordered in tmp);
+// its spline offsets, charge and base point are copied to that place too, so the spread and the gather read a tile's
+// atoms contiguously
+__global__ void k_tile_rank(int n, const float4* __restrict__ wq, const int4* __restrict__ base, const int64_t* __restrict__ tstart,
+                            const uint32_t* __restrict__ tmp, uint32_t* __restrict__ tatoms, float4* __restrict__ wsort,
+                            int4* __restrict__ bsort) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int t = base[i].w;
-  const int64_t b = tstart[t], e = tstart[t + 1];
+  const int4 bi = base[i];
+  const float4 wi = wq[i];
+  const int64_t b = tstart[bi.w], e = tstart[bi.w + 1];
   int64_t r = 0;
   for (int64_t k = b; k < e; ++k) r += tmp[k] < uint32_t(i) ? 1 : 0;
   tatoms[b + r] = uint32_t(i);
+  wsort[b + r] = wi;
+  bsort[b + r] = bi;
 }
 
 // one 64-lane workgroup per 4x4x4 tile. The atoms whose first point lies in a tile that can reach this one (the home
 // table lists those tiles per axis) are staged in LDS 64 at a time. With r_d = 4 t_d − b_d the offset of an atom's
 // first point from the tile's corner (folded into [−3, p]), a point (i, j, k) of the tile sees θa[i + r_a],
-// θb[j + r_b], θc[k + r_c]; so per atom: q·θb[dy]·θc[dz] for every (dy, dz), the four θa[i + r_a] of the tile's points
-// along a, and a 16-bit mask of the tile's rows (j, k) it reaches. Lane = (row (j, k), quarter of the atoms): it sums
-// its quarter's terms for the row's 4 points; the quarters are then added in a fixed order (every point written once,
-// no atomics). Dynamic LDS: kSpreadChunk · p² floats.
+// θb[j + r_b], θc[k + r_c]; so per atom: θb and q·θc, the four θa[i + r_a] of the tile's points along a, and a 16-bit
+// mask of the tile's rows (j, k) it reaches. Lane = (row (j, k), quarter of the atoms): it sums its quarter's terms
+// for the row's 4 points; the quarters are then added in a fixed order (every point written once, no atomics).
+// Dynamic LDS: kSpreadChunk · 2p floats (small enough for eight workgroups per SIMD).
 constexpr int kSpreadChunk = 64, kMaxHome = 64;   // at most 4 home tiles per axis reach a tile (p <= 8, partial last tile)
 template <int P>
 __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
-                                               const uint32_t* __restrict__ tatoms, const float4* __restrict__ wq,
-                                               const int4* __restrict__ base, float* __restrict__ grid) {
+                                               const float4* __restrict__ wsort, const int4* __restrict__ bsort, float* __restrict__ grid) {
   extern __shared__ float2 lds[];
   __shared__ float4 s_wx[kSpreadChunk];   // θa[r_a + i] for the tile's four points i along a (zero outside the support)
-  __shared__ int2 s_ent[kSpreadChunk];    // {j p² + r_b p + r_c + kYzBias, the rows (j, k) of the tile the atom reaches}
-  __shared__ float s_part[4][64];
-  __shared__ int64_t s_hb[kMaxHome], s_pre[kMaxHome + 1];
+  __shared__ int4 s_ent[kSpreadChunk];    // {2p j + r_b + 3, 2p j + p + r_c + 3, the rows (j, k) of the tile it reaches, 0}
+  __shared__ int s_hb[kMaxHome], s_pre[kMaxHome + 1];
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
   const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
   if (tile >= ntiles) return;
-  constexpr int p = P, pp = P * P, kYzBias = 4 * P + 4;   // −(r_b p + r_c) ≤ 3p + 3 < kYzBias
-  float* s_yz = reinterpret_cast<float*>(lds);          // [chunk][p²]: q θb[dy] θc[dz]
+  constexpr int p = P;
+  float* s_bc = reinterpret_cast<float*>(lds);          // [chunk][2p]: θb, then q θc
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
   // the candidate tiles in a fixed order (c, b, a) from the home table, their atom counts in parallel, a running sum
   const int* hx = home + (0 * G.ntm + tx) * 5;
   const int* hy = home + (1 * G.ntm + ty) * 5;
   const int* hz = home + (2 * G.ntm + tz) * 5;
   const int mx = hx[0], my = hy[0], mz = hz[0], nh = mx * my * mz;
-  int64_t cnt = 0;
+  int cnt = 0;   // atom counts fit 32 bits (the library's atom count is an int)
   if (lane < nh) {
     const int ix = lane % mx, iy = (lane / mx) % my, iz = lane / (mx * my);
     const int h = (hz[1 + iz] * G.nt[1] + hy[1 + iy]) * G.nt[0] + hx[1 + ix];
-    s_hb[lane] = tstart[h];
-    cnt = tstart[h + 1] - tstart[h];
+    s_hb[lane] = int(tstart[h]);
+    cnt = int(tstart[h + 1] - tstart[h]);
   }
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    const int64_t o = __shfl_up(cnt, d, 64);
+    const int o = __shfl_up(cnt, d, 64);
     if (lane >= d) cnt += o;
   }
   if (lane < nh) s_pre[lane + 1] = cnt;
   if (lane == 0) s_pre[0] = 0;
   __syncthreads();
-  const int64_t m = s_pre[nh];
+  const int m = s_pre[nh];
   // lane = (row (j, k) of the tile, quarter of the atoms); its row's bit in the reach mask is j + 4 k = lane & 15
   const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4, rowbit = 1 << (lane & 15);
-  const int lane_yz = gy * P + gz - kYzBias;
+  const int lane_b = gy - 3, lane_c = gz - 3;
   float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
-  for (int64_t c0 = 0; c0 < m; c0 += kSpreadChunk) {
-    const int64_t f = c0 + lane;
+  for (int c0 = 0; c0 < m; c0 += kSpreadChunk) {
+    const int f = c0 + lane;
     if (f < m) {
       int t = 0;
       while (s_pre[t + 1] <= f) ++t;
-      const uint32_t a = tatoms[s_hb[t] + (f - s_pre[t])];
-      const int4 b = base[a];
-      const float4 wv = wq[a];
+      const int at = s_hb[t] + (f - s_pre[t]);
+      const int4 b = bsort[at];
+      const float4 wv = wsort[at];
       // r = 4t − b lies in (−K, K + 3): to [0, K) and then into [−3, K − 4], and at most p (beyond, no point is reached)
       const int bb[3] = {b.x, b.y, b.z}, tt[3] = {tx, ty, tz};
       int r[3];
@@ -190,23 +194,21 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
       int reach = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) reach |= ((mc >> k) & 1) ? mb << (4 * k) : 0;
-      s_ent[lane] = make_int2(int(f - c0) * pp + r[1] * P + r[2] + kYzBias, reach);
-      float* syz = s_yz + (f - c0) * pp;
+      // a reached row has 0 ≤ j + r_b < p: the offsets below stay inside the atom's own θb and q θc
+      const int sl = (f - c0) * 2 * P;
+      s_ent[lane] = make_int4(sl + r[1] + 3, sl + P + r[2] + 3, reach, 0);
+      float* sbc = s_bc + sl;
 #pragma unroll
-      for (int dy = 0; dy < P; ++dy) {
-        const float qb = wv.w * tb[dy];
-#pragma unroll
-        for (int dz = 0; dz < P; ++dz) syz[dy * P + dz] = qb * tc[dz];
-      }
+      for (int d = 0; d < P; ++d) sbc[d] = tb[d], sbc[P + d] = wv.w * tc[d];
     }
     __syncthreads();
-    const int n_here = int(m - c0 < kSpreadChunk ? m - c0 : kSpreadChunk);
+    const int n_here = m - c0 < kSpreadChunk ? m - c0 : kSpreadChunk;
     for (int j0 = 0; j0 < n_here; j0 += 4) {
       const int j = j0 + quarter;
       if (j < n_here) {
-        const int2 e = s_ent[j];
-        if (e.y & rowbit) {
-          const float v = s_yz[e.x + lane_yz];
+        const int4 e = s_ent[j];
+        if (e.z & rowbit) {
+          const float v = s_bc[e.x + lane_b] * s_bc[e.y + lane_c];
           const float4 w = s_wx[j];
           acc0 += v * w.x, acc1 += v * w.y, acc2 += v * w.z, acc3 += v * w.w;
         }
@@ -214,11 +216,13 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
     }
     __syncthreads();
   }
-  // the quarters' sums by point (i + 4 j + 16 k), then each lane adds its point's four in order
+  // the quarters' sums by point (i + 4 j + 16 k), in the staging area, then each lane adds its point's four in order
+  float* s_part = s_bc;   // [4][64]; kSpreadChunk · 2p ≥ 256 floats
   const int row = 4 * gy + 16 * gz;
-  s_part[quarter][row] = acc0, s_part[quarter][row + 1] = acc1, s_part[quarter][row + 2] = acc2, s_part[quarter][row + 3] = acc3;
+  s_part[64 * quarter + row] = acc0, s_part[64 * quarter + row + 1] = acc1, s_part[64 * quarter + row + 2] = acc2,
+  s_part[64 * quarter + row + 3] = acc3;
   __syncthreads();
-  const float sum = ((s_part[0][lane] + s_part[1][lane]) + s_part[2][lane]) + s_part[3][lane];
+  const float sum = ((s_part[lane] + s_part[64 + lane]) + s_part[128 + lane]) + s_part[192 + lane];
   const int gx = 4 * tx + (lane & 3), gy_ = 4 * ty + ((lane >> 2) & 3), gz_ = 4 * tz + (lane >> 4);
   if (gx < G.k[0] && gy_ < G.k[1] && gz_ < G.k[2]) grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
 }
@@ -578,8 +582,8 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
 // (p + 3)³ floats, then per atom slot 6p weights.
 template <int P>
 __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
-                                               const float4* __restrict__ wq, const int4* __restrict__ base, const float* __restrict__ grid,
-                                               float* __restrict__ f) {
+                                               const float4* __restrict__ wsort, const int4* __restrict__ bsort,
+                                               const float* __restrict__ grid, float* __restrict__ f) {
   extern __shared__ float2 lds[];
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
@@ -609,21 +613,17 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
   for (int64_t g0 = 0; g0 < na; g0 += 8) {
     const int64_t ia = g0 + slot;
     const bool has = ia < na;
-    const uint32_t i = has ? tatoms[a0 + ia] : 0u;
-    const float4 w = wq[i];
-    const int4 b = base[i];
+    const int64_t at = a0 + (has ? ia : 0);
+    const uint32_t i = tatoms[at];
+    const float4 w = wsort[at];
+    const int4 b = bsort[at];
     float* ws = s_w + slot * 6 * P;
-    if (sub == 0) {
+    if (sub < 3) {
+      // lane sub of the atom's eight: the splines of axis sub
       float th[P], dth[P];
-      bspline<P>(w.x, th, dth);
+      bspline<P>(sub == 0 ? w.x : sub == 1 ? w.y : w.z, th, dth);
 #pragma unroll
-      for (int j = 0; j < P; ++j) ws[j] = th[j], ws[3 * P + j] = dth[j];
-      bspline<P>(w.y, th, dth);
-#pragma unroll
-      for (int j = 0; j < P; ++j) ws[P + j] = th[j], ws[4 * P + j] = dth[j];
-      bspline<P>(w.z, th, dth);
-#pragma unroll
-      for (int j = 0; j < P; ++j) ws[2 * P + j] = th[j], ws[5 * P + j] = dth[j];
+      for (int j = 0; j < P; ++j) ws[sub * P + j] = th[j], ws[(3 + sub) * P + j] = dth[j];
     }
     __syncthreads();
     // the atom's first point relative to the tile's corner (its home tile: 0 … 3 per axis)
@@ -832,15 +832,19 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tfill.p, P.tile.p);
   LAUNCHCHECK();
   kt_end(c);
+  P.wsort.resize(4 * size_t(n));
+  P.bsort.resize(4 * size_t(n));
+  float4* wsort = reinterpret_cast<float4*>(P.wsort.p);
+  int4* bsort = reinterpret_cast<int4*>(P.bsort.p);
   kt_begin(c, "k_tile_rank");
-  hipLaunchKernelGGL(k_tile_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tile.p, P.tatoms.p);
+  hipLaunchKernelGGL(k_tile_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, wq, base, P.tstart.p, P.tile.p, P.tatoms.p, wsort, bsort);
   LAUNCHCHECK();
   kt_end(c);
   kt_begin(c, "k_spread");
-  const size_t lds_spread = size_t(kSpreadChunk) * size_t(G.p * G.p) * sizeof(float);
+  const size_t lds_spread = size_t(kSpreadChunk) * size_t(2 * G.p) * sizeof(float);
   switch (G.p) {
 #define CAPSGPU_SPREAD(Q) \
-  case Q: hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), lds_spread, s, G, P.home.p, P.tstart.p, P.tatoms.p, wq, base, P.grid.p); break;
+  case Q: hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), lds_spread, s, G, P.home.p, P.tstart.p, wsort, bsort, P.grid.p); break;
     CAPSGPU_SPREAD(3) CAPSGPU_SPREAD(4) CAPSGPU_SPREAD(5) CAPSGPU_SPREAD(6) CAPSGPU_SPREAD(7) CAPSGPU_SPREAD(8)
 #undef CAPSGPU_SPREAD
   }
@@ -886,7 +890,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 3) + 8 * 6 * size_t(G.p)) * sizeof(float);
   switch (G.p) {
 #define CAPSGPU_GATHER(Q) \
-  case Q: hipLaunchKernelGGL(k_gather<Q>, dim3(unsigned(ntiles)), dim3(64), lds_gather, s, G, P.tstart.p, P.tatoms.p, wq, base, P.grid.p, P.fpme.p); break;
+  case Q: hipLaunchKernelGGL(k_gather<Q>, dim3(unsigned(ntiles)), dim3(64), lds_gather, s, G, P.tstart.p, P.tatoms.p, wsort, bsort, P.grid.p, P.fpme.p); break;
     CAPSGPU_GATHER(3) CAPSGPU_GATHER(4) CAPSGPU_GATHER(5) CAPSGPU_GATHER(6) CAPSGPU_GATHER(7) CAPSGPU_GATHER(8)
 #undef CAPSGPU_GATHER
   }
