@@ -310,10 +310,17 @@ __device__ inline void dft(float2* v, int sign) {
   }
 }
 
-// one Stockham stage of radix R over nl lines of length n: butterfly i of a line takes x_q = src[i + q n/R] × w^(q k),
-// w = exp(sign 2πi / (pR)), k = i mod p, and writes its DFT to dst[(i − k) R + k + q p]; twl holds exp(2πi m/n)
+// LDS layout of the lines being transformed: element j of a line at j + ⌊j/16⌋, lines n + ⌊n/16⌋ + 1 apart. The
+// radix-strided writes of the early Stockham stages and the column-wise loads and stores of the b and c passes then
+// fall in different banks (with plain n-apart lines of a power of two, sixteen lanes would hit one bank).
+__host__ __device__ inline int lds_at(int j) { return j + (j >> 4); }
+__host__ __device__ inline int line_stride(int n) { return n + (n >> 4) + 1; }
+
+// one Stockham stage of radix R over nl lines of length n (ls apart): butterfly i of a line takes
+// x_q = src[i + q n/R] × w^(q k), w = exp(sign 2πi / (pR)), k = i mod p, and writes its DFT to dst[(i − k) R + k + q p];
+// twl holds exp(2πi m/n)
 template <int R>
-__device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, int p, const float2* twl, int sign) {
+__device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, int ls, int p, const float2* twl, int sign) {
   const int nb = n / R, step = n / (p * R);
   for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
     const int line = t / nb, i = t - line * nb;
@@ -321,7 +328,7 @@ __device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, 
     float2 v[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const float2 x = src[line * n + i + q * nb];
+      const float2 x = src[line * ls + lds_at(i + q * nb)];
       if (q == 0) {
         v[q] = x;
       } else {
@@ -333,30 +340,49 @@ __device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, 
     dft<R>(v, sign);
     const int j = (i - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) dst[line * n + j + q * p] = v[q];
+    for (int q = 0; q < R; ++q) dst[line * ls + lds_at(j + q * p)] = v[q];
   }
 }
 
-// the axis's twiddles exp(2πi m/n) into LDS (the caller synchronises before the transform)
-__device__ inline void stage_twiddles(float2* twl, const float2* __restrict__ tw, const Fft1& F) {
-  for (int t = threadIdx.x; t < F.n; t += blockDim.x) twl[t] = tw[F.tw + t];
+// the axis's twiddles exp(2πi m/n): fetched into registers first (two per thread of 256; n ≤ 512) so their loads are
+// in flight with the data's, then stored to LDS (the caller synchronises before the transform)
+struct TwFetch {
+  float2 v[2];
+};
+static_assert(kMaxFft <= 2 * 256, "two twiddles per thread of a 256-thread transform");
+__device__ inline TwFetch fetch_twiddles(const float2* __restrict__ tw, const Fft1& F) {
+  TwFetch r;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int t = threadIdx.x + u * 256;
+    r.v[u] = t < F.n ? tw[F.tw + t] : make_float2(0.0f, 0.0f);
+  }
+  return r;
+}
+__device__ inline void store_twiddles(float2* twl, const TwFetch& r, const Fft1& F) {
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int t = threadIdx.x + u * 256;
+    if (t < F.n) twl[t] = r.v[u];
+  }
 }
 
-// nl lines of length n in A (line-major), transformed by the whole workgroup; returns the buffer with the result
+// nl lines of length n in A (laid out as lds_at and line_stride say), transformed by the whole workgroup; returns the
+// buffer with the result
 __device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const float2* twl, int sign) {
   float2* src = A;
   float2* dst = B;
-  const int n = F.n;
+  const int n = F.n, ls = line_stride(n);
   int p = 1;
   for (int s = 0; s < F.nr; ++s) {
     const int R = F.rad[s];
     switch (R) {
-      case 2: fft_stage<2>(src, dst, nl, n, p, twl, sign); break;
-      case 3: fft_stage<3>(src, dst, nl, n, p, twl, sign); break;
-      case 4: fft_stage<4>(src, dst, nl, n, p, twl, sign); break;
-      case 5: fft_stage<5>(src, dst, nl, n, p, twl, sign); break;
-      case 7: fft_stage<7>(src, dst, nl, n, p, twl, sign); break;
-      default: fft_stage<8>(src, dst, nl, n, p, twl, sign); break;
+      case 2: fft_stage<2>(src, dst, nl, n, ls, p, twl, sign); break;
+      case 3: fft_stage<3>(src, dst, nl, n, ls, p, twl, sign); break;
+      case 4: fft_stage<4>(src, dst, nl, n, ls, p, twl, sign); break;
+      case 5: fft_stage<5>(src, dst, nl, n, ls, p, twl, sign); break;
+      case 7: fft_stage<7>(src, dst, nl, n, ls, p, twl, sign); break;
+      default: fft_stage<8>(src, dst, nl, n, ls, p, twl, sign); break;
     }
     __syncthreads();
     float2* tmp = src;
@@ -367,35 +393,47 @@ __device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const 
 }
 
 constexpr int kLinePairsX = 8;   // pairs of real lines per workgroup along a
+// the transform kernels run 256 threads; their global loads are issued kFftBatch at a time into registers before the
+// LDS stores, so a thread waits once per batch rather than once per element
+constexpr int kFftThreads = 256, kFftBatch = 8;
 
 // real lines along a (index j + K2·k, pairs 2l, 2l+1) -> the half spectrum, as caps_f_r2c_lines (f90:392-423)
 __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
                                                    float2* __restrict__ half) {
   extern __shared__ float2 lds[];
-  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2;
+  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
   const int LP = kLinePairsX;
   float2* A = lds;
-  float2* B = lds + LP * K1;
-  float2* twl = B + LP * K1;
-  stage_twiddles(twl, tw, FP.ax[0]);
+  float2* B = lds + LP * ls;
+  float2* twl = B + LP * ls;
+  const TwFetch twf = fetch_twiddles(tw, FP.ax[0]);
   const int pb = blockIdx.x * LP;
-  for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
-    const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
-    float2 z = make_float2(0.0f, 0.0f);
-    if (pr < npairs) {
-      const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
-      z.x = grid[size_t(la) * K1 + m];
-      z.y = lb != la ? grid[size_t(lb) * K1 + m] : 0.0f;
+  for (int t0 = 0; t0 < LP * K1; t0 += kFftBatch * kFftThreads) {
+    float2 r[kFftBatch];
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, pl = t / K1, m = t - pl * K1, pr = pb + pl;
+      r[u] = make_float2(0.0f, 0.0f);
+      if (t < LP * K1 && pr < npairs) {
+        const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
+        r[u].x = grid[size_t(la) * K1 + m];
+        r[u].y = lb != la ? grid[size_t(lb) * K1 + m] : 0.0f;
+      }
     }
-    A[pl * K1 + m] = z;
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, pl = t / K1, m = t - pl * K1;
+      if (t < LP * K1) A[pl * ls + lds_at(m)] = r[u];
+    }
   }
+  store_twiddles(twl, twf, FP.ax[0]);
   __syncthreads();
   const float2* Z = fft_lines(A, B, LP, FP.ax[0], twl, +1);
   for (int t = threadIdx.x; t < LP * h1; t += blockDim.x) {
     const int pl = t / h1, m = t - pl * h1, pr = pb + pl;
     if (pr >= npairs) continue;
     const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
-    const float2 zm = Z[pl * K1 + m], zc0 = Z[pl * K1 + (K1 - m) % K1];
+    const float2 zm = Z[pl * ls + lds_at(m)], zc0 = Z[pl * ls + lds_at((K1 - m) % K1)];
     const float2 zc = make_float2(zc0.x, -zc0.y);
     half[size_t(la) * h1 + m] = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
     if (lb != la) {
@@ -409,35 +447,46 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const
 __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
                                                    float* __restrict__ grid) {
   extern __shared__ float2 lds[];
-  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2;
+  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
   const int LP = kLinePairsX;
   float2* A = lds;
-  float2* B = lds + LP * K1;
-  float2* twl = B + LP * K1;
-  stage_twiddles(twl, tw, FP.ax[0]);
+  float2* B = lds + LP * ls;
+  float2* twl = B + LP * ls;
+  const TwFetch twf = fetch_twiddles(tw, FP.ax[0]);
   const int pb = blockIdx.x * LP;
-  for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
-    const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
-    float2 z = make_float2(0.0f, 0.0f);
-    if (pr < npairs) {
-      const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
-      const bool up = m > K1 / 2;
-      const int mh = up ? K1 - m : m;
-      float2 a = half[size_t(la) * h1 + mh];
-      float2 b = lb != la ? half[size_t(lb) * h1 + mh] : make_float2(0.0f, 0.0f);
-      if (up) a.y = -a.y, b.y = -b.y;
-      z = make_float2(a.x - b.y, a.y + b.x);
+  for (int t0 = 0; t0 < LP * K1; t0 += kFftBatch * kFftThreads) {
+    float2 ra[kFftBatch], rb[kFftBatch];
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, pl = t / K1, m = t - pl * K1, pr = pb + pl;
+      ra[u] = rb[u] = make_float2(0.0f, 0.0f);
+      if (t < LP * K1 && pr < npairs) {
+        const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
+        const int mh = m > K1 / 2 ? K1 - m : m;
+        ra[u] = half[size_t(la) * h1 + mh];
+        if (lb != la) rb[u] = half[size_t(lb) * h1 + mh];
+      }
     }
-    A[pl * K1 + m] = z;
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, pl = t / K1, m = t - pl * K1;
+      if (t < LP * K1) {
+        float2 a = ra[u], b = rb[u];
+        if (m > K1 / 2) a.y = -a.y, b.y = -b.y;
+        A[pl * ls + lds_at(m)] = make_float2(a.x - b.y, a.y + b.x);
+      }
+    }
   }
+  store_twiddles(twl, twf, FP.ax[0]);
   __syncthreads();
   const float2* Z = fft_lines(A, B, LP, FP.ax[0], twl, -1);
   for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
     const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
     if (pr >= npairs) continue;
     const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
-    grid[size_t(la) * K1 + m] = Z[pl * K1 + m].x;
-    if (lb != la) grid[size_t(lb) * K1 + m] = Z[pl * K1 + m].y;
+    const float2 z = Z[pl * ls + lds_at(m)];
+    grid[size_t(la) * K1 + m] = z.x;
+    if (lb != la) grid[size_t(lb) * K1 + m] = z.y;
   }
 }
 
@@ -449,20 +498,30 @@ __global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const flo
   const int h1 = G.h1, K2 = G.k[1];
   const int mblocks = (h1 + kLinesYZ - 1) / kLinesYZ;
   const int kz = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) * kLinesYZ;
-  const int nl = min(kLinesYZ, h1 - m0);
+  const int nl = min(kLinesYZ, h1 - m0), ls = line_stride(K2);
   float2* A = lds;
-  float2* B = lds + kLinesYZ * K2;
-  float2* twl = B + kLinesYZ * K2;
-  stage_twiddles(twl, tw, FP.ax[1]);
-  for (int t = threadIdx.x; t < kLinesYZ * K2; t += blockDim.x) {
-    const int j = t / kLinesYZ, li = t % kLinesYZ;
-    if (li < nl) A[li * K2 + j] = half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)];
+  float2* B = lds + kLinesYZ * ls;
+  float2* twl = B + kLinesYZ * ls;
+  const TwFetch twf = fetch_twiddles(tw, FP.ax[1]);
+  for (int t0 = 0; t0 < kLinesYZ * K2; t0 += kFftBatch * kFftThreads) {
+    float2 r[kFftBatch];
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, j = t / kLinesYZ, li = t % kLinesYZ;
+      r[u] = t < kLinesYZ * K2 && li < nl ? half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, j = t / kLinesYZ, li = t % kLinesYZ;
+      if (t < kLinesYZ * K2 && li < nl) A[li * ls + lds_at(j)] = r[u];
+    }
   }
+  store_twiddles(twl, twf, FP.ax[1]);
   __syncthreads();
   const float2* Z = fft_lines(A, B, nl, FP.ax[1], twl, sign);
   for (int t = threadIdx.x; t < kLinesYZ * K2; t += blockDim.x) {
     const int j = t / kLinesYZ, li = t % kLinesYZ;
-    if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * K2 + j];
+    if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * ls + lds_at(j)];
   }
 }
 
@@ -526,40 +585,61 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
   const int h1 = G.h1, K2 = G.k[1], K3 = G.k[2];
   const int mblocks = (h1 + kLinesYZ - 1) / kLinesYZ;
   const int jy = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) * kLinesYZ;
-  const int nl = min(kLinesYZ, h1 - m0);
+  const int nl = min(kLinesYZ, h1 - m0), ls = line_stride(K3);
   float2* A = lds;
-  float2* B = lds + kLinesYZ * K3;
-  float2* twl = B + kLinesYZ * K3;
-  stage_twiddles(twl, tw, FP.ax[2]);
+  float2* B = lds + kLinesYZ * ls;
+  float2* twl = B + kLinesYZ * ls;
+  const TwFetch twf = fetch_twiddles(tw, FP.ax[2]);
   const size_t zs = size_t(h1) * K2;
-  for (int t = threadIdx.x; t < kLinesYZ * K3; t += blockDim.x) {
-    const int k = t / kLinesYZ, li = t % kLinesYZ;
-    if (li < nl) A[li * K3 + k] = half[size_t(m0 + li) + size_t(h1) * jy + zs * k];
+  for (int t0 = 0; t0 < kLinesYZ * K3; t0 += kFftBatch * kFftThreads) {
+    float2 r[kFftBatch];
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, k = t / kLinesYZ, li = t % kLinesYZ;
+      r[u] = t < kLinesYZ * K3 && li < nl ? half[size_t(m0 + li) + size_t(h1) * jy + zs * k] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, k = t / kLinesYZ, li = t % kLinesYZ;
+      if (t < kLinesYZ * K3 && li < nl) A[li * ls + lds_at(k)] = r[u];
+    }
   }
+  store_twiddles(twl, twf, FP.ax[2]);
   __syncthreads();
   float2* Z = fft_lines(A, B, nl, FP.ax[2], twl, +1);
   double e = 0, w[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
-    const int li = t / K3, k = t - li * K3, a = m0 + li;
-    const float g = gtab[size_t(a) + size_t(h1) * jy + zs * k];
-    float2 v = Z[li * K3 + k];
-    if (kE && (a || jy || k)) {
-      double em = 0.5 * (double(v.x) * v.x + double(v.y) * v.y);
-      if (a > 0 && 2 * a != G.k[0]) em *= 2;
-      double gd, wv[6];
-      green_at(G, a, jy, k, fac, kcoul, vol, bm, &gd, wv);
-      e += em * double(g);
-      for (int q = 0; q < 6; ++q) w[q] += em * wv[q];
+  for (int t0 = 0; t0 < nl * K3; t0 += kFftBatch * kFftThreads) {
+    float gb[kFftBatch];
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x, li = t / K3, k = t - li * K3;
+      gb[u] = t < nl * K3 ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
     }
-    v.x *= g, v.y *= g;
-    Z[li * K3 + k] = v;
+#pragma unroll
+    for (int u = 0; u < kFftBatch; ++u) {
+      const int t = t0 + u * kFftThreads + threadIdx.x;
+      if (t >= nl * K3) continue;
+      const int li = t / K3, k = t - li * K3, a = m0 + li;
+      const float g = gb[u];
+      float2 v = Z[li * ls + lds_at(k)];
+      if (kE && (a || jy || k)) {
+        double em = 0.5 * (double(v.x) * v.x + double(v.y) * v.y);
+        if (a > 0 && 2 * a != G.k[0]) em *= 2;
+        double gd, wv[6];
+        green_at(G, a, jy, k, fac, kcoul, vol, bm, &gd, wv);
+        e += em * double(g);
+        for (int q = 0; q < 6; ++q) w[q] += em * wv[q];
+      }
+      v.x *= g, v.y *= g;
+      Z[li * ls + lds_at(k)] = v;
+    }
   }
   __syncthreads();
   float2* other = Z == A ? B : A;
   const float2* R = fft_lines(Z, other, nl, FP.ax[2], twl, -1);
   for (int t = threadIdx.x; t < kLinesYZ * K3; t += blockDim.x) {
     const int k = t / kLinesYZ, li = t % kLinesYZ;
-    if (li < nl) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * K3 + k];
+    if (li < nl) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
   }
   if (kE) {
     // the workgroup's partial sums in a fixed order
@@ -579,7 +659,7 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
 // one 64-lane workgroup per 4x4x4 tile: the tile's box of (p + 3)³ mesh points in LDS (loaded row by row, wrapped),
 // then its atoms eight at a time, eight lanes per atom, each lane summing a share of the atom's p² rows; the eight
 // lanes' sums are added in a fixed order (butterfly) and the first lane writes the atom's force. Dynamic LDS: the box
-// (p + 3)³ floats, then per atom slot 6p weights.
+// (p + 3)² rows of p + 4 floats, then per atom slot 6p weights.
 template <int P>
 __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
                                                const float4* __restrict__ wsort, const int4* __restrict__ bsort,
@@ -591,9 +671,10 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
   if (tile >= ntiles) return;
   const int64_t a0 = tstart[tile], na = tstart[tile + 1] - a0;
   if (na == 0) return;
-  constexpr int p = P, W = P + 3;
-  float* box = reinterpret_cast<float*>(lds);            // [W][W][W] (c, b, a)
-  float* s_w = box + W * W * W;                          // [8][6 P]: θa θb θc dθa dθb dθc
+  constexpr int p = P, W = P + 3, WX = W + 1;          // rows padded to W + 1 floats: the eight rows an atom's lanes
+                                                         // read fall in different LDS banks
+  float* box = reinterpret_cast<float*>(lds);            // [W][W][WX] (c, b, a)
+  float* s_w = box + W * W * WX;                         // [8][6 P]: θa θb θc dθa dθb dθc
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
   for (int r = lane; r < W * W; r += 64) {
     int yy = 4 * ty + r % W, zz = 4 * tz + r / W;
@@ -606,7 +687,7 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
       int gx = 4 * tx + xx;
       gx -= gx >= G.k[0] ? G.k[0] : 0;
       gx -= gx >= G.k[0] ? G.k[0] : 0;
-      box[r * W + xx] = src[gx];
+      box[r * WX + xx] = src[gx];
     }
   }
   const int slot = lane >> 3, sub = lane & 7;
@@ -635,7 +716,7 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
     if (has)
       for (int r = sub; r < p * p; r += 8) {
         const int dy = r % p, dz = r / p;
-        const float* row = box + ((oz + dz) * W + (oy + dy)) * W + ox;
+        const float* row = box + ((oz + dz) * W + (oy + dy)) * WX + ox;
         float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
         for (int a = 0; a < P; ++a) {
@@ -851,14 +932,15 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   LAUNCHCHECK();
   kt_end(c);
   const int npairs = (P.k[1] * P.k[2] + 1) / 2;
-  const size_t lds_x = (2 * kLinePairsX + 1) * size_t(P.k[0]) * sizeof(float2);
+  const size_t lds_x = (2 * kLinePairsX * size_t(line_stride(P.k[0])) + P.k[0]) * sizeof(float2);
   kt_begin(c, "k_fft_x_r2c");
   hipLaunchKernelGGL(k_fft_x_r2c, dim3(unsigned((npairs + kLinePairsX - 1) / kLinePairsX)), dim3(256), lds_x, s, G, F, P.tw.p, P.grid.p,
                      P.half.p);
   LAUNCHCHECK();
   kt_end(c);
   const int mblocks = (P.h1 + kLinesYZ - 1) / kLinesYZ;
-  const size_t lds_y = (2 * kLinesYZ + 1) * size_t(P.k[1]) * sizeof(float2), lds_z = (2 * kLinesYZ + 1) * size_t(P.k[2]) * sizeof(float2);
+  const size_t lds_y = (2 * kLinesYZ * size_t(line_stride(P.k[1])) + P.k[1]) * sizeof(float2);
+  const size_t lds_z = (2 * kLinesYZ * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2);
   kt_begin(c, "k_fft_y");
   hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, +1);
   LAUNCHCHECK();
@@ -887,7 +969,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   kt_end(c);
   P.fpme.resize(3 * size_t(n));
   kt_begin(c, "k_gather");
-  const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 3) + 8 * 6 * size_t(G.p)) * sizeof(float);
+  const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 4) + 8 * 6 * size_t(G.p)) * sizeof(float);
   switch (G.p) {
 #define CAPSGPU_GATHER(Q) \
   case Q: hipLaunchKernelGGL(k_gather<Q>, dim3(unsigned(ntiles)), dim3(64), lds_gather, s, G, P.tstart.p, P.tatoms.p, wsort, bsort, P.grid.p, P.fpme.p); break;
