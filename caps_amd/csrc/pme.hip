@@ -122,7 +122,6 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
   extern __shared__ float2 lds[];
   __shared__ float4 s_wx[kSpreadChunk];   // θa[r_a + i] for the tile's four points i along a (zero outside the support)
   __shared__ int4 s_ent[kSpreadChunk];    // {2p j + r_b + 3, 2p j + p + r_c + 3, the rows (j, k) of the tile it reaches, 0}
-  __shared__ int s_hb[kMaxHome], s_pre[kMaxHome + 1];
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
   const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
@@ -135,32 +134,36 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
   const int* hy = home + (1 * G.ntm + ty) * 5;
   const int* hz = home + (2 * G.ntm + tz) * 5;
   const int mx = hx[0], my = hy[0], mz = hz[0], nh = mx * my * mz;
-  int cnt = 0;   // atom counts fit 32 bits (the library's atom count is an int)
+  // lane k < nh: home tile k's first atom (hb) and atom count; the counts' running sum gives each tile's first candidate
+  // number (ex); the lanes' values are read back with readlane (the home tiles are few: at most 27 for p = 8)
+  int cnt = 0, hb = 0;   // atom counts fit 32 bits (the library's atom count is an int)
   if (lane < nh) {
     const int ix = lane % mx, iy = (lane / mx) % my, iz = lane / (mx * my);
     const int h = (hz[1 + iz] * G.nt[1] + hy[1 + iy]) * G.nt[0] + hx[1 + ix];
-    s_hb[lane] = int(tstart[h]);
+    hb = int(tstart[h]);
     cnt = int(tstart[h + 1] - tstart[h]);
   }
+  const int own = cnt;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const int o = __shfl_up(cnt, d, 64);
     if (lane >= d) cnt += o;
   }
-  if (lane < nh) s_pre[lane + 1] = cnt;
-  if (lane == 0) s_pre[0] = 0;
-  __syncthreads();
-  const int m = s_pre[nh];
+  const int ex = cnt - own;
+  const int m = __builtin_amdgcn_readlane(cnt, nh - 1);
   // lane = (row (j, k) of the tile, quarter of the atoms); its row's bit in the reach mask is j + 4 k = lane & 15
   const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4, rowbit = 1 << (lane & 15);
   const int lane_b = gy - 3, lane_c = gz - 3;
   float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
   for (int c0 = 0; c0 < m; c0 += kSpreadChunk) {
     const int f = c0 + lane;
+    // candidate f lies in the last home tile whose first candidate number is ≤ f (empty tiles are passed over)
+    int at = 0;
+    for (int k = 0; k < nh; ++k) {
+      const int exk = __builtin_amdgcn_readlane(ex, k), hbk = __builtin_amdgcn_readlane(hb, k);
+      at = f >= exk ? hbk + (f - exk) : at;
+    }
     if (f < m) {
-      int t = 0;
-      while (s_pre[t + 1] <= f) ++t;
-      const int at = s_hb[t] + (f - s_pre[t]);
       const int4 b = bsort[at];
       const float4 wv = wsort[at];
       // r = 4t − b lies in (−K, K + 3): to [0, K) and then into [−3, K − 4], and at most p (beyond, no point is reached)
@@ -202,17 +205,18 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
       for (int d = 0; d < P; ++d) sbc[d] = tb[d], sbc[P + d] = wv.w * tc[d];
     }
     __syncthreads();
+    // branch-free: an atom that does not reach the lane's row (or a slot past the chunk's end) adds an exact zero, so the
+    // loop unrolls and its LDS reads overlap
     const int n_here = m - c0 < kSpreadChunk ? m - c0 : kSpreadChunk;
+#pragma unroll 4
     for (int j0 = 0; j0 < n_here; j0 += 4) {
-      const int j = j0 + quarter;
-      if (j < n_here) {
-        const int4 e = s_ent[j];
-        if (e.z & rowbit) {
-          const float v = s_bc[e.x + lane_b] * s_bc[e.y + lane_c];
-          const float4 w = s_wx[j];
-          acc0 += v * w.x, acc1 += v * w.y, acc2 += v * w.z, acc3 += v * w.w;
-        }
-      }
+      const int j = j0 + quarter < n_here ? j0 + quarter : 0;
+      const int4 e = s_ent[j];
+      const bool hit = j0 + quarter < n_here && (e.z & rowbit) != 0;
+      const float vb = s_bc[hit ? e.x + lane_b : 0], vc = s_bc[hit ? e.y + lane_c : 0];
+      const float v = hit ? vb * vc : 0.0f;
+      const float4 w = s_wx[j];
+      acc0 += v * w.x, acc1 += v * w.y, acc2 += v * w.z, acc3 += v * w.w;
     }
     __syncthreads();
   }
