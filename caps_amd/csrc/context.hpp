@@ -94,6 +94,7 @@ struct Pme {
   DevBuf<int32_t> base;                 // per atom: the first grid index per axis and the home tile (int4)
   DevBuf<int32_t> home;                 // per axis and tile: the home tiles that reach it (count + 4)
   DevBuf<uint32_t> tile, tcount, tfill, tatoms;   // tile: the atoms filled unordered per tile; tatoms: in index order
+  bool counts_clear = false;            // tcount and tfill are zero on the device (k_tile_rank clears them)
   DevBuf<float> wsort;                  // theta in tatoms's order (float4 per atom)
   DevBuf<int32_t> bsort;                // base in tatoms's order (int4 per atom)
   DevBuf<int64_t> tstart;

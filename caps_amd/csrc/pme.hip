@@ -92,11 +92,14 @@ __global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t*
 
 // the atoms of each tile in index order: an atom's place is its rank among its tile's atoms (filled unordered in tmp);
 // its spline offsets, charge and base point are copied to that place too, so the spread and the gather read a tile's
-// atoms contiguously
-__global__ void k_tile_rank(int n, const float4* __restrict__ wq, const int4* __restrict__ base, const int64_t* __restrict__ tstart,
-                            const uint32_t* __restrict__ tmp, uint32_t* __restrict__ tatoms, float4* __restrict__ wsort,
-                            int4* __restrict__ bsort) {
+// atoms contiguously. The tile counters (read by the scan and by k_tile_fill, both done) are cleared here for the next
+// evaluation.
+__global__ void k_tile_rank(int n, int ntiles, const float4* __restrict__ wq, const int4* __restrict__ base,
+                            const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tmp, uint32_t* __restrict__ tatoms,
+                            float4* __restrict__ wsort, int4* __restrict__ bsort, uint32_t* __restrict__ tcount,
+                            uint32_t* __restrict__ tfill) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int t = i; t < ntiles; t += gridDim.x * blockDim.x) tcount[t] = 0u, tfill[t] = 0u;
   if (i >= n) return;
   const int4 bi = base[i];
   const float4 wi = wq[i];
@@ -887,6 +890,7 @@ void pme_setup(capsgpu_ctx& c) {
   P.half.resize(nh);
   for (int q = 0; q < 12; ++q) P.cell[q] = cell[q];
   P.green_ok = true;
+  P.counts_clear = false;   // a new grid: its tile counters are cleared before the next spread
 }
 
 void pme_forces(capsgpu_ctx& c, bool want) {
@@ -900,7 +904,14 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   P.theta.resize(4 * size_t(n));
   P.base.resize(4 * size_t(n));
   P.tcount.resize(size_t(ntiles));
-  P.tcount.zero(s);
+  P.tfill.resize(size_t(ntiles));
+  // the tile counters start at zero: cleared here after a new grid (or an evaluation that did not finish), else left
+  // clear by the previous evaluation's k_tile_rank
+  if (!P.counts_clear) {
+    P.tcount.zero(s);
+    P.tfill.zero(s);
+  }
+  P.counts_clear = false;
   float4* wq = reinterpret_cast<float4*>(P.theta.p);
   int4* base = reinterpret_cast<int4*>(P.base.p);
   kt_begin(c, "k_splines");
@@ -909,8 +920,6 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   kt_end(c);
   P.tstart.resize(size_t(ntiles) + 1);
   c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
-  P.tfill.resize(size_t(ntiles));
-  P.tfill.zero(s);
   P.tile.resize(size_t(n));
   P.tatoms.resize(size_t(n));
   kt_begin(c, "k_tile_fill");
@@ -922,8 +931,10 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   float4* wsort = reinterpret_cast<float4*>(P.wsort.p);
   int4* bsort = reinterpret_cast<int4*>(P.bsort.p);
   kt_begin(c, "k_tile_rank");
-  hipLaunchKernelGGL(k_tile_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, wq, base, P.tstart.p, P.tile.p, P.tatoms.p, wsort, bsort);
+  hipLaunchKernelGGL(k_tile_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ntiles, wq, base, P.tstart.p, P.tile.p, P.tatoms.p, wsort,
+                     bsort, P.tcount.p, P.tfill.p);
   LAUNCHCHECK();
+  P.counts_clear = true;
   kt_end(c);
   kt_begin(c, "k_spread");
   const size_t lds_spread = size_t(kSpreadChunk) * size_t(2 * G.p) * sizeof(float);
