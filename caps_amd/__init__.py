@@ -129,6 +129,8 @@ def lib():
         L.capsgpu_learned_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                                C.POINTER(C.c_double)]
         L.capsgpu_learned_descriptors.argtypes = [C.c_void_p, C.c_void_p]
+        L.capsgpu_tune.argtypes = [C.c_void_p, C.c_void_p]
+        L.capsgpu_set_tiles.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.capsgpu_batch_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
         L.capsgpu_batch_run.argtypes = [C.c_void_p, C.POINTER(BOpts), C.c_void_p, C.c_void_p, C.POINTER(BReport)]
         L.capsgpu_batch_destroy.argtypes = [C.c_void_p]
@@ -321,6 +323,19 @@ class Evaluator:
     def profile_only(self, kernel=None):
         """time only the launches of one kernel (None: every kernel) and no stages"""
         _check(lib().capsgpu_profile_only(self._h, kernel.encode() if kernel else None))
+
+    def tune(self):
+        """picks this context's mesh-transform tile shape by timing (no reference counterpart); returns (line pairs along
+        a, lines along b and c) per workgroup. Needs PME and a set state (set_state or a compute first)."""
+        t = np.zeros(2, np.int32)
+        _check(lib().capsgpu_tune(self._h, _p(t)))
+        return int(t[0]), int(t[1])
+
+    def set_tiles(self, lines_x, lines_yz):
+        """sets the mesh-transform tile shape (powers of two, 1 to 16 lines)"""
+        t = np.zeros(2, np.int32)
+        _check(lib().capsgpu_set_tiles(self._h, int(lines_x), int(lines_yz), _p(t)))
+        return int(t[0]), int(t[1])
 
     def profile_read(self):
         """{stage: (summed device ms, evaluations)} since profiling started"""

@@ -239,8 +239,7 @@ void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts,
       colsum(c, c.partials.p, blocks_for(c.L.g.ns, 256), 9, out + 11);
     }
     if (recip) {
-      const int mblocks = (c.pme.h1 + 15) / 16;
-      colsum(c, c.pme.epart.p, int64_t(mblocks) * c.pme.k[1], 7, out + 20);
+      colsum(c, c.pme.epart.p, c.pme.erows, 7, out + 20);
     }
     if (bonded) colsum(c, c.bonded.epart.p, c.bonded.eblocks, 9, out + 27);
     HIPCHECK(hipMemcpyAsync(c.h_terms, out, 36 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -783,6 +782,29 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       rep->full_steps = int32_t(full_steps);
       rep->fallbacks = int32_t(fallbacks);
     }
+  });
+}
+
+int32_t capsgpu_tune(capsgpu_ctx* c, int32_t tiles[2]) {
+  return guard([&] {
+    if (!c) throw Error(CAPSGPU_EINVAL, "null argument");
+    if (!pme_on(*c)) throw Error(CAPSGPU_EINVAL, "the tuned tile shapes are the PME transforms': the context has no PME");
+    if (!c->state_set) throw Error(CAPSGPU_EINVAL, "set the positions first (capsgpu_set_state)");
+    if (!c->pme.green_ok) pme_forces(*c, false);
+    int t[2];
+    pme_tune(*c, t);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (tiles) tiles[0] = t[0], tiles[1] = t[1];
+  });
+}
+
+int32_t capsgpu_set_tiles(capsgpu_ctx* c, int32_t lines_x, int32_t lines_yz, int32_t tiles[2]) {
+  return guard([&] {
+    if (!c) throw Error(CAPSGPU_EINVAL, "null argument");
+    auto pow2 = [](int32_t v) { return v >= 1 && v <= 16 && (v & (v - 1)) == 0; };
+    if (!pow2(lines_x) || !pow2(lines_yz)) throw Error(CAPSGPU_EINVAL, "tile shapes are powers of two from 1 to 16 lines");
+    c->pme.lines_x = lines_x, c->pme.lines_yz = lines_yz;
+    if (tiles) tiles[0] = lines_x, tiles[1] = lines_yz;
   });
 }
 

@@ -26,6 +26,10 @@ namespace capsgpu {
 void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts, double* fdst);
 }
 
+namespace {
+bool pme_on_ctx(const capsgpu_ctx& c) { return c.opt.coulomb && c.opt.electrostatics == 1 && !c.opt.no_reciprocal && c.pme.green_ok; }
+}
+
 using namespace capsgpu;
 
 namespace {
@@ -312,6 +316,14 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
     sh = b.tuned;
   } else {
     const auto t0 = std::chrono::steady_clock::now();
+    // each replica's transform tiles (its mesh is the first evaluation's; the next evaluation overwrites it), then the
+    // launch shape
+    for (Rep& R : b.r)
+      if (pme_on_ctx(*R.c)) {
+        int t[2];
+        pme_tune(*R.c, t);
+      }
+    HIPCHECK(hipDeviceSynchronize());
     sh = tune(b);
     tune_s = seconds_since(t0);
     b.have_tuned = true, b.tuned = sh, b.tuned_key = key;

@@ -101,6 +101,8 @@ struct Pme {
   DevBuf<float> fpme;                   // 3 per atom
   DevBuf<float2> tw;                    // twiddles of the three axes
   DevBuf<double> epart;                 // energy and virial partials
+  int64_t erows = 0;                    // rows of epart written by the last evaluation with energies
+  int lines_x = 8, lines_yz = 16;       // the transforms' tile shape (line pairs along a, lines along b and c)
 };
 
 struct Bonded {
@@ -184,6 +186,8 @@ void pair_forces(capsgpu_ctx& c, bool want);
 // pme.hip: reciprocal forces into pme.fpme (atom order); energy and virial partials when want
 void pme_setup(capsgpu_ctx& c);
 void pme_forces(capsgpu_ctx& c, bool want);
+// picks the transforms' tile shape by timing (out: line pairs along a, lines along b and c per workgroup)
+void pme_tune(capsgpu_ctx& c, int out[2]);
 // bonded.hip
 void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff);
 void bonded_forces(capsgpu_ctx& c, bool want);

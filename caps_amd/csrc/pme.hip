@@ -57,6 +57,8 @@ __device__ inline void bspline(float w, float* th, float* dth) {
 
 struct GridArgs {
   int k[3], nt[3], p, h1, ntm;   // ntm: the largest of nt (the home table's stride)
+  int lx;                        // the transforms' tile shape: pairs of real lines per workgroup along a,
+  int ly, ly_log2;               // and lines per workgroup along b and c (powers of two)
   double hi[3][3];
 };
 
@@ -399,7 +401,6 @@ __device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const 
   return src;
 }
 
-constexpr int kLinePairsX = 8;   // pairs of real lines per workgroup along a
 // the transform kernels run 256 threads; their global loads are issued kFftBatch at a time into registers before the
 // LDS stores, so a thread waits once per batch rather than once per element
 constexpr int kFftThreads = 256, kFftBatch = 8;
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const
                                                    float2* __restrict__ half) {
   extern __shared__ float2 lds[];
   const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
-  const int LP = kLinePairsX;
+  const int LP = G.lx;
   float2* A = lds;
   float2* B = lds + LP * ls;
   float2* twl = B + LP * ls;
@@ -455,7 +456,7 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
                                                    float* __restrict__ grid) {
   extern __shared__ float2 lds[];
   const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
-  const int LP = kLinePairsX;
+  const int LP = G.lx;
   float2* A = lds;
   float2* B = lds + LP * ls;
   float2* twl = B + LP * ls;
@@ -497,37 +498,36 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
   }
 }
 
-constexpr int kLinesYZ = 16;   // lines (consecutive a-frequencies) per workgroup along b and c
-
 // lines along b of the half spectrum: element (m, j, k) at m + h1 (j + K2 k)
 __global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half, int sign) {
   extern __shared__ float2 lds[];
   const int h1 = G.h1, K2 = G.k[1];
-  const int mblocks = (h1 + kLinesYZ - 1) / kLinesYZ;
-  const int kz = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) * kLinesYZ;
-  const int nl = min(kLinesYZ, h1 - m0), ls = line_stride(K2);
+  const int LY = G.ly, lg = G.ly_log2;   // lines (consecutive a-frequencies) per workgroup
+  const int mblocks = (h1 + LY - 1) >> lg;
+  const int kz = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) << lg;
+  const int nl = min(LY, h1 - m0), ls = line_stride(K2);
   float2* A = lds;
-  float2* B = lds + kLinesYZ * ls;
-  float2* twl = B + kLinesYZ * ls;
+  float2* B = lds + LY * ls;
+  float2* twl = B + LY * ls;
   const TwFetch twf = fetch_twiddles(tw, FP.ax[1]);
-  for (int t0 = 0; t0 < kLinesYZ * K2; t0 += kFftBatch * kFftThreads) {
+  for (int t0 = 0; t0 < LY * K2; t0 += kFftBatch * kFftThreads) {
     float2 r[kFftBatch];
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
-      const int t = t0 + u * kFftThreads + threadIdx.x, j = t / kLinesYZ, li = t % kLinesYZ;
-      r[u] = t < kLinesYZ * K2 && li < nl ? half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] : make_float2(0.0f, 0.0f);
+      const int t = t0 + u * kFftThreads + threadIdx.x, j = t >> lg, li = t & (LY - 1);
+      r[u] = t < LY * K2 && li < nl ? half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
-      const int t = t0 + u * kFftThreads + threadIdx.x, j = t / kLinesYZ, li = t % kLinesYZ;
-      if (t < kLinesYZ * K2 && li < nl) A[li * ls + lds_at(j)] = r[u];
+      const int t = t0 + u * kFftThreads + threadIdx.x, j = t >> lg, li = t & (LY - 1);
+      if (t < LY * K2 && li < nl) A[li * ls + lds_at(j)] = r[u];
     }
   }
   store_twiddles(twl, twf, FP.ax[1]);
   __syncthreads();
   const float2* Z = fft_lines(A, B, nl, FP.ax[1], twl, sign);
-  for (int t = threadIdx.x; t < kLinesYZ * K2; t += blockDim.x) {
-    const int j = t / kLinesYZ, li = t % kLinesYZ;
+  for (int t = threadIdx.x; t < LY * K2; t += blockDim.x) {
+    const int j = t >> lg, li = t & (LY - 1);
     if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * ls + lds_at(j)];
   }
 }
@@ -590,25 +590,26 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
                                                     const double* __restrict__ bm, double* __restrict__ epart) {
   extern __shared__ float2 lds[];
   const int h1 = G.h1, K2 = G.k[1], K3 = G.k[2];
-  const int mblocks = (h1 + kLinesYZ - 1) / kLinesYZ;
-  const int jy = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) * kLinesYZ;
-  const int nl = min(kLinesYZ, h1 - m0), ls = line_stride(K3);
+  const int LY = G.ly, lg = G.ly_log2;
+  const int mblocks = (h1 + LY - 1) >> lg;
+  const int jy = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) << lg;
+  const int nl = min(LY, h1 - m0), ls = line_stride(K3);
   float2* A = lds;
-  float2* B = lds + kLinesYZ * ls;
-  float2* twl = B + kLinesYZ * ls;
+  float2* B = lds + LY * ls;
+  float2* twl = B + LY * ls;
   const TwFetch twf = fetch_twiddles(tw, FP.ax[2]);
   const size_t zs = size_t(h1) * K2;
-  for (int t0 = 0; t0 < kLinesYZ * K3; t0 += kFftBatch * kFftThreads) {
+  for (int t0 = 0; t0 < LY * K3; t0 += kFftBatch * kFftThreads) {
     float2 r[kFftBatch];
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
-      const int t = t0 + u * kFftThreads + threadIdx.x, k = t / kLinesYZ, li = t % kLinesYZ;
-      r[u] = t < kLinesYZ * K3 && li < nl ? half[size_t(m0 + li) + size_t(h1) * jy + zs * k] : make_float2(0.0f, 0.0f);
+      const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
+      r[u] = t < LY * K3 && li < nl ? half[size_t(m0 + li) + size_t(h1) * jy + zs * k] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
-      const int t = t0 + u * kFftThreads + threadIdx.x, k = t / kLinesYZ, li = t % kLinesYZ;
-      if (t < kLinesYZ * K3 && li < nl) A[li * ls + lds_at(k)] = r[u];
+      const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
+      if (t < LY * K3 && li < nl) A[li * ls + lds_at(k)] = r[u];
     }
   }
   store_twiddles(twl, twf, FP.ax[2]);
@@ -644,8 +645,8 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
   __syncthreads();
   float2* other = Z == A ? B : A;
   const float2* R = fft_lines(Z, other, nl, FP.ax[2], twl, -1);
-  for (int t = threadIdx.x; t < kLinesYZ * K3; t += blockDim.x) {
-    const int k = t / kLinesYZ, li = t % kLinesYZ;
+  for (int t = threadIdx.x; t < LY * K3; t += blockDim.x) {
+    const int k = t >> lg, li = t & (LY - 1);
     if (li < nl) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
   }
   if (kE) {
@@ -757,6 +758,10 @@ GridArgs grid_args(const Pme& P) {
   G.ntm = std::max(G.nt[0], std::max(G.nt[1], G.nt[2]));
   G.p = P.order;
   G.h1 = P.h1;
+  G.lx = P.lines_x;
+  G.ly = P.lines_yz;
+  G.ly_log2 = 0;
+  while ((1 << G.ly_log2) < G.ly) ++G.ly_log2;
   for (int r = 0; r < 3; ++r)
     for (int q = 0; q < 3; ++q) G.hi[r][q] = P.hi[r][q];
   return G;
@@ -893,6 +898,120 @@ void pme_setup(capsgpu_ctx& c) {
   P.counts_clear = false;   // a new grid: its tile counters are cleared before the next spread
 }
 
+namespace {
+// the transforms' launches with the context's tile shape (G.lx line pairs, G.ly lines per workgroup)
+void launch_fft_x(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool forward) {
+  Pme& P = c.pme;
+  const int npairs = (P.k[1] * P.k[2] + 1) / 2;
+  const size_t lds_x = (2 * size_t(G.lx) * size_t(line_stride(P.k[0])) + P.k[0]) * sizeof(float2);
+  const dim3 grid(unsigned((npairs + G.lx - 1) / G.lx));
+  if (forward) {
+    kt_begin(c, "k_fft_x_r2c");
+    hipLaunchKernelGGL(k_fft_x_r2c, grid, dim3(256), lds_x, c.stream, G, F, P.tw.p, P.grid.p, P.half.p);
+  } else {
+    kt_begin(c, "k_fft_x_c2r");
+    hipLaunchKernelGGL(k_fft_x_c2r, grid, dim3(256), lds_x, c.stream, G, F, P.tw.p, P.half.p, P.grid.p);
+  }
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+void launch_fft_y(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, int sign) {
+  Pme& P = c.pme;
+  const int mblocks = (P.h1 + G.ly - 1) / G.ly;
+  const size_t lds_y = (2 * size_t(G.ly) * size_t(line_stride(P.k[1])) + P.k[1]) * sizeof(float2);
+  kt_begin(c, "k_fft_y");
+  hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, c.stream, G, F, P.tw.p, P.half.p, sign);
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+void launch_fft_z_conv(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool want) {
+  Pme& P = c.pme;
+  const int mblocks = (P.h1 + G.ly - 1) / G.ly;
+  const size_t lds_z = (2 * size_t(G.ly) * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2);
+  const unsigned zb = unsigned(mblocks * P.k[1]);
+  const double fac = kPi * kPi / (P.beta * P.beta);
+  kt_begin(c, "k_fft_z_conv");
+  if (want) {
+    P.epart.resize(7 * size_t(zb));
+    P.erows = int64_t(zb);
+    hipLaunchKernelGGL(k_fft_z_conv<true>, dim3(zb), dim3(256), lds_z, c.stream, G, F, P.tw.p, P.half.p, P.gtab.p, fac, kCoulomb, P.vol,
+                       P.bmod.p, P.epart.p);
+  } else {
+    hipLaunchKernelGGL(k_fft_z_conv<false>, dim3(zb), dim3(256), lds_z, c.stream, G, F, P.tw.p, P.half.p, P.gtab.p, fac, kCoulomb,
+                       P.vol, P.bmod.p, nullptr);
+  }
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+// mesh → half spectrum → × influence function → mesh, in place on P.grid
+void fft_forward_conv_inverse(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool want) {
+  launch_fft_x(c, G, F, true);
+  launch_fft_y(c, G, F, +1);
+  launch_fft_z_conv(c, G, F, want);
+  launch_fft_y(c, G, F, -1);
+  launch_fft_x(c, G, F, false);
+}
+}  // namespace
+
+// the transforms' tile shape of this context, picked by timing on its own mesh: the forward and inverse passes along a
+// with 2, 4, 8 or 16 line pairs per workgroup, then the passes along b and c (with the convolution) with 4, 8 or 16
+// lines, each the best of three runs timed with HIP events on the context's stream. The passes run in place on the
+// mesh of the last evaluation, which the next evaluation overwrites; every shape transforms each line with the same
+// arithmetic, so the choice changes the time and not the forces.
+void pme_tune(capsgpu_ctx& c, int out[2]) {
+  Pme& P = c.pme;
+  if (!P.green_ok) throw Error(-1, "tuning the mesh transforms needs one evaluation first");
+  const FftPlan F = plan_of(P);
+  const bool prof = c.profile;
+  c.profile = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto best_of = [&](auto&& run) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      HIPCHECK(hipEventRecord(e0, c.stream));
+      run();
+      HIPCHECK(hipEventRecord(e1, c.stream));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  try {
+    float tbest = 1e30f;
+    int pick = P.lines_x;
+    for (int lx : {2, 4, 8, 16}) {
+      P.lines_x = lx;
+      const GridArgs G = grid_args(P);
+      const float t = best_of([&] { launch_fft_x(c, G, F, true), launch_fft_x(c, G, F, false); });
+      if (t < tbest) tbest = t, pick = lx;
+    }
+    P.lines_x = pick;
+    tbest = 1e30f;
+    pick = P.lines_yz;
+    for (int ly : {4, 8, 16}) {
+      P.lines_yz = ly;
+      const GridArgs G = grid_args(P);
+      const float t = best_of([&] { launch_fft_y(c, G, F, +1), launch_fft_z_conv(c, G, F, false), launch_fft_y(c, G, F, -1); });
+      if (t < tbest) tbest = t, pick = ly;
+    }
+    P.lines_yz = pick;
+  } catch (...) {
+    c.profile = prof;
+    (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+    throw;
+  }
+  c.profile = prof;
+  (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+  out[0] = P.lines_x, out[1] = P.lines_yz;
+}
+
 void pme_forces(capsgpu_ctx& c, bool want) {
   Pme& P = c.pme;
   pme_setup(c);
@@ -946,42 +1065,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   }
   LAUNCHCHECK();
   kt_end(c);
-  const int npairs = (P.k[1] * P.k[2] + 1) / 2;
-  const size_t lds_x = (2 * kLinePairsX * size_t(line_stride(P.k[0])) + P.k[0]) * sizeof(float2);
-  kt_begin(c, "k_fft_x_r2c");
-  hipLaunchKernelGGL(k_fft_x_r2c, dim3(unsigned((npairs + kLinePairsX - 1) / kLinePairsX)), dim3(256), lds_x, s, G, F, P.tw.p, P.grid.p,
-                     P.half.p);
-  LAUNCHCHECK();
-  kt_end(c);
-  const int mblocks = (P.h1 + kLinesYZ - 1) / kLinesYZ;
-  const size_t lds_y = (2 * kLinesYZ * size_t(line_stride(P.k[1])) + P.k[1]) * sizeof(float2);
-  const size_t lds_z = (2 * kLinesYZ * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2);
-  kt_begin(c, "k_fft_y");
-  hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, +1);
-  LAUNCHCHECK();
-  kt_end(c);
-  const unsigned zb = unsigned(mblocks * P.k[1]);
-  const double fac = kPi * kPi / (P.beta * P.beta);
-  kt_begin(c, "k_fft_z_conv");
-  if (want) {
-    P.epart.resize(7 * size_t(zb));
-    hipLaunchKernelGGL(k_fft_z_conv<true>, dim3(zb), dim3(256), lds_z, s, G, F, P.tw.p, P.half.p, P.gtab.p, fac, kCoulomb, P.vol,
-                       P.bmod.p, P.epart.p);
-  } else {
-    hipLaunchKernelGGL(k_fft_z_conv<false>, dim3(zb), dim3(256), lds_z, s, G, F, P.tw.p, P.half.p, P.gtab.p, fac, kCoulomb, P.vol,
-                       P.bmod.p, nullptr);
-  }
-  LAUNCHCHECK();
-  kt_end(c);
-  kt_begin(c, "k_fft_y");
-  hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, s, G, F, P.tw.p, P.half.p, -1);
-  LAUNCHCHECK();
-  kt_end(c);
-  kt_begin(c, "k_fft_x_c2r");
-  hipLaunchKernelGGL(k_fft_x_c2r, dim3(unsigned((npairs + kLinePairsX - 1) / kLinePairsX)), dim3(256), lds_x, s, G, F, P.tw.p, P.half.p,
-                     P.grid.p);
-  LAUNCHCHECK();
-  kt_end(c);
+  fft_forward_conv_inverse(c, G, F, want);
   P.fpme.resize(3 * size_t(n));
   kt_begin(c, "k_gather");
   const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 4) + 8 * 6 * size_t(G.p)) * sizeof(float);

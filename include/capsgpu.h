@@ -153,13 +153,22 @@ typedef struct capsgpu_dyn_report {
 int32_t capsgpu_run_dynamics(capsgpu_ctx* ctx, const capsgpu_dyn_opts* opts, capsgpu_progress_fn progress, void* user,
                              capsgpu_thermo* rows, int32_t cap, int32_t* nrows, capsgpu_dyn_report* report);
 
+/* ---- kernel tile shapes (no counterpart in the reference: its kernels have fixed loop shapes) --------------------------
+ * The mesh transforms run a tile of lines per workgroup: line pairs along a (tiles[0]) and lines along b and c
+ * (tiles[1]), powers of two. capsgpu_tune picks them for this context by timing each candidate on its own mesh (after
+ * one reciprocal-space evaluation, run here if there was none; needs PME and a set state) and keeps the fastest;
+ * capsgpu_set_tiles sets them. Every shape gives bit-identical forces. tiles: the shape now in use (may be NULL). */
+int32_t capsgpu_tune(capsgpu_ctx* ctx, int32_t tiles[2]);
+int32_t capsgpu_set_tiles(capsgpu_ctx* ctx, int32_t lines_x, int32_t lines_yz, int32_t tiles[2]);
+
 /* ---- replicas stepped together ------------------------------------------------------------------------------------
  * The reference runs the replicas of a cooling scan one after another, each through run_dynamics
  * (core/src/mechanics.cpp:1381-1470 via caps::run_cooling, core/include/caps/mechanics.hpp:323-353); here many contexts
  * (each its own force field, cell, PME grid and temperature) take Langevin BAOAB steps together. A chunk of steps of
  * every replica is captured once into HIP graphs and replayed; the launch shape (how many streams the replicas are
  * spread over, each replaying its own graph, and how many steps one graph holds) is picked by timing the candidates on
- * the batch itself, from a snapshot of the state that is restored afterwards. Each replica ends bit-identical to
+ * the batch itself, from a snapshot of the state that is restored afterwards; in the tuned mode each replica's kernel
+ * tile shapes are first picked as capsgpu_tune does. Each replica ends bit-identical to
  * capsgpu_run_dynamics of that context alone with thermostat 2, seed + r and the same dt, tau_t and temperature.
  * The replicas' energy options must leave the pair path out (parts 1, 4 or 5): its list rebuilds need the host. */
 typedef struct capsgpu_batch capsgpu_batch;

@@ -87,3 +87,10 @@ def test_batch_arguments_are_rejected_without_a_device():
     assert b"twice" in L.capsgpu_last_error()
     with pytest.raises(ValueError, match="mode"):
         ca.Batch.run(object.__new__(ca.Batch), [], [], ca.BatchOptions(mode="fast"))
+
+
+def test_tile_entry_points_reject_bad_arguments_without_a_device():
+    L = ca.lib()
+    assert L.capsgpu_tune(None, None) == -1
+    assert b"null" in L.capsgpu_last_error()
+    assert L.capsgpu_set_tiles(None, 8, 16, None) == -1

@@ -62,6 +62,29 @@ def test_reciprocal_forces_follow_the_atoms():
     assert np.sqrt(((f1 - f0) ** 2).sum() / (f0 ** 2).sum()) < 1e-5
 
 
+@pytest.mark.parametrize("case,spacing", [("tri", 0.7), ("melt", 0.45)], ids=["35x32x35", "56^3"])
+def test_every_transform_tile_shape_gives_the_same_forces(case, spacing):
+    # the tile shape (lines per workgroup) only regroups the lines: forces bit-identical, energies to summation order;
+    # the tuner keeps one of the candidates and the forces stay the same
+    print("case transform tiles", case, flush=True)
+    s = H.triclinic_case(n=900) if case == "tri" else H.melt_case(chains=16)
+    op = ca.EnergyOptions(cutoff=9.0, coulomb=True, electrostatics="pme", pme_order=5, pme_spacing=spacing, tail=False, parts=4)
+    ev = ca.Evaluator(s.ff, op)
+    t0, f0 = ev.compute(s.x, s.cell)
+    for lx, ly in [(1, 1), (2, 4), (16, 8), (4, 16)]:
+        assert ev.set_tiles(lx, ly) == (lx, ly)
+        t, f = ev.compute(s.x, s.cell)
+        assert np.array_equal(f, f0), (lx, ly)
+        assert t.coulomb == pytest.approx(t0.coulomb, rel=1e-12, abs=1e-9)
+    lx, ly = ev.tune()
+    print("tuned tiles", lx, ly, flush=True)
+    assert lx in (2, 4, 8, 16) and ly in (4, 8, 16)
+    t, f = ev.compute(s.x, s.cell)
+    assert np.array_equal(f, f0)
+    with pytest.raises(ValueError, match="powers of two"):
+        ev.set_tiles(3, 8)
+
+
 @pytest.mark.parametrize("chains,order", [(256, 4), (2048, 5)], ids=["32k order 4 on 64^3", "262k order 5 on 128^3"])
 def test_reciprocal_part_at_full_size(chains, order):
     # the 32,768- and 262,144-atom melts of the benchmark workloads, against the oracle's FP64 PME (seconds on the CPU)
