@@ -111,8 +111,8 @@ struct Bonded {
   DevBuf<double> bkr, akt, fp;
   DevBuf<int32_t> aoff;                 // per atom: its slots (CSR)
   DevBuf<int32_t> aslot;                // a member's slot: bond t 2t + k, angle t 2nb + 3t + k, FENE t 2nb + 3na + 2t + k
-  DevBuf<double> slot;                  // 3 per slot: the term's force on that member
-  DevBuf<double> f;                     // 3 per atom
+  DevBuf<int32_t> spos;                 // per member slot: its place in the per-atom order (the CSR above)
+  DevBuf<double> slot;                  // 3 per place: the term's force on that member, by atom
   DevBuf<double> epart;                 // per block of terms: bond, angle, virial, w[6]
   int eblocks = 0;                      // rows of epart
 };

@@ -3,7 +3,8 @@
 //  k_bonded     one thread per term: the term's force on each member atom into that member's slot, energy and
 //               virial into per-block sums. Harmonic bond and angle (core/src/field.cpp:1980-2003), FENE + WCA
 //               (:2150-2180).
-//  k_bonded_sum per atom: its slots added in the order of its terms (no atomics, a fixed order)
+//  k_sum_forces per atom: the pair, reciprocal and bonded forces; its bonded slots (contiguous, in the order of its
+//               terms) added first (no atomics, a fixed order)
 //  k_sum_forces the total force per atom: pair forces (by slot, float) + reciprocal (float) + bonded (double)
 //  k_vv_open    velocity Verlet's opening half kick and drift (core/src/dynamics.cpp:405-431, 657-666), Bussi's
 //               scaling of the previous step folded in
@@ -58,14 +59,15 @@ __device__ inline void vir_add(double* A, const double* d, const double* f) {
 constexpr int kEB = 9;   // bond, angle, virial, w[6]
 
 // one thread per term (bonds, then angles, then FENE bonds): the term's force on each member atom into that member's
-// slot (bond t: slots 2t, 2t + 1; angle t: 2nb + 3t + 0…2; FENE t: 2nb + 3na + 2t + 0…1), energies and virial into
-// per-block sums. Every term is computed once; k_bonded_sum adds each atom's slots in a fixed order.
+// slot (bond t: members 2t, 2t + 1; angle t: 2nb + 3t + 0…2; FENE t: 2nb + 3na + 2t + 0…1), stored at the member's place
+// spos[member] in the per-atom order of slots, energies and virial into per-block sums. Every term is computed once;
+// k_sum_forces adds each atom's slots, which lie together, in a fixed order.
 template <bool kE>
 __global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs C, const double* __restrict__ x,
                                                 const int32_t* __restrict__ bij, const double* __restrict__ bkr,
                                                 const int32_t* __restrict__ aijk, const double* __restrict__ akt,
-                                                const int32_t* __restrict__ fij, const double* __restrict__ fp, double* __restrict__ slot,
-                                                double* __restrict__ epart, uint32_t* __restrict__ flags) {
+                                                const int32_t* __restrict__ fij, const double* __restrict__ fp, const int32_t* __restrict__ spos,
+                                                double* __restrict__ slot, double* __restrict__ epart, uint32_t* __restrict__ flags) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (g < nb || (g >= nb + na && g < nb + na + nf)) {
@@ -99,8 +101,10 @@ __global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs
     double fj[3] = {0, 0, 0};
     if (ok && r > 1e-12)
       for (int m = 0; m < 3; ++m) fj[m] = d[m] * (-dEdr / r);
-    double* o = slot + 3 * (fene ? 2 * size_t(nb) + 3 * size_t(na) + 2 * size_t(t) : 2 * size_t(t));
-    for (int m = 0; m < 3; ++m) o[m] = -fj[m], o[3 + m] = fj[m];
+    const size_t s0 = fene ? 2 * size_t(nb) + 3 * size_t(na) + 2 * size_t(t) : 2 * size_t(t);
+    double* o0 = slot + 3 * size_t(spos[s0]);
+    double* o1 = slot + 3 * size_t(spos[s0 + 1]);
+    for (int m = 0; m < 3; ++m) o0[m] = -fj[m], o1[m] = fj[m];
     if (kE && ok) {
       A[0] += en;
       vir_add(A, d, fj);
@@ -123,8 +127,11 @@ __global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs
       fi[m] = (v[m] * (1 / (lu * lv)) - u[m] * (cs / (lu * lu))) * gg;
       fk[m] = (u[m] * (1 / (lu * lv)) - v[m] * (cs / (lv * lv))) * gg;
     }
-    double* o = slot + 3 * (2 * size_t(nb) + 3 * size_t(t));
-    for (int m = 0; m < 3; ++m) o[m] = fi[m], o[3 + m] = -(fi[m] + fk[m]), o[6 + m] = fk[m];
+    const size_t s0 = 2 * size_t(nb) + 3 * size_t(t);
+    double* o0 = slot + 3 * size_t(spos[s0]);
+    double* o1 = slot + 3 * size_t(spos[s0 + 1]);
+    double* o2 = slot + 3 * size_t(spos[s0 + 2]);
+    for (int m = 0; m < 3; ++m) o0[m] = fi[m], o1[m] = -(fi[m] + fk[m]), o2[m] = fk[m];
     if (kE) {
       A[1] += kt * dth * dth;
       vir_add(A, u, fi);
@@ -144,36 +151,19 @@ __global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs
   }
 }
 
-// each atom's bonded force: its slots summed in the order of its terms
-__global__ void __launch_bounds__(256) k_bonded_sum(int n, const int32_t* __restrict__ aoff, const int32_t* __restrict__ aslot,
-                                                    const double* __restrict__ slot, double* __restrict__ fout) {
+// per atom: the pair forces (by slot of the cluster list), the reciprocal forces and the bonded forces — the atom's
+// bonded slots aoff[i] … aoff[i+1] (contiguous) summed first, in the order of its terms (aoff null: no bonded part)
+__global__ void __launch_bounds__(256) k_sum_forces(int n, const uint32_t* __restrict__ iperm, const float4* __restrict__ fnb,
+                                                    const float* __restrict__ frec, const int32_t* __restrict__ aoff,
+                                                    const double* __restrict__ slot, double* __restrict__ f) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double F[3] = {0, 0, 0};
-  const int32_t e0 = aoff[i], ne = aoff[i + 1] - e0;
-  // the first eight slots' indexes, then their forces, loaded together; more than eight in a loop
-  int32_t sl[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) sl[k] = k < ne ? aslot[e0 + k] : -1;
-  double v[8][3];
-#pragma unroll
-  for (int k = 0; k < 8; ++k)
-    for (int m = 0; m < 3; ++m) v[k][m] = sl[k] >= 0 ? slot[3 * size_t(sl[k]) + m] : 0.0;
-#pragma unroll
-  for (int k = 0; k < 8; ++k)
-    if (k < ne)
-      for (int m = 0; m < 3; ++m) F[m] += v[k][m];
-  for (int32_t e = e0 + 8; e < e0 + ne; ++e) {
-    const double* o = slot + 3 * size_t(aslot[e]);
-    for (int m = 0; m < 3; ++m) F[m] += o[m];
+  double Fb[3] = {0, 0, 0};
+  if (aoff) {
+    const int32_t e0 = aoff[i], e1 = aoff[i + 1];
+    for (int32_t e = e0; e < e1; ++e)
+      for (int m = 0; m < 3; ++m) Fb[m] += slot[3 * size_t(e) + m];
   }
-  for (int m = 0; m < 3; ++m) fout[3 * i + m] = F[m];
-}
-
-__global__ void k_sum_forces(int n, const uint32_t* __restrict__ iperm, const float4* __restrict__ fnb, const float* __restrict__ frec,
-                             const double* __restrict__ fb, double* __restrict__ f) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   double F[3] = {0, 0, 0};
   if (fnb) {
     const float4 p = fnb[iperm[i]];
@@ -181,8 +171,8 @@ __global__ void k_sum_forces(int n, const uint32_t* __restrict__ iperm, const fl
   }
   if (frec)
     for (int m = 0; m < 3; ++m) F[m] += double(frec[3 * i + m]);
-  if (fb)
-    for (int m = 0; m < 3; ++m) F[m] += fb[3 * i + m];
+  if (aoff)
+    for (int m = 0; m < 3; ++m) F[m] += Fb[m];
   for (int m = 0; m < 3; ++m) f[3 * i + m] = F[m];
 }
 
@@ -457,8 +447,12 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
     idx.insert(idx.end(), per[size_t(i)].begin(), per[size_t(i)].end());
   }
   if (idx.empty()) idx.push_back(0);
+  // each member slot's place in that per-atom order (k_bonded writes there; k_sum_forces reads an atom's run)
+  std::vector<int32_t> spos(2 * size_t(B.nb) + 3 * size_t(B.na) + 2 * size_t(B.nf) + 1, 0);
+  for (int32_t e = 0; e < off[size_t(n)]; ++e) spos[size_t(idx[size_t(e)])] = e;
   B.aoff.upload(off, s);
   B.aslot.upload(idx, s);
+  B.spos.upload(spos, s);
   B.slot.resize(3 * (2 * size_t(B.nb) + 3 * size_t(B.na) + 2 * size_t(B.nf)) + 3);
   auto up_i = [&](DevBuf<int32_t>& d, const int32_t* p, size_t m) {
     if (m) d.upload(p, m, s);
@@ -474,12 +468,11 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
   up_d(B.akt, ff.angle_param, 2 * size_t(B.na));
   up_i(B.fij, ff.fene_atoms, 2 * size_t(B.nf));
   up_d(B.fp, ff.fene_param, 4 * size_t(B.nf));
-  B.f.resize(3 * size_t(n));
 }
 
 void bonded_forces(capsgpu_ctx& c, bool want) {
   Bonded& B = c.bonded;
-  const int n = c.n, nt = B.nb + B.na + B.nf;
+  const int nt = B.nb + B.na + B.nf;
   const CellArgs C = cell_args(c);
   const unsigned tb = blocks_for(size_t(std::max(nt, 1)), 256);
   B.eblocks = int(tb);
@@ -487,15 +480,11 @@ void bonded_forces(capsgpu_ctx& c, bool want) {
   if (want) {
     B.epart.resize(kEB * size_t(tb));
     hipLaunchKernelGGL(k_bonded<true>, dim3(tb), dim3(256), 0, c.stream, B.nb, B.na, B.nf, C, c.x.p, B.bij.p, B.bkr.p, B.aijk.p, B.akt.p,
-                       B.fij.p, B.fp.p, B.slot.p, B.epart.p, c.flags.p);
+                       B.fij.p, B.fp.p, B.spos.p, B.slot.p, B.epart.p, c.flags.p);
   } else {
     hipLaunchKernelGGL(k_bonded<false>, dim3(tb), dim3(256), 0, c.stream, B.nb, B.na, B.nf, C, c.x.p, B.bij.p, B.bkr.p, B.aijk.p, B.akt.p,
-                       B.fij.p, B.fp.p, B.slot.p, nullptr, c.flags.p);
+                       B.fij.p, B.fp.p, B.spos.p, B.slot.p, nullptr, c.flags.p);
   }
-  LAUNCHCHECK();
-  kt_end(c);
-  kt_begin(c, "k_bonded_sum");
-  hipLaunchKernelGGL(k_bonded_sum, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, B.aoff.p, B.aslot.p, B.slot.p, B.f.p);
   LAUNCHCHECK();
   kt_end(c);
 }
@@ -503,7 +492,7 @@ void bonded_forces(capsgpu_ctx& c, bool want) {
 void sum_forces(capsgpu_ctx& c, bool nb, bool rec, bool bond, double* fdst) {
   kt_begin(c, "k_sum_forces");
   hipLaunchKernelGGL(k_sum_forces, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, c.L.iperm.p, nb ? c.L.fnb.p : nullptr,
-                     rec ? c.pme.fpme.p : nullptr, bond ? c.bonded.f.p : nullptr, fdst);
+                     rec ? c.pme.fpme.p : nullptr, bond ? c.bonded.aoff.p : nullptr, c.bonded.slot.p, fdst);
   LAUNCHCHECK();
   kt_end(c);
 }
