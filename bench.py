@@ -90,9 +90,9 @@ def cpu_baseline(system, budget_s=20.0):
 def replica_batch(ca, torch, steps, count=64, seed=SEED):
     """Many melts of the all-atom chemistry stepped together: `count` replicas with sizes drawn uniformly from 3,000 to
     40,000 atoms (whole 128-atom chains), each its own cubic box and PME grid (order 4, 1.0 A), Langevin on a 500 -> 200 K
-    ladder; PME reciprocal part + bonded terms (the pair path is held back). Timed three ways over the same steps: one
-    replica after another through run_dynamics (as a cooling scan runs them), the batch with direct launches, and the
-    batch with its tuned launch shape (graphs)."""
+    ladder; PME reciprocal part + bonded terms (the pair path is held back). Timed four ways over the same steps: one
+    replica after another through run_dynamics (as a cooling scan runs them), the batch with direct launches, the batch
+    as grouped launches (each kernel once for all replicas, 16 steps per graph), and the batch with its tuned shape."""
     import numpy as np
     rng = np.random.default_rng(seed)
     sizes = rng.integers(3000, 40001, count)
@@ -123,8 +123,8 @@ def replica_batch(ca, torch, steps, count=64, seed=SEED):
     secs = sequential(steps, False)
     out["one_after_another"] = {"seconds": secs, "atom_steps_per_s": atoms * steps / secs, "ns_per_day": steps * DT * 1e-6 * 86400 / secs}
     b = ca.Batch(evs)
-    for mode in ("direct", "tuned"):
-        o = ca.BatchOptions(dt=DT, steps=steps, tau_t=TAU, seed=seed, new_velocities=False, mode=mode)
+    for mode in ("direct", "grouped", "tuned"):
+        o = ca.BatchOptions(dt=DT, steps=steps, tau_t=TAU, seed=seed, new_velocities=False, mode=mode, graph_steps=16)
         r = b.run(systems, temps, o)
         out[mode] = {"seconds": r.seconds, "atom_steps_per_s": r.atom_steps_per_s, "ns_per_day": r.ns_per_day, "mode": r.mode,
                      "branches": r.branches, "graph_steps": r.graph_steps, "tune_seconds": r.tune_seconds,

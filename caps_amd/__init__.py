@@ -482,7 +482,8 @@ def run_dynamics(system, o, evaluator=None):
 class BatchOptions:
     """Langevin BAOAB steps of many replicas together (no reference counterpart: the reference runs a cooling scan's
     replicas one after another through run_dynamics, core/src/mechanics.cpp:1381-1470). mode: "tuned" (the launch shape
-    timed on the batch), "graph" (branches streams, graph_steps steps per captured graph) or "direct" (no graph)."""
+    timed on the batch), "graph" (branches streams, graph_steps steps per captured graph), "direct" (no graph) or
+    "grouped" (each kernel of the step launched once for all replicas, graph_steps steps per graph)."""
     dt: float = 1.0
     steps: int = 1000
     tau_t: float = 100.0
@@ -506,7 +507,7 @@ class BatchReport:
     temperatures: np.ndarray          # the kinetic temperature of each replica at the end
 
 
-_BATCH_MODES = {"tuned": 0, "graph": 1, "direct": 2}
+_BATCH_MODES = {"tuned": 0, "graph": 1, "direct": 2, "grouped": 3}
 
 
 class Batch:
@@ -553,7 +554,7 @@ class Batch:
         for ev, s in zip(self.evaluators, systems):
             x, v, _ = ev.get_state()
             s.x, s.v = x, v
-        mode = {1: "graph", 2: "direct"}.get(r.mode, "tuned")
+        mode = {1: "graph", 2: "direct", 3: "grouped"}.get(r.mode, "tuned")
         return BatchReport(r.steps, r.seconds, r.atom_steps_per_s, r.ns_per_day, mode, r.branches, r.graph_steps, r.tune_seconds, Tout)
 
 

@@ -50,7 +50,7 @@ struct Rep {
 };
 
 struct Shape {
-  int mode = 0;              // 1 graphs, 2 direct
+  int mode = 0;              // 1 graphs per stream, 2 direct, 3 graphs of grouped launches (one stream)
   int branches = 1, steps = 1;
   bool operator==(const Shape& o) const { return mode == o.mode && branches == o.branches && steps == o.steps; }
 };
@@ -68,10 +68,26 @@ struct capsgpu_batch {
   bool have_tuned = false;
   Shape tuned;
   std::vector<double> tuned_key;
+  // the grouped launches of this run (null when the replicas cannot share launches)
+  PmeGroup* pg = nullptr;
+  StepGroup* sg = nullptr;
   ~capsgpu_batch();
 };
 
 namespace {
+
+void drop_groups(capsgpu_batch& b) {
+  if (b.pg) pme_group_destroy(b.pg);
+  if (b.sg) step_group_destroy(b.sg);
+  b.pg = nullptr, b.sg = nullptr;
+}
+
+// one Langevin step of every replica as grouped launches on stream s (the noise for step *stepc + k)
+void grouped_step(capsgpu_batch& b, hipStream_t s, uint64_t k) {
+  step_group_open(*b.sg, s, k);
+  pme_group_forces(*b.pg, s);
+  step_group_close(*b.sg, s);
+}
 
 void drop_graphs(capsgpu_batch& b) {
   for (auto& e : b.exec)
@@ -119,9 +135,36 @@ void one_step(Rep& R, hipStream_t s, uint64_t k) {
   c.stream = own;
 }
 
-// the graphs of shape sh: branch j captures S steps of each of its replicas and the counters' advance
+// the graphs of shape sh: branch j captures S steps of each of its replicas and the counters' advance; in the grouped
+// shape one stream captures S steps of grouped launches and one advance of every counter
 void capture(capsgpu_batch& b, const Shape& sh) {
   drop_graphs(b);
+  if (sh.mode == 3) {
+    ensure_streams(b, 1);
+    b.exec.assign(1, nullptr);
+    hipStream_t s = b.streams[0];
+    HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      for (int k = 1; k <= sh.steps; ++k) grouped_step(b, s, uint64_t(k));
+      step_group_advance(*b.sg, s, uint64_t(sh.steps));
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      drop_graphs(b);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&b.exec[0], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      b.exec[0] = nullptr;
+      drop_graphs(b);
+      HIPCHECK(e);
+    }
+    b.captured = sh;
+    return;
+  }
   const auto groups = assign(b, sh.branches);
   ensure_streams(b, sh.branches);
   b.exec.assign(size_t(sh.branches), nullptr);
@@ -169,14 +212,21 @@ void advance(capsgpu_batch& b, const Shape& sh, int64_t steps) {
       }
   } else {
     const int64_t reps = steps / sh.steps, rest = steps % sh.steps;
+    const int nb = sh.mode == 3 ? 1 : sh.branches;
     for (int64_t q = 0; q < reps; ++q)
-      for (int j = 0; j < sh.branches; ++j) HIPCHECK(hipGraphLaunch(b.exec[size_t(j)], b.streams[size_t(j)]));
+      for (int j = 0; j < nb; ++j) HIPCHECK(hipGraphLaunch(b.exec[size_t(j)], b.streams[size_t(j)]));
     HIPCHECK(hipDeviceSynchronize());
-    for (int64_t q = 0; q < rest; ++q)
+    for (int64_t q = 0; q < rest; ++q) {
+      if (sh.mode == 3) {
+        grouped_step(b, b.streams[0], 1);
+        step_group_advance(*b.sg, b.streams[0], 1);
+        continue;
+      }
       for (Rep& R : b.r) {
         one_step(R, R.c->stream, 1);
         step_advance(*R.c, R.stepc.p, 1);
       }
+    }
   }
   HIPCHECK(hipDeviceSynchronize());
 }
@@ -215,12 +265,14 @@ Shape tune(capsgpu_batch& b) {
     if (B <= n)
       for (int S : {4, 16, 64})
         if (int64_t(S) * ((n + B - 1) / B) <= 1024) cand.push_back({1, B, S});
+  if (b.pg)
+    for (int S : {4, 16, 64}) cand.push_back({3, 1, S});
   snapshot(b, false);
   Shape best;
   double best_t = 1e300;
   for (const Shape& sh : cand) {
     const int64_t chunk = sh.mode == 2 ? 1 : sh.steps, timed = std::max<int64_t>(16, chunk);
-    if (sh.mode == 1) capture(b, sh);
+    if (sh.mode == 1 || sh.mode == 3) capture(b, sh);
     advance(b, sh, chunk);
     const auto t0 = std::chrono::steady_clock::now();
     advance(b, sh, timed);
@@ -236,6 +288,7 @@ Shape tune(capsgpu_batch& b) {
 capsgpu_batch::~capsgpu_batch() {
   (void)hipDeviceSynchronize();
   drop_graphs(*this);
+  drop_groups(*this);
   for (hipStream_t s : streams) (void)hipStreamDestroy(s);
 }
 
@@ -262,7 +315,9 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
   if (!(o.dt > 0) || o.dt > 5) throw Error(CAPSGPU_EINVAL, "time step must be in (0, 5] fs");
   if (o.steps < 0) throw Error(CAPSGPU_EINVAL, "the number of steps must be 0 or more");
   if (!(o.tau_t > 0)) throw Error(CAPSGPU_EINVAL, "the Langevin time constant must be positive");
-  if (o.mode < 0 || o.mode > 2) throw Error(CAPSGPU_EINVAL, "mode must be 0 (tuned), 1 (graphs) or 2 (direct launches)");
+  if (o.mode < 0 || o.mode > 3)
+    throw Error(CAPSGPU_EINVAL, "mode must be 0 (tuned), 1 (graphs), 2 (direct launches) or 3 (grouped launches)");
+  if (o.mode == 3 && (o.graph_steps < 1 || o.graph_steps > 256)) throw Error(CAPSGPU_EINVAL, "grouped launches need 1 … 256 steps per graph");
   if (o.mode == 1 && (o.branches < 1 || o.branches > n || o.graph_steps < 1 || o.graph_steps > 256))
     throw Error(CAPSGPU_EINVAL, "graphs need 1 … replicas branches and 1 … 256 steps per graph");
   for (int i = 0; i < n; ++i) {
@@ -306,29 +361,58 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
   }
   HIPCHECK(hipDeviceSynchronize());
   // the launch shape
-  Shape sh;
-  double tune_s = 0;
-  if (o.mode == 2) {
-    sh = {2, 0, 1};
-  } else if (o.mode == 1) {
-    sh = {1, o.branches, o.graph_steps};
-  } else if (b.have_tuned && b.tuned_key == key) {
-    sh = b.tuned;
-  } else {
+  // tuned mode, first run with these constants: each replica's transform tiles (on its mesh of the first evaluation,
+  // which the next evaluation overwrites), before the group tables take them and the launch shape is timed
+  const bool retune = o.mode == 0 && !(b.have_tuned && b.tuned_key == key);
+  double tile_s = 0;
+  if (retune) {
     const auto t0 = std::chrono::steady_clock::now();
-    // each replica's transform tiles (its mesh is the first evaluation's; the next evaluation overwrites it), then the
-    // launch shape
     for (Rep& R : b.r)
       if (pme_on_ctx(*R.c)) {
         int t[2];
         pme_tune(*R.c, t);
       }
     HIPCHECK(hipDeviceSynchronize());
+    tile_s = seconds_since(t0);
+  }
+  // grouped launches when every replica has PME's reciprocal part with one spline order
+  drop_groups(b);
+  {
+    bool ok = true;
+    for (Rep& R : b.r) ok = ok && pme_on_ctx(*R.c) && R.c->pme.counts_clear && R.c->pme.order == b.r[0].c->pme.order;
+    if (ok) {
+      std::vector<capsgpu_ctx*> cs;
+      std::vector<StepGroupInput> in;
+      for (Rep& R : b.r) {
+        capsgpu_ctx& c = *R.c;
+        cs.push_back(&c);
+        in.push_back({&c, R.hk, R.hdt, R.c1, R.c2, R.seed, R.stepc.p, R.sdv.p, R.bpart.p, R.kpart.p, (c.opt.parts & 1) != 0, true});
+      }
+      ensure_streams(b, 1);
+      b.pg = pme_group_create(cs, b.streams[0]);
+      b.sg = step_group_create(in, b.streams[0]);
+      HIPCHECK(hipStreamSynchronize(b.streams[0]));
+    } else if (o.mode == 3) {
+      throw Error(CAPSGPU_EINVAL, "grouped launches need PME's reciprocal part with one spline order in every replica");
+    }
+  }
+  Shape sh;
+  double tune_s = 0;
+  if (o.mode == 2) {
+    sh = {2, 0, 1};
+  } else if (o.mode == 1) {
+    sh = {1, o.branches, o.graph_steps};
+  } else if (o.mode == 3) {
+    sh = {3, 1, o.graph_steps};
+  } else if (b.have_tuned && b.tuned_key == key) {
+    sh = b.tuned;
+  } else {
+    const auto t0 = std::chrono::steady_clock::now();
     sh = tune(b);
-    tune_s = seconds_since(t0);
+    tune_s = seconds_since(t0) + tile_s;
     b.have_tuned = true, b.tuned = sh, b.tuned_key = key;
   }
-  if (sh.mode == 1) capture(b, sh);
+  if (sh.mode == 1 || sh.mode == 3) capture(b, sh);
   const auto t0 = std::chrono::steady_clock::now();
   advance(b, sh, o.steps);
   const double secs = seconds_since(t0);
@@ -354,8 +438,8 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
     rep->atom_steps_per_s = secs > 0 ? double(atoms) * double(o.steps) / secs : 0;
     rep->ns_per_day = secs > 0 ? double(o.steps) * o.dt * 1e-6 / secs * 86400 : 0;
     rep->mode = sh.mode;
-    rep->branches = sh.mode == 1 ? sh.branches : n;
-    rep->graph_steps = sh.mode == 1 ? sh.steps : 0;
+    rep->branches = sh.mode == 1 ? sh.branches : sh.mode == 3 ? 1 : n;
+    rep->graph_steps = sh.mode == 1 || sh.mode == 3 ? sh.steps : 0;
     rep->tune_seconds = tune_s;
   }
 }

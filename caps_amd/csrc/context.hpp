@@ -188,6 +188,11 @@ void pme_setup(capsgpu_ctx& c);
 void pme_forces(capsgpu_ctx& c, bool want);
 // picks the transforms' tile shape by timing (out: line pairs along a, lines along b and c per workgroup)
 void pme_tune(capsgpu_ctx& c, int out[2]);
+// many contexts' reciprocal forces, one launch per kernel (each context evaluated once before; one spline order)
+struct PmeGroup;
+PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s);
+void pme_group_forces(PmeGroup& g, hipStream_t s);
+void pme_group_destroy(PmeGroup* g);
 // bonded.hip
 void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff);
 void bonded_forces(capsgpu_ctx& c, bool want);

@@ -63,9 +63,9 @@ struct GridArgs {
 };
 
 // fractional coordinate s = h⁻¹ x, u = (s − ⌊s⌋) K, the first point ⌊u⌋ − p + 1 (mod K) and w = u − ⌊u⌋
-__global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
+__device__ __forceinline__ void b_splines(int bx, int gx, GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
                           int4* __restrict__ base, uint32_t* __restrict__ tcount) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float w[3];
   int b[3];
@@ -83,25 +83,33 @@ __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const
   base[i] = make_int4(b[0], b[1], b[2], tile);
   atomicAdd(&tcount[tile], 1u);
 }
+__global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
+                          int4* __restrict__ base, uint32_t* __restrict__ tcount) {
+  b_splines(blockIdx.x, gridDim.x, G, n, x, q, wq, base, tcount);
+}
 
-__global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
+__device__ __forceinline__ void b_tile_fill(int bx, int gx, int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
                             uint32_t* __restrict__ tmp) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int t = base[i].w;
   tmp[tstart[t] + atomicAdd(&tfill[t], 1u)] = uint32_t(i);
+}
+__global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
+                            uint32_t* __restrict__ tmp) {
+  b_tile_fill(blockIdx.x, gridDim.x, n, base, tstart, tfill, tmp);
 }
 
 // the atoms of each tile in index order: an atom's place is its rank among its tile's atoms (filled unordered in tmp);
 // its spline offsets, charge and base point are copied to that place too, so the spread and the gather read a tile's
 // atoms contiguously. The tile counters (read by the scan and by k_tile_fill, both done) are cleared here for the next
 // evaluation.
-__global__ void k_tile_rank(int n, int ntiles, const float4* __restrict__ wq, const int4* __restrict__ base,
+__device__ __forceinline__ void b_tile_rank(int bx, int gx, int n, int ntiles, const float4* __restrict__ wq, const int4* __restrict__ base,
                             const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tmp, uint32_t* __restrict__ tatoms,
                             float4* __restrict__ wsort, int4* __restrict__ bsort, uint32_t* __restrict__ tcount,
                             uint32_t* __restrict__ tfill) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int t = i; t < ntiles; t += gridDim.x * blockDim.x) tcount[t] = 0u, tfill[t] = 0u;
+  const int i = bx * blockDim.x + threadIdx.x;
+  for (int t = i; t < ntiles; t += gx * blockDim.x) tcount[t] = 0u, tfill[t] = 0u;
   if (i >= n) return;
   const int4 bi = base[i];
   const float4 wi = wq[i];
@@ -111,6 +119,12 @@ __global__ void k_tile_rank(int n, int ntiles, const float4* __restrict__ wq, co
   tatoms[b + r] = uint32_t(i);
   wsort[b + r] = wi;
   bsort[b + r] = bi;
+}
+__global__ void k_tile_rank(int n, int ntiles, const float4* __restrict__ wq, const int4* __restrict__ base,
+                            const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tmp, uint32_t* __restrict__ tatoms,
+                            float4* __restrict__ wsort, int4* __restrict__ bsort, uint32_t* __restrict__ tcount,
+                            uint32_t* __restrict__ tfill) {
+  b_tile_rank(blockIdx.x, gridDim.x, n, ntiles, wq, base, tstart, tmp, tatoms, wsort, bsort, tcount, tfill);
 }
 
 // one 64-lane workgroup per 4x4x4 tile. The atoms whose first point lies in a tile that can reach this one (the home
@@ -122,15 +136,13 @@ __global__ void k_tile_rank(int n, int ntiles, const float4* __restrict__ wq, co
 // Dynamic LDS: kSpreadChunk · 2p floats (small enough for eight workgroups per SIMD).
 constexpr int kSpreadChunk = 64, kMaxHome = 64;   // at most 4 home tiles per axis reach a tile (p <= 8, partial last tile)
 template <int P>
-__global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
+__device__ __forceinline__ void b_spread(int tile, GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
                                                const float4* __restrict__ wsort, const int4* __restrict__ bsort, float* __restrict__ grid) {
   extern __shared__ float2 lds[];
   __shared__ float4 s_wx[kSpreadChunk];   // θa[r_a + i] for the tile's four points i along a (zero outside the support)
   __shared__ int4 s_ent[kSpreadChunk];    // {2p j + r_b + 3, 2p j + p + r_c + 3, the rows (j, k) of the tile it reaches, 0}
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
-  const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
-  if (tile >= ntiles) return;
   constexpr int p = P;
   float* s_bc = reinterpret_cast<float*>(lds);          // [chunk][2p]: θb, then q θc
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
@@ -235,12 +247,21 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
   const int gx = 4 * tx + (lane & 3), gy_ = 4 * ty + ((lane >> 2) & 3), gz_ = 4 * tz + (lane >> 4);
   if (gx < G.k[0] && gy_ < G.k[1] && gz_ < G.k[2]) grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
 }
+template <int P>
+__global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
+                                               const float4* __restrict__ wsort, const int4* __restrict__ bsort, float* __restrict__ grid) {
+  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+  const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
+  if (tile >= ntiles) return;
+  b_spread<P>(tile, G, home, tstart, wsort, bsort, grid);
+}
 
 // ---- FFT ------------------------------------------------------------------------------------------------------------
 struct Fft1 {
   int n, nr;
-  int rad[12];
-  int tw;   // offset of this axis's twiddles in the table
+  uint64_t rad;   // the stages' radices, 4 bits each from the lowest (no array: a plan copied into a kernel body stays in
+                  // registers rather than private memory)
+  int tw;         // offset of this axis's twiddles in the table
 };
 struct FftPlan {
   Fft1 ax[3];
@@ -253,7 +274,8 @@ Fft1 make_fft(int n, int twoff) {
   for (const int r : {8, 4, 2, 3, 5, 7}) {
     while (m % r == 0) {
       if (r == 8 && m % 8 != 0) break;
-      f.rad[f.nr++] = r;
+      if (f.nr == 12) throw Error(-1, "PME grid sizes need at most 12 transform stages");
+      f.rad |= uint64_t(r) << (4 * f.nr++);
       m /= r;
     }
   }
@@ -384,7 +406,7 @@ __device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const 
   const int n = F.n, ls = line_stride(n);
   int p = 1;
   for (int s = 0; s < F.nr; ++s) {
-    const int R = F.rad[s];
+    const int R = int((F.rad >> (4 * s)) & 15u);
     switch (R) {
       case 2: fft_stage<2>(src, dst, nl, n, ls, p, twl, sign); break;
       case 3: fft_stage<3>(src, dst, nl, n, ls, p, twl, sign); break;
@@ -406,7 +428,7 @@ __device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const 
 constexpr int kFftThreads = 256, kFftBatch = 8;
 
 // real lines along a (index j + K2·k, pairs 2l, 2l+1) -> the half spectrum, as caps_f_r2c_lines (f90:392-423)
-__global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
+__device__ __forceinline__ void b_fft_x_r2c(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
                                                    float2* __restrict__ half) {
   extern __shared__ float2 lds[];
   const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
@@ -415,7 +437,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const
   float2* B = lds + LP * ls;
   float2* twl = B + LP * ls;
   const TwFetch twf = fetch_twiddles(tw, FP.ax[0]);
-  const int pb = blockIdx.x * LP;
+  const int pb = bx * LP;
   for (int t0 = 0; t0 < LP * K1; t0 += kFftBatch * kFftThreads) {
     float2 r[kFftBatch];
 #pragma unroll
@@ -450,9 +472,13 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const
     }
   }
 }
+__global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
+                                                   float2* __restrict__ half) {
+  b_fft_x_r2c(blockIdx.x, gridDim.x, G, FP, tw, grid, half);
+}
 
 // the inverse: half spectra of two real lines rebuilt whole, Z = A + iB, one inverse transform (f90:427-461)
-__global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
+__device__ __forceinline__ void b_fft_x_c2r(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
                                                    float* __restrict__ grid) {
   extern __shared__ float2 lds[];
   const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
@@ -461,7 +487,7 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
   float2* B = lds + LP * ls;
   float2* twl = B + LP * ls;
   const TwFetch twf = fetch_twiddles(tw, FP.ax[0]);
-  const int pb = blockIdx.x * LP;
+  const int pb = bx * LP;
   for (int t0 = 0; t0 < LP * K1; t0 += kFftBatch * kFftThreads) {
     float2 ra[kFftBatch], rb[kFftBatch];
 #pragma unroll
@@ -497,14 +523,18 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
     if (lb != la) grid[size_t(lb) * K1 + m] = z.y;
   }
 }
+__global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
+                                                   float* __restrict__ grid) {
+  b_fft_x_c2r(blockIdx.x, gridDim.x, G, FP, tw, half, grid);
+}
 
 // lines along b of the half spectrum: element (m, j, k) at m + h1 (j + K2 k)
-__global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half, int sign) {
+__device__ __forceinline__ void b_fft_y(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half, int sign) {
   extern __shared__ float2 lds[];
   const int h1 = G.h1, K2 = G.k[1];
   const int LY = G.ly, lg = G.ly_log2;   // lines (consecutive a-frequencies) per workgroup
   const int mblocks = (h1 + LY - 1) >> lg;
-  const int kz = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) << lg;
+  const int kz = bx / mblocks, m0 = (bx % mblocks) << lg;
   const int nl = min(LY, h1 - m0), ls = line_stride(K2);
   float2* A = lds;
   float2* B = lds + LY * ls;
@@ -530,6 +560,9 @@ __global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const flo
     const int j = t >> lg, li = t & (LY - 1);
     if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * ls + lds_at(j)];
   }
+}
+__global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half, int sign) {
+  b_fft_y(blockIdx.x, gridDim.x, G, FP, tw, half, sign);
 }
 
 // the influence function and its virial weights at half-spectrum index (a, b, c), in double (f90:518-564)
@@ -585,14 +618,14 @@ __global__ void k_green(GridArgs G, double fac, double kcoul, double vol, const 
 
 // lines along c: forward transform, × G (energy and virial when kE), inverse transform
 template <bool kE>
-__global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half,
+__device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half,
                                                     const float* __restrict__ gtab, double fac, double kcoul, double vol,
                                                     const double* __restrict__ bm, double* __restrict__ epart) {
   extern __shared__ float2 lds[];
   const int h1 = G.h1, K2 = G.k[1], K3 = G.k[2];
   const int LY = G.ly, lg = G.ly_log2;
   const int mblocks = (h1 + LY - 1) >> lg;
-  const int jy = blockIdx.x / mblocks, m0 = (blockIdx.x % mblocks) << lg;
+  const int jy = bx / mblocks, m0 = (bx % mblocks) << lg;
   const int nl = min(LY, h1 - m0), ls = line_stride(K3);
   float2* A = lds;
   float2* B = lds + LY * ls;
@@ -660,8 +693,14 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
         for (int q = 0; q < 7; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x < 7) epart[7 * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 7) epart[7 * size_t(bx) + threadIdx.x] = red[threadIdx.x][0];
   }
+}
+template <bool kE>
+__global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half,
+                                                    const float* __restrict__ gtab, double fac, double kcoul, double vol,
+                                                    const double* __restrict__ bm, double* __restrict__ epart) {
+  b_fft_z_conv<kE>(blockIdx.x, gridDim.x, G, FP, tw, half, gtab, fac, kcoul, vol, bm, epart);
 }
 
 // one 64-lane workgroup per 4x4x4 tile: the tile's box of (p + 3)³ mesh points in LDS (loaded row by row, wrapped),
@@ -669,14 +708,12 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
 // lanes' sums are added in a fixed order (butterfly) and the first lane writes the atom's force. Dynamic LDS: the box
 // (p + 3)² rows of p + 4 floats, then per atom slot 6p weights.
 template <int P>
-__global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
+__device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
                                                const float4* __restrict__ wsort, const int4* __restrict__ bsort,
                                                const float* __restrict__ grid, float* __restrict__ f) {
   extern __shared__ float2 lds[];
   const int lane = threadIdx.x;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
-  const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
-  if (tile >= ntiles) return;
   const int64_t a0 = tstart[tile], na = tstart[tile + 1] - a0;
   if (na == 0) return;
   constexpr int p = P, W = P + 3, WX = W + 1;          // rows padded to W + 1 floats: the eight rows an atom's lanes
@@ -750,6 +787,15 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
       }
     __syncthreads();
   }
+}
+template <int P>
+__global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
+                                               const float4* __restrict__ wsort, const int4* __restrict__ bsort,
+                                               const float* __restrict__ grid, float* __restrict__ f) {
+  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+  const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
+  if (tile >= ntiles) return;
+  b_gather<P>(tile, G, tstart, tatoms, wsort, bsort, grid, f);
 }
 
 GridArgs grid_args(const Pme& P) {
@@ -1077,6 +1123,218 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   }
   LAUNCHCHECK();
   kt_end(c);
+}
+
+
+// ---- many contexts in one launch per kernel (the replica batch) ----------------------------------------------------
+// Each grouped kernel covers every replica's workgroups: a workgroup finds its replica in a prefix array of workgroup
+// counts (binary search, uniform across the workgroup) and runs that replica's body with its own arguments from a
+// table in device memory; the tile kernels map the whole range of tiles to XCDs first, so a replica's neighbouring
+// tiles stay together. The arithmetic per replica is the single-context kernels', so results are bit-identical.
+namespace {
+struct PmeRep {
+  GridArgs G;
+  FftPlan F;
+  int n, ntiles;
+  const double* x;
+  const double* q;
+  float4* wq;
+  int4* base;
+  uint32_t *tcount, *tfill, *tile, *tatoms;
+  int64_t* tstart;
+  float4* wsort;
+  int4* bsort;
+  const int* home;
+  float* grid;
+  float2* half;
+  const float2* tw;
+  const float* gtab;
+  const double* bmod;
+  double fac, vol;
+  float* fpme;
+};
+
+// the largest r with off[r] ≤ b (off[0] = 0, off[nr] = the launch's workgroups)
+__device__ inline int find_rep(const int* __restrict__ off, int nr, int b) {
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) kg_splines(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const PmeRep& a = R[r];
+  b_splines(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.n, a.x, a.q, a.wq, a.base, a.tcount);
+}
+
+// one workgroup of 1024 threads per replica: its tile counts' exclusive sums (tstart[ntiles] = the total); thread t
+// takes a run of consecutive tiles, the runs' sums are scanned across the workgroup's 16 waves
+__global__ void __launch_bounds__(1024) kg_tile_scan(const PmeRep* __restrict__ R) {
+  __shared__ int64_t wsum[16];
+  const PmeRep& a = R[blockIdx.x];
+  const int n = a.ntiles, per = (n + 1 + 1023) / 1024, b0 = int(threadIdx.x) * per;
+  int64_t s = 0;
+  for (int k = 0; k < per; ++k) s += b0 + k < n ? int64_t(a.tcount[b0 + k]) : 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int64_t v = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  if (lane == 63) wsum[wave] = v;
+  __syncthreads();
+  int64_t run = v - s;
+  for (int w = 0; w < wave; ++w) run += wsum[w];
+  for (int k = 0; k < per && b0 + k <= n; ++k) {
+    a.tstart[b0 + k] = run;
+    run += b0 + k < n ? int64_t(a.tcount[b0 + k]) : 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) kg_tile_fill(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const PmeRep& a = R[r];
+  b_tile_fill(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.base, a.tstart, a.tfill, a.tile);
+}
+
+__global__ void __launch_bounds__(256) kg_tile_rank(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const PmeRep& a = R[r];
+  b_tile_rank(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.ntiles, a.wq, a.base, a.tstart, a.tile, a.tatoms, a.wsort, a.bsort,
+              a.tcount, a.tfill);
+}
+
+template <int P>
+__global__ void __launch_bounds__(64) kg_spread(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int g = int(xcd_block(blockIdx.x, off[nr]));
+  const int r = find_rep(off, nr, g);
+  const PmeRep& a = R[r];
+  b_spread<P>(g - off[r], a.G, a.home, a.tstart, a.wsort, a.bsort, a.grid);
+}
+
+template <int P>
+__global__ void __launch_bounds__(64) kg_gather(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int g = int(xcd_block(blockIdx.x, off[nr]));
+  const int r = find_rep(off, nr, g);
+  const PmeRep& a = R[r];
+  b_gather<P>(g - off[r], a.G, a.tstart, a.tatoms, a.wsort, a.bsort, a.grid, a.fpme);
+}
+
+__global__ void __launch_bounds__(256) kg_fft_x(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr, int forward) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const PmeRep& a = R[r];
+  if (forward) b_fft_x_r2c(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.F, a.tw, a.grid, a.half);
+  else b_fft_x_c2r(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.F, a.tw, a.half, a.grid);
+}
+
+__global__ void __launch_bounds__(256) kg_fft_y(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr, int sign) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const PmeRep& a = R[r];
+  b_fft_y(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.F, a.tw, a.half, sign);
+}
+
+__global__ void __launch_bounds__(256) kg_fft_z_conv(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const PmeRep& a = R[r];
+  b_fft_z_conv<false>(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.F, a.tw, a.half, a.gtab, a.fac, kCoulomb, a.vol, a.bmod,
+                      nullptr);
+}
+}  // namespace
+
+struct PmeGroup {
+  int nr = 0, order = 0;
+  DevBuf<unsigned char> reps;                         // PmeRep[nr]
+  DevBuf<int32_t> off_atoms, off_tiles, off_x, off_yz, off_z;   // workgroup prefixes per launch
+  int h_atoms = 0, h_tiles = 0, h_x = 0, h_yz = 0, h_z = 0;       // the launches' workgroups
+  size_t lds_x = 0, lds_y = 0, lds_z = 0;
+};
+
+PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
+  auto g = std::make_unique<PmeGroup>();
+  g->nr = int(cs.size());
+  if (g->nr < 1) throw Error(-1, "a PME group needs a replica");
+  std::vector<PmeRep> R(cs.size());
+  std::vector<int32_t> oa(1, 0), ot(1, 0), ox(1, 0), oy(1, 0), oz(1, 0);
+  for (size_t i = 0; i < cs.size(); ++i) {
+    capsgpu_ctx& c = *cs[i];
+    Pme& P = c.pme;
+    if (!P.green_ok || !P.counts_clear) throw Error(-1, "a PME group needs each replica evaluated once");
+    if (i == 0) g->order = P.order;
+    if (P.order != g->order) throw Error(-1, "a PME group needs one spline order");
+    const GridArgs G = grid_args(P);
+    PmeRep& a = R[i];
+    a.G = G;
+    a.F = plan_of(P);
+    a.n = c.n;
+    a.ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+    a.x = c.x.p, a.q = c.d_charge.p;
+    a.wq = reinterpret_cast<float4*>(P.theta.p), a.base = reinterpret_cast<int4*>(P.base.p);
+    a.tcount = P.tcount.p, a.tfill = P.tfill.p, a.tile = P.tile.p, a.tatoms = P.tatoms.p, a.tstart = P.tstart.p;
+    a.wsort = reinterpret_cast<float4*>(P.wsort.p), a.bsort = reinterpret_cast<int4*>(P.bsort.p);
+    a.home = P.home.p, a.grid = P.grid.p, a.half = P.half.p, a.tw = P.tw.p, a.gtab = P.gtab.p, a.bmod = P.bmod.p;
+    a.fac = kPi * kPi / (P.beta * P.beta), a.vol = P.vol, a.fpme = P.fpme.p;
+    const int npairs = (P.k[1] * P.k[2] + 1) / 2, mblocks = (P.h1 + G.ly - 1) / G.ly;
+    oa.push_back(oa.back() + int(blocks_for(size_t(c.n), 256)));
+    ot.push_back(ot.back() + a.ntiles);
+    ox.push_back(ox.back() + (npairs + G.lx - 1) / G.lx);
+    oy.push_back(oy.back() + mblocks * P.k[2]);
+    oz.push_back(oz.back() + mblocks * P.k[1]);
+    g->lds_x = std::max(g->lds_x, (2 * size_t(G.lx) * size_t(line_stride(P.k[0])) + P.k[0]) * sizeof(float2));
+    g->lds_y = std::max(g->lds_y, (2 * size_t(G.ly) * size_t(line_stride(P.k[1])) + P.k[1]) * sizeof(float2));
+    g->lds_z = std::max(g->lds_z, (2 * size_t(G.ly) * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2));
+  }
+  g->reps.upload(reinterpret_cast<const unsigned char*>(R.data()), R.size() * sizeof(PmeRep), s);
+  g->off_atoms.upload(oa, s), g->off_tiles.upload(ot, s), g->off_x.upload(ox, s), g->off_yz.upload(oy, s), g->off_z.upload(oz, s);
+  g->h_atoms = oa.back(), g->h_tiles = ot.back(), g->h_x = ox.back(), g->h_yz = oy.back(), g->h_z = oz.back();
+  return g.release();
+}
+
+void pme_group_destroy(PmeGroup* g) { delete g; }
+
+// the reciprocal forces of every replica into its pme.fpme: eleven launches for the whole group
+void pme_group_forces(PmeGroup& g, hipStream_t s) {
+  const PmeRep* R = reinterpret_cast<const PmeRep*>(g.reps.p);
+  const int nr = g.nr;
+  const int na = g.h_atoms, nt = g.h_tiles, nx = g.h_x, ny = g.h_yz, nz = g.h_z;
+  hipLaunchKernelGGL(kg_splines, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_tile_scan, dim3(unsigned(nr)), dim3(1024), 0, s, R);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_tile_fill, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_tile_rank, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
+  LAUNCHCHECK();
+  const size_t lds_spread = size_t(kSpreadChunk) * size_t(2 * g.order) * sizeof(float);
+  const size_t lds_gather = (size_t(g.order + 3) * (g.order + 3) * (g.order + 4) + 8 * 6 * size_t(g.order)) * sizeof(float);
+  switch (g.order) {
+#define CAPSGPU_GSPREAD(Q) \
+  case Q: hipLaunchKernelGGL(kg_spread<Q>, dim3(unsigned(nt)), dim3(64), lds_spread, s, R, g.off_tiles.p, nr); break;
+    CAPSGPU_GSPREAD(3) CAPSGPU_GSPREAD(4) CAPSGPU_GSPREAD(5) CAPSGPU_GSPREAD(6) CAPSGPU_GSPREAD(7) CAPSGPU_GSPREAD(8)
+#undef CAPSGPU_GSPREAD
+  }
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_fft_x, dim3(unsigned(nx)), dim3(256), g.lds_x, s, R, g.off_x.p, nr, 1);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_fft_y, dim3(unsigned(ny)), dim3(256), g.lds_y, s, R, g.off_yz.p, nr, +1);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_fft_z_conv, dim3(unsigned(nz)), dim3(256), g.lds_z, s, R, g.off_z.p, nr);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_fft_y, dim3(unsigned(ny)), dim3(256), g.lds_y, s, R, g.off_yz.p, nr, -1);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_fft_x, dim3(unsigned(nx)), dim3(256), g.lds_x, s, R, g.off_x.p, nr, 0);
+  LAUNCHCHECK();
+  switch (g.order) {
+#define CAPSGPU_GGATHER(Q) \
+  case Q: hipLaunchKernelGGL(kg_gather<Q>, dim3(unsigned(nt)), dim3(64), lds_gather, s, R, g.off_tiles.p, nr); break;
+    CAPSGPU_GGATHER(3) CAPSGPU_GGATHER(4) CAPSGPU_GGATHER(5) CAPSGPU_GGATHER(6) CAPSGPU_GGATHER(7) CAPSGPU_GGATHER(8)
+#undef CAPSGPU_GGATHER
+  }
+  LAUNCHCHECK();
 }
 
 }  // namespace capsgpu

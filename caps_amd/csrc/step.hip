@@ -63,12 +63,15 @@ constexpr int kEB = 9;   // bond, angle, virial, w[6]
 // spos[member] in the per-atom order of slots, energies and virial into per-block sums. Every term is computed once;
 // k_sum_forces adds each atom's slots, which lie together, in a fixed order.
 template <bool kE>
-__global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs C, const double* __restrict__ x,
+__device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf, CellArgs C, const double* __restrict__ x,
                                                 const int32_t* __restrict__ bij, const double* __restrict__ bkr,
                                                 const int32_t* __restrict__ aijk, const double* __restrict__ akt,
                                                 const int32_t* __restrict__ fij, const double* __restrict__ fp, const int32_t* __restrict__ spos,
                                                 double* __restrict__ slot, double* __restrict__ epart, uint32_t* __restrict__ flags) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  // no contraction of a·b + c into FMAs here: the energy variant's extra arithmetic must not change how the forces are
+  // rounded (forces are the same whether or not energies are asked for)
+#pragma clang fp contract(off)
+  const int g = bx * blockDim.x + threadIdx.x;
   double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (g < nb || (g >= nb + na && g < nb + na + nf)) {
     const bool fene = g >= nb;
@@ -147,16 +150,24 @@ __global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs
         for (int q = 0; q < kEB; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x < kEB) epart[kEB * size_t(blockIdx.x) + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < kEB) epart[kEB * size_t(bx) + threadIdx.x] = red[threadIdx.x][0];
   }
+}
+template <bool kE>
+__global__ void __launch_bounds__(256) k_bonded(int nb, int na, int nf, CellArgs C, const double* __restrict__ x,
+                                                const int32_t* __restrict__ bij, const double* __restrict__ bkr,
+                                                const int32_t* __restrict__ aijk, const double* __restrict__ akt,
+                                                const int32_t* __restrict__ fij, const double* __restrict__ fp, const int32_t* __restrict__ spos,
+                                                double* __restrict__ slot, double* __restrict__ epart, uint32_t* __restrict__ flags) {
+  b_bonded<kE>(blockIdx.x, gridDim.x, nb, na, nf, C, x, bij, bkr, aijk, akt, fij, fp, spos, slot, epart, flags);
 }
 
 // per atom: the pair forces (by slot of the cluster list), the reciprocal forces and the bonded forces — the atom's
 // bonded slots aoff[i] … aoff[i+1] (contiguous) summed first, in the order of its terms (aoff null: no bonded part)
-__global__ void __launch_bounds__(256) k_sum_forces(int n, const uint32_t* __restrict__ iperm, const float4* __restrict__ fnb,
+__device__ __forceinline__ void b_sum_forces(int bx, int gx, int n, const uint32_t* __restrict__ iperm, const float4* __restrict__ fnb,
                                                     const float* __restrict__ frec, const int32_t* __restrict__ aoff,
                                                     const double* __restrict__ slot, double* __restrict__ f) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double Fb[3] = {0, 0, 0};
   if (aoff) {
@@ -174,6 +185,11 @@ __global__ void __launch_bounds__(256) k_sum_forces(int n, const uint32_t* __res
   if (aoff)
     for (int m = 0; m < 3; ++m) F[m] += Fb[m];
   for (int m = 0; m < 3; ++m) f[3 * i + m] = F[m];
+}
+__global__ void __launch_bounds__(256) k_sum_forces(int n, const uint32_t* __restrict__ iperm, const float4* __restrict__ fnb,
+                                                    const float* __restrict__ frec, const int32_t* __restrict__ aoff,
+                                                    const double* __restrict__ slot, double* __restrict__ f) {
+  b_sum_forces(blockIdx.x, gridDim.x, n, iperm, fnb, frec, aoff, slot, f);
 }
 
 // Σ_slots x ⊗ F of the pair forces, x the listed positions X + origin (core/src/nbnxm.cpp:585-592)
@@ -275,7 +291,7 @@ __device__ inline void normals3(uint64_t seed, uint64_t step, uint32_t atom, dou
 
 // the workgroup's sums of v into its row of part (kAdd: added to what the row holds)
 template <int kParts, bool kAdd = false>
-__device__ inline void block_sum_store(double (&v)[kParts], double* part) {
+__device__ inline void block_sum_store(double (&v)[kParts], double* part, int row) {
   __shared__ double red[kParts][256];
   for (int q = 0; q < kParts; ++q) red[q][threadIdx.x] = v[q];
   __syncthreads();
@@ -285,7 +301,7 @@ __device__ inline void block_sum_store(double (&v)[kParts], double* part) {
     __syncthreads();
   }
   if (threadIdx.x < kParts) {
-    double& d = part[kParts * size_t(blockIdx.x) + threadIdx.x];
+    double& d = part[kParts * size_t(row) + threadIdx.x];
     d = kAdd ? d + red[threadIdx.x][0] : red[threadIdx.x][0];
   }
 }
@@ -308,13 +324,13 @@ __global__ void __launch_bounds__(256) k_vv_open(int n, double hk, double dt, co
 }
 
 // B (half kick), A (half drift), O, A; partials: kinetic energy before and after O (the bath's share)
-__global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
+__device__ __forceinline__ void b_baoab_open(int bx, int gx, int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
                                                     const double* __restrict__ sdv, uint64_t seed, const uint64_t* __restrict__ stepc,
                                                     uint64_t step_add, const double* __restrict__ f,
                                                     double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
                                                     const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
                                                     double* __restrict__ part) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bx * blockDim.x + threadIdx.x;
   const uint64_t step = (stepc ? *stepc : 0) + step_add;
   double ke[1] = {0};
   if (i < n) {
@@ -335,14 +351,22 @@ __global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt
     ke[0] = 0.5 * mi * (k1 - k0) / kAcc;
   }
   move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
-  block_sum_store<1, true>(ke, part);
+  block_sum_store<1, true>(ke, part, bx);
+}
+__global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
+                                                    const double* __restrict__ sdv, uint64_t seed, const uint64_t* __restrict__ stepc,
+                                                    uint64_t step_add, const double* __restrict__ f,
+                                                    double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
+                                                    const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
+                                                    double* __restrict__ part) {
+  b_baoab_open(blockIdx.x, gridDim.x, n, hk, hdt, c1, c2, minv, sdv, seed, stepc, step_add, f, x, v, x0, x0p, lim_b, lim_p, flags, part);
 }
 
 // the closing half kick; kinetic energy partials (Σ ½ m v², and the tensor m v_a v_b) when asked
 template <bool kKE>
-__global__ void __launch_bounds__(256) k_kick(int n, double hk, const double* __restrict__ minv, const double* __restrict__ f,
+__device__ __forceinline__ void b_kick(int bx, int gx, int n, double hk, const double* __restrict__ minv, const double* __restrict__ f,
                                               double* __restrict__ v, double* __restrict__ part) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bx * blockDim.x + threadIdx.x;
   double k[7] = {0, 0, 0, 0, 0, 0, 0};
   if (i < n) {
     const double a = hk * minv[i];
@@ -355,7 +379,12 @@ __global__ void __launch_bounds__(256) k_kick(int n, double hk, const double* __
       k[4] = mi * vv[0] * vv[1], k[5] = mi * vv[0] * vv[2], k[6] = mi * vv[1] * vv[2];
     }
   }
-  if (kKE) block_sum_store<7>(k, part);
+  if (kKE) block_sum_store<7>(k, part, bx);
+}
+template <bool kKE>
+__global__ void __launch_bounds__(256) k_kick(int n, double hk, const double* __restrict__ minv, const double* __restrict__ f,
+                                              double* __restrict__ v, double* __restrict__ part) {
+  b_kick<kKE>(blockIdx.x, gridDim.x, n, hk, minv, f, v, part);
 }
 
 // Bussi's α from the reduced kinetic energy (dynamics.cpp:466-477); r1 and Σ R² drawn on the host in the reference's
@@ -403,8 +432,11 @@ __global__ void k_add_scalar(double* __restrict__ part, int64_t rows, double* __
 }
 
 // the step counter of captured dynamics: the steps a replay took
+__device__ __forceinline__ void b_step_advance(int bx, int gx, uint64_t* __restrict__ stepc, uint64_t by) {
+  if (bx == 0 && threadIdx.x == 0) *stepc += by;
+}
 __global__ void k_step_advance(uint64_t* __restrict__ stepc, uint64_t by) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *stepc += by;
+  b_step_advance(blockIdx.x, gridDim.x, stepc, by);
 }
 
 CellArgs cell_args(const capsgpu_ctx& c) {
@@ -575,6 +607,129 @@ void bath_add(capsgpu_ctx& c, double* part, double* bath) {
   hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(256), 0, c.stream, part, int64_t(blocks_for(c.n, 256)), bath);
   LAUNCHCHECK();
   kt_end(c);
+}
+
+
+// ---- many contexts in one launch per kernel (the replica batch) ----------------------------------------------------
+// As the grouped PME kernels: a workgroup finds its replica in a prefix array of workgroup counts and runs the
+// single-context body with that replica's arguments, so each replica's arithmetic (and result) is unchanged.
+namespace {
+struct StepRep {
+  int n;
+  double hk, hdt, c1, c2;
+  uint64_t seed;
+  uint64_t* stepc;
+  const double *minv, *sdv;
+  double *x, *v, *f;
+  const double *x0, *x0p;
+  uint32_t* flags;
+  double *bpart, *kpart;
+  int nb, na, nf;
+  CellArgs C;
+  const int32_t *bij, *aijk, *fij, *spos, *aoff;
+  const double *bkr, *akt, *fp;
+  double* slot;
+  const float* frec;
+};
+
+__device__ inline int find_rep(const int* __restrict__ off, int nr, int b) {
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) kg_baoab_open(const StepRep* __restrict__ R, const int* __restrict__ off, int nr, uint64_t step_add) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const StepRep& a = R[r];
+  b_baoab_open(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.hk, a.hdt, a.c1, a.c2, a.minv, a.sdv, a.seed, a.stepc, step_add, a.f,
+               a.x, a.v, a.x0, a.x0p, -1.0, -1.0, a.flags, a.bpart);
+}
+
+__global__ void __launch_bounds__(256) kg_bonded(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const StepRep& a = R[r];
+  b_bonded<false>(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.nb, a.na, a.nf, a.C, a.x, a.bij, a.bkr, a.aijk, a.akt, a.fij, a.fp,
+                  a.spos, a.slot, nullptr, a.flags);
+}
+
+__global__ void __launch_bounds__(256) kg_sum_forces(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const StepRep& a = R[r];
+  b_sum_forces(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, nullptr, nullptr, a.frec, a.aoff, a.slot, a.f);
+}
+
+__global__ void __launch_bounds__(256) kg_kick(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const StepRep& a = R[r];
+  b_kick<false>(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.hk, a.minv, a.f, a.v, a.kpart);
+}
+
+__global__ void kg_step_advance(const StepRep* __restrict__ R, int nr, uint64_t by) {
+  const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r < nr) *R[r].stepc += by;
+}
+}  // namespace
+
+struct StepGroup {
+  int nr = 0;
+  DevBuf<unsigned char> reps;             // StepRep[nr]
+  DevBuf<int32_t> off_atoms, off_terms;   // workgroup prefixes
+  int h_atoms = 0, h_terms = 0;
+};
+
+StepGroup* step_group_create(const std::vector<StepGroupInput>& in, hipStream_t s) {
+  auto g = std::make_unique<StepGroup>();
+  g->nr = int(in.size());
+  if (g->nr < 1) throw Error(-1, "a step group needs a replica");
+  std::vector<StepRep> R(in.size());
+  std::vector<int32_t> oa(1, 0), ot(1, 0);
+  for (size_t i = 0; i < in.size(); ++i) {
+    capsgpu_ctx& c = *in[i].c;
+    Bonded& B = c.bonded;
+    StepRep& a = R[i];
+    a.n = c.n;
+    a.hk = in[i].hk, a.hdt = in[i].hdt, a.c1 = in[i].c1, a.c2 = in[i].c2, a.seed = in[i].seed, a.stepc = in[i].stepc;
+    a.minv = c.d_minv.p, a.sdv = in[i].sdv, a.x = c.x.p, a.v = c.v.p, a.f = c.f.p, a.x0 = c.x0.p, a.x0p = c.x0p.p;
+    a.flags = c.flags.p, a.bpart = in[i].bpart, a.kpart = in[i].kpart;
+    a.nb = in[i].bonded ? B.nb : 0, a.na = in[i].bonded ? B.na : 0, a.nf = in[i].bonded ? B.nf : 0;
+    a.C = cell_args(c);
+    a.bij = B.bij.p, a.aijk = B.aijk.p, a.fij = B.fij.p, a.spos = B.spos.p, a.aoff = in[i].bonded ? B.aoff.p : nullptr;
+    a.bkr = B.bkr.p, a.akt = B.akt.p, a.fp = B.fp.p, a.slot = B.slot.p;
+    a.frec = in[i].recip ? c.pme.fpme.p : nullptr;
+    oa.push_back(oa.back() + int(blocks_for(size_t(c.n), 256)));
+    ot.push_back(ot.back() + int(blocks_for(size_t(std::max(a.nb + a.na + a.nf, 1)), 256)));
+  }
+  g->reps.upload(reinterpret_cast<const unsigned char*>(R.data()), R.size() * sizeof(StepRep), s);
+  g->off_atoms.upload(oa, s), g->off_terms.upload(ot, s);
+  g->h_atoms = oa.back(), g->h_terms = ot.back();
+  return g.release();
+}
+
+void step_group_destroy(StepGroup* g) { delete g; }
+
+void step_group_open(StepGroup& g, hipStream_t s, uint64_t step_add) {
+  hipLaunchKernelGGL(kg_baoab_open, dim3(unsigned(g.h_atoms)), dim3(256), 0, s, reinterpret_cast<const StepRep*>(g.reps.p), g.off_atoms.p,
+                     g.nr, step_add);
+  LAUNCHCHECK();
+}
+
+void step_group_close(StepGroup& g, hipStream_t s) {
+  const StepRep* R = reinterpret_cast<const StepRep*>(g.reps.p);
+  hipLaunchKernelGGL(kg_bonded, dim3(unsigned(g.h_terms)), dim3(256), 0, s, R, g.off_terms.p, g.nr);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_sum_forces, dim3(unsigned(g.h_atoms)), dim3(256), 0, s, R, g.off_atoms.p, g.nr);
+  LAUNCHCHECK();
+  hipLaunchKernelGGL(kg_kick, dim3(unsigned(g.h_atoms)), dim3(256), 0, s, R, g.off_atoms.p, g.nr);
+  LAUNCHCHECK();
+}
+
+void step_group_advance(StepGroup& g, hipStream_t s, uint64_t by) {
+  hipLaunchKernelGGL(kg_step_advance, dim3(unsigned((g.nr + 63) / 64)), dim3(64), 0, s, reinterpret_cast<const StepRep*>(g.reps.p), g.nr, by);
+  LAUNCHCHECK();
 }
 
 }  // namespace capsgpu

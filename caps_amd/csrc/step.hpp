@@ -19,4 +19,22 @@ void bussi(capsgpu_ctx& c, const double* part, double kt_target, double ndof, do
 void scale_v(capsgpu_ctx& c, const double* alpha);
 // *bath −= Σ part over the rows of c's atom blocks; the rows are cleared
 void bath_add(capsgpu_ctx& c, double* part, double* bath);
+// many contexts' Langevin step, one launch per kernel: per replica its constants and device arrays
+struct StepGroupInput {
+  capsgpu_ctx* c;
+  double hk, hdt, c1, c2;
+  uint64_t seed;
+  uint64_t* stepc;
+  const double* sdv;
+  double *bpart, *kpart;
+  bool bonded, recip;
+};
+struct StepGroup;
+StepGroup* step_group_create(const std::vector<StepGroupInput>& in, hipStream_t s);
+void step_group_destroy(StepGroup* g);
+// BAOAB's opening B, A, O, A (the noise for step *stepc + step_add of each replica)
+void step_group_open(StepGroup& g, hipStream_t s, uint64_t step_add);
+// the bonded terms, each atom's force sum (reciprocal + bonded) and the closing kick
+void step_group_close(StepGroup& g, hipStream_t s);
+void step_group_advance(StepGroup& g, hipStream_t s, uint64_t by);
 }  // namespace capsgpu

@@ -179,9 +179,11 @@ typedef struct capsgpu_batch_opts {
   double tau_t;                 /* fs: Langevin 1/γ */
   uint64_t seed;                /* replica r draws its noise (and new velocities) from seed + r */
   int32_t new_velocities;       /* 1: Maxwell–Boltzmann at each replica's temperature, else the set state */
-  int32_t mode;                 /* 0: the tuned launch shape; 1: graphs with the shape below; 2: direct launches */
+  int32_t mode;                 /* 0: the tuned launch shape; 1: graphs with the shape below; 2: direct launches;
+                                   3: grouped launches — each kernel of the step launched once for every replica (they
+                                   need PME's reciprocal part with one spline order), graph_steps steps per graph */
   int32_t branches;             /* mode 1: streams the replicas are spread over (longest first), each with its graph */
-  int32_t graph_steps;          /* mode 1: steps one graph holds */
+  int32_t graph_steps;          /* modes 1 and 3: steps one graph holds */
 } capsgpu_batch_opts;
 
 typedef struct capsgpu_batch_report {

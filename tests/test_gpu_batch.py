@@ -1,6 +1,7 @@
 """GPU checks of replicas stepped together (no reference counterpart: the reference runs a cooling scan's replicas one
 after another through run_dynamics, core/src/mechanics.cpp:1381-1470): every launch shape — graphs replayed on several
-streams, direct launches, the tuned choice — leaves each replica bit-identical to run_dynamics of that replica alone."""
+streams, direct launches, grouped launches (each kernel once for every replica), the tuned choice — leaves each replica
+bit-identical to run_dynamics of that replica alone."""
 import copy
 
 import numpy as np
@@ -37,7 +38,7 @@ def alone():
     return _alone()
 
 
-@pytest.mark.parametrize("mode,branches,graph_steps", [("graph", 2, 8), ("graph", 1, 4), ("direct", 1, 1)])
+@pytest.mark.parametrize("mode,branches,graph_steps", [("graph", 2, 8), ("graph", 1, 4), ("direct", 1, 1), ("grouped", 1, 8), ("grouped", 1, 5)])
 def test_batch_matches_each_replica_alone(alone, mode, branches, graph_steps):
     print("case batch %s branches %d graph_steps %d" % (mode, branches, graph_steps), flush=True)
     sys_ = _systems()
@@ -61,7 +62,7 @@ def test_tuned_batch_matches_and_keeps_its_choice(alone):
     rep = b.run(sys_, TEMPS, o)
     print("tuned shape: %s, %d branches, %d steps per graph (tuning %.3f s)" % (rep.mode, rep.branches, rep.graph_steps, rep.tune_seconds),
           flush=True)
-    assert rep.mode in ("graph", "direct") and rep.tune_seconds > 0
+    assert rep.mode in ("graph", "direct", "grouped") and rep.tune_seconds > 0
     for r, s in enumerate(sys_):
         assert np.array_equal(s.x, alone[r][0]) and np.array_equal(s.v, alone[r][1]), r
     # the same run again: the tuned shape is reused (no timing) and, from the same start, the same end
