@@ -258,6 +258,20 @@ def main():
             "force_rmse_stale": float(np.mean([r["learned_rmse_stale"] for r in judged])) if judged else None,
             "rmse_definition": "|F_slow predicted - F_slow PME| / |F total| over all atoms at each full step",
             "speedup_vs_plain": (secs / lsecs)}
+    # r-RESPA (dynamics.cpp:113-123, 643-656) on the same system and terms: Bussi 300 K, the slow forces (here PME's
+    # reciprocal part) kicking over a 2 fs outer step, the bonded terms over two 1 fs inner steps
+    ro = ca.DynamicsOptions(dt=2.0, temperature=TEMP, thermostat="bussi", tau_t=TAU, seed=SEED, new_velocities=False,
+                            thermo_every=10 ** 9, energy=dyn.energy, respa=2, steps=10)
+    ca.run_dynamics(system, ro, evaluator=ev)
+    ro.steps = a.steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ca.run_dynamics(system, ro, evaluator=ev)
+    torch.cuda.synchronize()
+    rsecs = time.perf_counter() - t0
+    out["respa"] = {"ns_per_day": ro.dt * 1e-6 * 86400.0 / (rsecs / a.steps), "ms_per_step": 1e3 * rsecs / a.steps, "outer_dt_fs": ro.dt,
+                    "inner_steps": 2, "thermostat": "bussi",
+                    "what": "outer kicks with the slow forces (PME reciprocal part), bonded terms on the inner steps"}
     if a.replicas > 0:
         out["replica_batch"] = replica_batch(ca, torch, a.steps, a.replicas)
     if not a.no_cpu:

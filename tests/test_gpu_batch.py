@@ -103,3 +103,33 @@ def test_batch_refuses_the_pair_path_and_bad_shapes():
         b.run([s], [300.0], ca.BatchOptions(steps=2, mode="graph", branches=2, graph_steps=4))
     with pytest.raises(ValueError, match="temperature"):
         b.run([s], [-1.0], ca.BatchOptions(steps=2))
+
+
+def test_grouped_launches_need_one_spline_order_and_keep_parts_per_replica():
+    # a replica with spline order 5 among order-4 ones: grouped launches are refused, the tuned shape falls back to
+    # per-replica launches and stays exact; a replica without bonded terms (parts = 4) takes part in grouped launches
+    print("case batch mixed orders and parts", flush=True)
+    o5 = ca.EnergyOptions(cutoff=9.0, coulomb=True, electrostatics="pme", pme_order=5, pme_spacing=1.0, tail=False, parts=5)
+    o4r = ca.EnergyOptions(cutoff=9.0, coulomb=True, electrostatics="pme", pme_order=4, pme_spacing=1.0, tail=False, parts=4)
+
+    def alone(systems, opts):
+        out = []
+        for r, (s, op) in enumerate(zip(systems, opts)):
+            ca.run_dynamics(s, ca.DynamicsOptions(dt=1.0, steps=9, temperature=TEMPS[r], thermostat="langevin", tau_t=100, seed=SEED + r,
+                                                  thermo_every=9, new_velocities=True, energy=op), evaluator=ca.Evaluator(s.ff, op))
+            out.append((s.x.copy(), s.v.copy()))
+        return out
+
+    for opts, mode in (((OPTS, o5, OPTS), "tuned"), ((OPTS, o4r, OPTS), "grouped")):
+        ref = alone(_systems(), opts)
+        sys_ = _systems()
+        b = ca.Batch([ca.Evaluator(s.ff, op) for s, op in zip(sys_, opts)])
+        if opts[1] is o5:
+            with pytest.raises(ValueError, match="one spline order"):
+                b.run(sys_, TEMPS, ca.BatchOptions(dt=1.0, steps=9, tau_t=100, seed=SEED, mode="grouped", graph_steps=4))
+            sys_ = _systems()
+        rep = b.run(sys_, TEMPS, ca.BatchOptions(dt=1.0, steps=9, tau_t=100, seed=SEED, mode=mode, graph_steps=4))
+        if opts[1] is o5:
+            assert rep.mode in ("graph", "direct")
+        for r, s in enumerate(sys_):
+            assert np.array_equal(s.x, ref[r][0]) and np.array_equal(s.v, ref[r][1]), (mode, r)
