@@ -955,8 +955,23 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
   double tf[CAPSGPU_NTERMS];
   evaluate(c, want, tf, true, fast_parts, c.f.p);
   const bool training = L.frames < L.train_frames;
-  const bool full = training || !L.learned_mode || L.last_full < 0 || step - L.last_full >= L.k;
   const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
+  // after a fallback the interval keeps its k steps: plain PME in between (no descriptors, no prediction), and at its
+  // end a full step that judges the k-step prediction made from its start, trains on the pair and decides whether the
+  // next interval is learned — the network is always judged on the horizon it predicts
+  if (!training && !L.learned_mode && L.last_full >= 0 && step - L.last_full < L.k) {
+    double ts[CAPSGPU_NTERMS];
+    evaluate(c, want, ts, true, 4, L.fslow.p);
+    if (want) std::memcpy(L.t_slow, ts, sizeof ts);
+    kt_begin(c, "k_axpy");
+    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fslow.p, c.f.p);
+    LAUNCHCHECK();
+    kt_end(c);
+    if (want)
+      for (int k = 0; k < CAPSGPU_NTERMS; ++k) t[k] = tf[k] + L.t_slow[k];
+    return 0;
+  }
+  const bool full = training || !L.learned_mode || L.last_full < 0 || step - L.last_full >= L.k;
   descriptors(c, L, L.Dcur.p);
   int result;
   if (full) {
