@@ -130,21 +130,22 @@ __global__ void k_tile_rank(int n, int ntiles, const float4* __restrict__ wq, co
 // one 64-lane workgroup per 4x4x4 tile. The atoms whose first point lies in a tile that can reach this one (the home
 // table lists those tiles per axis) are staged in LDS 64 at a time. With r_d = 4 t_d − b_d the offset of an atom's
 // first point from the tile's corner (folded into [−3, p]), a point (i, j, k) of the tile sees θa[i + r_a],
-// θb[j + r_b], θc[k + r_c]; so per atom: θb and q·θc, the four θa[i + r_a] of the tile's points along a, and a 16-bit
-// mask of the tile's rows (j, k) it reaches. Lane = (row (j, k), quarter of the atoms): it sums its quarter's terms
-// for the row's 4 points; the quarters are then added in a fixed order (every point written once, no atomics).
-// Dynamic LDS: kSpreadChunk · 2p floats (small enough for eight workgroups per SIMD).
+// θb[j + r_b], θc[k + r_c]. Staging (one atom per lane): the four θa[i + r_a] of the tile's points along a and the
+// sixteen row weights θb[j + r_b]·(q θc[k + r_c]) of the tile's rows (j, k), zero outside the support; each window of
+// four is read from the lane's own zero-padded column of LDS scratch at offset r + 3, so no selects (column-major:
+// element e of lane L at 64 e + L, every access of the wave one word per bank). Lane = (row (j, k), quarter of the
+// atoms): per atom one row weight, one float4 of θa and four FMAs; the quarters are then added in a fixed order
+// (every point written once, no atomics). An atom's row weights take 20 floats (16 and a pad: its float4 stores meet
+// no bank twice). Static LDS 6 KB (eight workgroups per SIMD).
+constexpr int kRowStride = 20;
 constexpr int kSpreadChunk = 64, kMaxHome = 64;   // at most 4 home tiles per axis reach a tile (p <= 8, partial last tile)
 template <int P>
 __device__ __forceinline__ void b_spread(int tile, GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
-                                               const float4* __restrict__ wsort, const int4* __restrict__ bsort, float* __restrict__ grid) {
-  extern __shared__ float2 lds[];
-  __shared__ float4 s_wx[kSpreadChunk];   // θa[r_a + i] for the tile's four points i along a (zero outside the support)
-  __shared__ int4 s_ent[kSpreadChunk];    // {2p j + r_b + 3, 2p j + p + r_c + 3, the rows (j, k) of the tile it reaches, 0}
+                                         const float4* __restrict__ wsort, const int4* __restrict__ bsort, float* __restrict__ grid) {
+  static_assert(P + 6 < 16, "a window's last read must stay inside the lane's 16 scratch elements");
+  __shared__ float4 s_row[kSpreadChunk * kRowStride / 4];   // per atom: row weights (row j + 4k); first the scratch
+  __shared__ float4 s_wx[kSpreadChunk];                     // per atom: θa[r_a + i] for the tile's four points i along a
   const int lane = threadIdx.x;
-  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
-  constexpr int p = P;
-  float* s_bc = reinterpret_cast<float*>(lds);          // [chunk][2p]: θb, then q θc
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
   // the candidate tiles in a fixed order (c, b, a) from the home table, their atom counts in parallel, a running sum
   const int* hx = home + (0 * G.ntm + tx) * 5;
@@ -155,7 +156,11 @@ __device__ __forceinline__ void b_spread(int tile, GridArgs G, const int* __rest
   // number (ex); the lanes' values are read back with readlane (the home tiles are few: at most 27 for p = 8)
   int cnt = 0, hb = 0;   // atom counts fit 32 bits (the library's atom count is an int)
   if (lane < nh) {
-    const int ix = lane % mx, iy = (lane / mx) % my, iz = lane / (mx * my);
+    // lane = ix + mx (iy + my iz) with lane < 64 and mx, my <= 4: a quotient q = ⌊lane / d⌋ is ⌊lane · rcp(d) + 1e-3⌋
+    // exactly (the product's error is below 1e-5 and a non-integer quotient's fraction at most 15/16)
+    const int mxy = mx * my;
+    const int iz = int(float(lane) * __builtin_amdgcn_rcpf(float(mxy)) + 1e-3f), rem = lane - iz * mxy;
+    const int iy = int(float(rem) * __builtin_amdgcn_rcpf(float(mx)) + 1e-3f), ix = rem - iy * mx;
     const int h = (hz[1 + iz] * G.nt[1] + hy[1 + iy]) * G.nt[0] + hx[1 + ix];
     hb = int(tstart[h]);
     cnt = int(tstart[h + 1] - tstart[h]);
@@ -168,9 +173,10 @@ __device__ __forceinline__ void b_spread(int tile, GridArgs G, const int* __rest
   }
   const int ex = cnt - own;
   const int m = __builtin_amdgcn_readlane(cnt, nh - 1);
-  // lane = (row (j, k) of the tile, quarter of the atoms); its row's bit in the reach mask is j + 4 k = lane & 15
-  const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4, rowbit = 1 << (lane & 15);
-  const int lane_b = gy - 3, lane_c = gz - 3;
+  // lane = (row (j, k) of the tile, quarter of the atoms); row j + 4 k = lane & 15
+  const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4;
+  float* scr = reinterpret_cast<float*>(s_row) + lane;   // scratch element e of this lane: scr[64 e], e < 16
+  const float* srow = reinterpret_cast<const float*>(s_row) + (lane & 15);
   float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
   for (int c0 = 0; c0 < m; c0 += kSpreadChunk) {
     const int f = c0 + lane;
@@ -180,6 +186,14 @@ __device__ __forceinline__ void b_spread(int tile, GridArgs G, const int* __rest
       const int exk = __builtin_amdgcn_readlane(ex, k), hbk = __builtin_amdgcn_readlane(hb, k);
       at = f >= exk ? hbk + (f - exk) : at;
     }
+    // the scratch's zero padding around θ (elements [0, 3) and [3 + p, p + 7)); the lane's writes and reads below
+    // touch its own column only, and its LDS accesses complete in order, so each read sees the write before it
+#pragma unroll
+    for (int e = 0; e < 3; ++e) scr[64 * e] = 0.0f;
+#pragma unroll
+    for (int e = 3 + P; e < P + 7; ++e) scr[64 * e] = 0.0f;
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 rw[4] = {z4, z4, z4, z4}, wxv = z4;   // a slot past the candidates adds exact zeros
     if (f < m) {
       const int4 b = bsort[at];
       const float4 wv = wsort[at];
@@ -192,53 +206,46 @@ __device__ __forceinline__ void b_spread(int tile, GridArgs G, const int* __rest
         v += v < 0 ? G.k[d] : 0;
         v -= v >= G.k[d] ? G.k[d] : 0;
         v -= v > G.k[d] - 4 ? G.k[d] : 0;
-        r[d] = v < p ? v : p;
+        r[d] = v < P ? v : P;
       }
       float ta[P], tb[P], tc[P], dd[P];
       bspline<P>(wv.x, ta, dd);
       bspline<P>(wv.y, tb, dd);
       bspline<P>(wv.z, tc, dd);
-      float wx[4];
+      // the windows: θ at elements [3, 3 + p) between zeros, four read from element r + 3 on (r + 6 <= p + 6 < 16)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float w = 0.0f;
+      for (int d = 0; d < P; ++d) scr[64 * (3 + d)] = ta[d];
+      const float* wa = scr + 64 * (r[0] + 3);
+      wxv = make_float4(wa[0], wa[64], wa[128], wa[192]);
 #pragma unroll
-        for (int k = 0; k < P; ++k) w = (i + r[0] == k) ? ta[k] : w;
-        wx[i] = w;
-      }
-      s_wx[lane] = make_float4(wx[0], wx[1], wx[2], wx[3]);
-      // the rows (j, k) with 0 ≤ j + r_b < p and 0 ≤ k + r_c < p
-      int mb = 0, mc = 0;
+      for (int d = 0; d < P; ++d) scr[64 * (3 + d)] = tb[d];
+      const float* wb = scr + 64 * (r[1] + 3);
+      const float b0 = wb[0], b1 = wb[64], b2 = wb[128], b3 = wb[192];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) mb |= (unsigned(i + r[1]) < unsigned(P) ? 1 : 0) << i, mc |= (unsigned(i + r[2]) < unsigned(P) ? 1 : 0) << i;
-      int reach = 0;
+      for (int d = 0; d < P; ++d) scr[64 * (3 + d)] = wv.w * tc[d];
+      const float* wc = scr + 64 * (r[2] + 3);
+      const float cw[4] = {wc[0], wc[64], wc[128], wc[192]};
 #pragma unroll
-      for (int k = 0; k < 4; ++k) reach |= ((mc >> k) & 1) ? mb << (4 * k) : 0;
-      // a reached row has 0 ≤ j + r_b < p: the offsets below stay inside the atom's own θb and q θc
-      const int sl = (f - c0) * 2 * P;
-      s_ent[lane] = make_int4(sl + r[1] + 3, sl + P + r[2] + 3, reach, 0);
-      float* sbc = s_bc + sl;
-#pragma unroll
-      for (int d = 0; d < P; ++d) sbc[d] = tb[d], sbc[P + d] = wv.w * tc[d];
+      for (int k = 0; k < 4; ++k) rw[k] = make_float4(b0 * cw[k], b1 * cw[k], b2 * cw[k], b3 * cw[k]);
     }
+    // every lane's scratch reads are done before the row weights overwrite the scratch
     __syncthreads();
-    // branch-free: an atom that does not reach the lane's row (or a slot past the chunk's end) adds an exact zero, so the
-    // loop unrolls and its LDS reads overlap
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s_row[lane * (kRowStride / 4) + k] = rw[k];
+    s_wx[lane] = wxv;
+    __syncthreads();
     const int n_here = m - c0 < kSpreadChunk ? m - c0 : kSpreadChunk;
 #pragma unroll 4
     for (int j0 = 0; j0 < n_here; j0 += 4) {
-      const int j = j0 + quarter < n_here ? j0 + quarter : 0;
-      const int4 e = s_ent[j];
-      const bool hit = j0 + quarter < n_here && (e.z & rowbit) != 0;
-      const float vb = s_bc[hit ? e.x + lane_b : 0], vc = s_bc[hit ? e.y + lane_c : 0];
-      const float v = hit ? vb * vc : 0.0f;
+      const int j = j0 + quarter;   // j < 64; slots at or past n_here hold zeros
+      const float v = srow[kRowStride * j];
       const float4 w = s_wx[j];
       acc0 += v * w.x, acc1 += v * w.y, acc2 += v * w.z, acc3 += v * w.w;
     }
     __syncthreads();
   }
   // the quarters' sums by point (i + 4 j + 16 k), in the staging area, then each lane adds its point's four in order
-  float* s_part = s_bc;   // [4][64]; kSpreadChunk · 2p ≥ 256 floats
+  float* s_part = reinterpret_cast<float*>(s_row);   // [4][64]
   const int row = 4 * gy + 16 * gz;
   s_part[64 * quarter + row] = acc0, s_part[64 * quarter + row + 1] = acc1, s_part[64 * quarter + row + 2] = acc2,
   s_part[64 * quarter + row + 3] = acc3;
@@ -721,6 +728,12 @@ __device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __
   float* box = reinterpret_cast<float*>(lds);            // [W][W][WX] (c, b, a)
   float* s_w = box + W * W * WX;                         // [8][6 P]: θa θb θc dθa dθb dθc
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
+  const int slot = lane >> 3, sub = lane & 7;
+  // the first eight atoms' data are requested before the box, so the two loads' latencies overlap
+  int64_t at = a0 + (slot < na ? slot : 0);
+  uint32_t i_next = tatoms[at];
+  float4 w_next = wsort[at];
+  int4 b_next = bsort[at];
   for (int r = lane; r < W * W; r += 64) {
     int yy = 4 * ty + r % W, zz = 4 * tz + r / W;
     yy -= yy >= G.k[1] ? G.k[1] : 0;
@@ -735,14 +748,16 @@ __device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __
       box[r * WX + xx] = src[gx];
     }
   }
-  const int slot = lane >> 3, sub = lane & 7;
   for (int64_t g0 = 0; g0 < na; g0 += 8) {
-    const int64_t ia = g0 + slot;
-    const bool has = ia < na;
-    const int64_t at = a0 + (has ? ia : 0);
-    const uint32_t i = tatoms[at];
-    const float4 w = wsort[at];
-    const int4 b = bsort[at];
+    const bool has = g0 + slot < na;
+    const uint32_t i = i_next;
+    const float4 w = w_next;
+    const int4 b = b_next;
+    if (g0 + 8 < na) {
+      const int64_t ia = g0 + 8 + slot;
+      at = a0 + (ia < na ? ia : 0);
+      i_next = tatoms[at], w_next = wsort[at], b_next = bsort[at];
+    }
     float* ws = s_w + slot * 6 * P;
     if (sub < 3) {
       // lane sub of the atom's eight: the splines of axis sub
@@ -1102,10 +1117,9 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   P.counts_clear = true;
   kt_end(c);
   kt_begin(c, "k_spread");
-  const size_t lds_spread = size_t(kSpreadChunk) * size_t(2 * G.p) * sizeof(float);
   switch (G.p) {
 #define CAPSGPU_SPREAD(Q) \
-  case Q: hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), lds_spread, s, G, P.home.p, P.tstart.p, wsort, bsort, P.grid.p); break;
+  case Q: hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), 0, s, G, P.home.p, P.tstart.p, wsort, bsort, P.grid.p); break;
     CAPSGPU_SPREAD(3) CAPSGPU_SPREAD(4) CAPSGPU_SPREAD(5) CAPSGPU_SPREAD(6) CAPSGPU_SPREAD(7) CAPSGPU_SPREAD(8)
 #undef CAPSGPU_SPREAD
   }
@@ -1309,11 +1323,10 @@ void pme_group_forces(PmeGroup& g, hipStream_t s) {
   LAUNCHCHECK();
   hipLaunchKernelGGL(kg_tile_rank, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
   LAUNCHCHECK();
-  const size_t lds_spread = size_t(kSpreadChunk) * size_t(2 * g.order) * sizeof(float);
   const size_t lds_gather = (size_t(g.order + 3) * (g.order + 3) * (g.order + 4) + 8 * 6 * size_t(g.order)) * sizeof(float);
   switch (g.order) {
 #define CAPSGPU_GSPREAD(Q) \
-  case Q: hipLaunchKernelGGL(kg_spread<Q>, dim3(unsigned(nt)), dim3(64), lds_spread, s, R, g.off_tiles.p, nr); break;
+  case Q: hipLaunchKernelGGL(kg_spread<Q>, dim3(unsigned(nt)), dim3(64), 0, s, R, g.off_tiles.p, nr); break;
     CAPSGPU_GSPREAD(3) CAPSGPU_GSPREAD(4) CAPSGPU_GSPREAD(5) CAPSGPU_GSPREAD(6) CAPSGPU_GSPREAD(7) CAPSGPU_GSPREAD(8)
 #undef CAPSGPU_GSPREAD
   }
