@@ -734,19 +734,29 @@ __device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __
   uint32_t i_next = tatoms[at];
   float4 w_next = wsort[at];
   int4 b_next = bsort[at];
-  for (int r = lane; r < W * W; r += 64) {
-    int yy = 4 * ty + r % W, zz = 4 * tz + r / W;
-    yy -= yy >= G.k[1] ? G.k[1] : 0;
-    yy -= yy >= G.k[1] ? G.k[1] : 0;
-    zz -= zz >= G.k[2] ? G.k[2] : 0;
-    zz -= zz >= G.k[2] ? G.k[2] : 0;
-    const float* src = grid + (size_t(zz) * G.k[1] + yy) * G.k[0];
-    for (int xx = 0; xx < W; ++xx) {
-      int gx = 4 * tx + xx;
+  // the box, element e = x + W (y + W z) by lane: a load instruction reads 64 / W whole rows (few cache lines), all
+  // loads in flight before the LDS stores
+  constexpr int kBox = W * W * W, kIter = (kBox + 63) / 64;
+  float bv[kIter];
+#pragma unroll
+  for (int it = 0; it < kIter; ++it) {
+    const int e = lane + 64 * it;
+    if (e < kBox) {
+      const int xx = e % W, r = e / W;
+      int yy = 4 * ty + r % W, zz = 4 * tz + r / W, gx = 4 * tx + xx;
+      yy -= yy >= G.k[1] ? G.k[1] : 0;
+      yy -= yy >= G.k[1] ? G.k[1] : 0;
+      zz -= zz >= G.k[2] ? G.k[2] : 0;
+      zz -= zz >= G.k[2] ? G.k[2] : 0;
       gx -= gx >= G.k[0] ? G.k[0] : 0;
       gx -= gx >= G.k[0] ? G.k[0] : 0;
-      box[r * WX + xx] = src[gx];
+      bv[it] = grid[(size_t(zz) * G.k[1] + yy) * G.k[0] + gx];
     }
+  }
+#pragma unroll
+  for (int it = 0; it < kIter; ++it) {
+    const int e = lane + 64 * it;
+    if (e < kBox) box[(e / W) * WX + e % W] = bv[it];
   }
   for (int64_t g0 = 0; g0 < na; g0 += 8) {
     const bool has = g0 + slot < na;
