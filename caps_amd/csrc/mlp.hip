@@ -562,6 +562,10 @@ struct Learned {
   int k = 4, train_frames = 2000, frames = 0, samples = 8192, iters = 4;
   double tol = 1e-3, rmse = 0, rmse_stale = 0, e_scale = 1.0;
   int64_t last_full = -1;
+  // after a failed judgment: plain steps for `wait` steps (doubling per failure, at most kMaxWait), then a reference
+  // full step at ref_at and the judging full step k steps later
+  int wait = 0;
+  int64_t ref_at = -1;
   bool learned_mode = false, scaled = false;
   uint64_t rng = 0x9E3779B97F4A7C15ull;
   float lr = 1e-3f;
@@ -933,6 +937,8 @@ Learned* learned_begin(capsgpu_ctx& c, const capsgpu_dyn_opts& o) {
   L.train_frames = o.learned_train >= 0 ? o.learned_train : 2000;
   L.last_full = -1;
   L.ring_n = 0;
+  L.wait = 0;
+  L.ref_at = -1;
   (void)desc_geom(c);   // the cell must hold 5 cells of 4 Å per axis
   const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
   L.f0.resize(n3);
@@ -949,6 +955,8 @@ Learned* learned_begin(capsgpu_ctx& c, const capsgpu_dyn_opts& o) {
 
 // one step's forces into c.f (and the terms): the fast part (everything but PME's reciprocal part) at every step; the
 // slow part by PME at full steps (training, judging, a new reference point) or by the network in between
+constexpr int kMaxWait = 64;   // the longest wait after failed judgments, in steps
+
 int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t) {
   hipStream_t s = c.stream;
   const int fast_parts = (c.opt.parts & 3) | 8;
@@ -956,10 +964,14 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
   evaluate(c, want, tf, true, fast_parts, c.f.p);
   const bool training = L.frames < L.train_frames;
   const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
-  // after a fallback the interval keeps its k steps: plain PME in between (no descriptors, no prediction), and at its
-  // end a full step that judges the k-step prediction made from its start, trains on the pair and decides whether the
-  // next interval is learned — the network is always judged on the horizon it predicts
-  if (!training && !L.learned_mode && L.last_full >= 0 && step - L.last_full < L.k) {
+  // after a failed judgment the run waits with plain PME steps (no descriptors, no prediction): `wait` steps, doubling
+  // with each failure in a row up to kMaxWait; then a reference full step, and k steps later a full step that judges
+  // the k-step prediction made from the reference, trains on the pair and decides whether the next interval is learned
+  // — the network is always judged on the horizon it predicts, and a network that misses the tolerance costs a few
+  // full steps per kMaxWait + k steps instead of one per k
+  const bool waiting = L.ref_at >= 0 && step < L.ref_at;
+  const bool ref_only = L.ref_at >= 0 && step >= L.ref_at;   // a reference point after a wait: no judgment, no pair
+  if (!training && !L.learned_mode && L.last_full >= 0 && (waiting || (!ref_only && step - L.last_full < L.k))) {
     double ts[CAPSGPU_NTERMS];
     evaluate(c, want, ts, true, 4, L.fslow.p);
     if (want) std::memcpy(L.t_slow, ts, sizeof ts);
@@ -979,7 +991,7 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
     evaluate(c, true, ts, true, 4, L.fslow.p);
     std::memcpy(L.t_slow, ts, sizeof ts);
     // judge the prediction made from the last full step, and the stale force, against this one (after training)
-    if (L.last_full >= 0 && !training) {
+    if (L.last_full >= 0 && !training && !ref_only) {
       estimate(c, L);
       HIPCHECK(hipMemcpyAsync(L.ftot.p, c.f.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
       kt_begin(c, "k_axpy");
@@ -996,7 +1008,7 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
       HIPCHECK(hipMemcpyAsync(L.Dring[slot].p, L.Dcur.p, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
       HIPCHECK(hipMemcpyAsync(L.Fring[slot].p, L.fslow.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
       L.ring_n = std::min(L.ring_n + 1, L.k);
-    } else if (L.last_full >= 0 && step - L.last_full == L.k) {
+    } else if (!ref_only && L.last_full >= 0 && step - L.last_full == L.k) {
       train_on_pair(c, L, L.D0.p, L.f0.p);
     }
     ++L.frames;
@@ -1004,7 +1016,15 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
     std::swap(L.D0, L.Dcur);
     HIPCHECK(hipMemcpyAsync(L.f0.p, L.fslow.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     const bool was = L.learned_mode;
-    L.learned_mode = L.frames >= L.train_frames && L.scaled && (training || L.rmse <= L.tol);
+    L.learned_mode = L.frames >= L.train_frames && L.scaled && (training || (!ref_only && L.rmse <= L.tol));
+    if (L.learned_mode || training) {
+      L.wait = 0, L.ref_at = -1;
+    } else if (ref_only) {
+      L.ref_at = -1;   // judged k steps from here
+    } else {
+      L.wait = std::min(std::max(L.k, 2 * L.wait), kMaxWait);
+      L.ref_at = step + L.wait;
+    }
     L.last_full = step;
     result = (was && !L.learned_mode) ? 2 : 0;
     kt_begin(c, "k_axpy");
