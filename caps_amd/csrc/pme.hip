@@ -114,8 +114,15 @@ __device__ __forceinline__ void b_tile_rank(int bx, int gx, int n, int ntiles, c
   const int4 bi = base[i];
   const float4 wi = wq[i];
   const int64_t b = tstart[bi.w], e = tstart[bi.w + 1];
-  int64_t r = 0;
-  for (int64_t k = b; k < e; ++k) r += tmp[k] < uint32_t(i) ? 1 : 0;
+  int64_t r = 0, k = b;
+  for (; k + 8 <= e; k += 8) {   // eight loads in flight
+    uint32_t t8[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t8[u] = tmp[k + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r += t8[u] < uint32_t(i) ? 1 : 0;
+  }
+  for (; k < e; ++k) r += tmp[k] < uint32_t(i) ? 1 : 0;
   tatoms[b + r] = uint32_t(i);
   wsort[b + r] = wi;
   bsort[b + r] = bi;
@@ -750,7 +757,8 @@ __device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __
       zz -= zz >= G.k[2] ? G.k[2] : 0;
       gx -= gx >= G.k[0] ? G.k[0] : 0;
       gx -= gx >= G.k[0] ? G.k[0] : 0;
-      bv[it] = grid[(size_t(zz) * G.k[1] + yy) * G.k[0] + gx];
+      // the mesh index fits 32 bits and each product's factors 24 bits (K <= 512 per axis)
+      bv[it] = grid[__umul24(__umul24(unsigned(zz), unsigned(G.k[1])) + unsigned(yy), unsigned(G.k[0])) + unsigned(gx)];
     }
   }
 #pragma unroll

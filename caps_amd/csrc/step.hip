@@ -169,19 +169,30 @@ __device__ __forceinline__ void b_sum_forces(int bx, int gx, int n, const uint32
                                                     const double* __restrict__ slot, double* __restrict__ f) {
   const int i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // every load first (the pair and reciprocal forces, then the slots four at a time), the sums in the fixed order
+  const float4 p = fnb ? fnb[iperm[i]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float pr[3] = {0.0f, 0.0f, 0.0f};
+  if (frec)
+    for (int m = 0; m < 3; ++m) pr[m] = frec[3 * i + m];
   double Fb[3] = {0, 0, 0};
   if (aoff) {
     const int32_t e0 = aoff[i], e1 = aoff[i + 1];
-    for (int32_t e = e0; e < e1; ++e)
+    int32_t e = e0;
+    for (; e + 4 <= e1; e += 4) {
+      double v[12];
+#pragma unroll
+      for (int u = 0; u < 12; ++u) v[u] = slot[3 * size_t(e) + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        for (int m = 0; m < 3; ++m) Fb[m] += v[3 * u + m];
+    }
+    for (; e < e1; ++e)
       for (int m = 0; m < 3; ++m) Fb[m] += slot[3 * size_t(e) + m];
   }
   double F[3] = {0, 0, 0};
-  if (fnb) {
-    const float4 p = fnb[iperm[i]];
-    F[0] += double(p.x), F[1] += double(p.y), F[2] += double(p.z);
-  }
+  if (fnb) F[0] += double(p.x), F[1] += double(p.y), F[2] += double(p.z);
   if (frec)
-    for (int m = 0; m < 3; ++m) F[m] += double(frec[3 * i + m]);
+    for (int m = 0; m < 3; ++m) F[m] += double(pr[m]);
   if (aoff)
     for (int m = 0; m < 3; ++m) F[m] += Fb[m];
   for (int m = 0; m < 3; ++m) f[3 * i + m] = F[m];
