@@ -63,25 +63,49 @@ struct GridArgs {
 };
 
 // fractional coordinate s = h⁻¹ x, u = (s − ⌊s⌋) K, the first point ⌊u⌋ − p + 1 (mod K) and w = u − ⌊u⌋
+// Runs of a wave's lanes: a valid lane whose key equals the previous lane's continues that lane's run. Returns the lane's
+// place in its run and its run's first lane; the first lane also gets the run's length. Consecutive atoms (neighbours
+// along a chain) mostly share a tile, so one atomic per run replaces one per atom. Every lane of the wave must call it.
+__device__ inline int wave_run(bool valid, uint32_t key, int* head, int* len) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const uint64_t vmask = __ballot(valid);
+  const bool prev_valid = lane > 0 && ((vmask >> (lane - 1)) & 1ull);
+  const bool first = valid && !(prev_valid && prev == key);
+  const uint64_t heads = __ballot(first);
+  const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;   // lanes 0 … lane
+  const uint64_t below = heads & upto;
+  *head = below ? 63 - __builtin_clzll(below) : lane;
+  const uint64_t ends = (heads | ~vmask) & ~upto;                       // run boundaries past this lane
+  *len = (ends ? __builtin_ffsll((long long)ends) - 1 : 64) - lane;
+  return lane - *head;
+}
+
 __device__ __forceinline__ void b_splines(int bx, int gx, GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
                           int4* __restrict__ base, uint32_t* __restrict__ tcount) {
   const int i = bx * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float w[3];
-  int b[3];
-  for (int d = 0; d < 3; ++d) {
-    const double s = G.hi[d][0] * x[3 * i] + G.hi[d][1] * x[3 * i + 1] + G.hi[d][2] * x[3 * i + 2];
-    const double u = (s - floor(s)) * G.k[d];
-    const double fl = floor(u);
-    w[d] = float(u - fl);
-    int t = (int(fl) - G.p + 1) % G.k[d];
-    if (t < 0) t += G.k[d];
-    b[d] = t;
+  const bool valid = i < n;
+  int tile = 0;
+  if (valid) {
+    float w[3];
+    int b[3];
+    for (int d = 0; d < 3; ++d) {
+      const double s = G.hi[d][0] * x[3 * i] + G.hi[d][1] * x[3 * i + 1] + G.hi[d][2] * x[3 * i + 2];
+      const double u = (s - floor(s)) * G.k[d];
+      const double fl = floor(u);
+      w[d] = float(u - fl);
+      int t = (int(fl) - G.p + 1) % G.k[d];
+      if (t < 0) t += G.k[d];
+      b[d] = t;
+    }
+    tile = ((b[2] >> 2) * G.nt[1] + (b[1] >> 2)) * G.nt[0] + (b[0] >> 2);
+    wq[i] = make_float4(w[0], w[1], w[2], float(q[i]));
+    base[i] = make_int4(b[0], b[1], b[2], tile);
   }
-  const int tile = ((b[2] >> 2) * G.nt[1] + (b[1] >> 2)) * G.nt[0] + (b[0] >> 2);
-  wq[i] = make_float4(w[0], w[1], w[2], float(q[i]));
-  base[i] = make_int4(b[0], b[1], b[2], tile);
-  atomicAdd(&tcount[tile], 1u);
+  // the tile's count: one atomic per run of lanes in the same tile
+  int head, len;
+  const int pos = wave_run(valid, uint32_t(tile), &head, &len);
+  if (valid && pos == 0) atomicAdd(&tcount[tile], uint32_t(len));
 }
 __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
                           int4* __restrict__ base, uint32_t* __restrict__ tcount) {
@@ -91,9 +115,16 @@ __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const
 __device__ __forceinline__ void b_tile_fill(int bx, int gx, int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
                             uint32_t* __restrict__ tmp) {
   const int i = bx * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int t = base[i].w;
-  tmp[tstart[t] + atomicAdd(&tfill[t], 1u)] = uint32_t(i);
+  const bool valid = i < n;
+  const int t = valid ? base[i].w : 0;
+  // one atomic per run of lanes in the same tile; the run's lanes take consecutive places from its result (the order
+  // within a tile does not matter here: k_tile_rank orders each tile by atom index)
+  int head, len;
+  const int pos = wave_run(valid, uint32_t(t), &head, &len);
+  uint32_t at = 0;
+  if (valid && pos == 0) at = atomicAdd(&tfill[t], uint32_t(len));
+  at = __shfl(at, head, 64);
+  if (valid) tmp[tstart[t] + at + uint32_t(pos)] = uint32_t(i);
 }
 __global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
                             uint32_t* __restrict__ tmp) {

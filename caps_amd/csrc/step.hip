@@ -77,6 +77,9 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
     const bool fene = g >= nb;
     const int t = fene ? g - nb - na : g;
     const int32_t a = fene ? fij[2 * t] : bij[2 * t], b = fene ? fij[2 * t + 1] : bij[2 * t + 1];
+    // the members' places, requested with the atoms (they depend on the term only)
+    const size_t s0 = fene ? 2 * size_t(nb) + 3 * size_t(na) + 2 * size_t(t) : 2 * size_t(t);
+    const int32_t p0 = spos[s0], p1 = spos[s0 + 1];
     double d[3] = {x[3 * b] - x[3 * a], x[3 * b + 1] - x[3 * a + 1], x[3 * b + 2] - x[3 * a + 2]};
     min_image(C, d);
     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -104,9 +107,8 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
     double fj[3] = {0, 0, 0};
     if (ok && r > 1e-12)
       for (int m = 0; m < 3; ++m) fj[m] = d[m] * (-dEdr / r);
-    const size_t s0 = fene ? 2 * size_t(nb) + 3 * size_t(na) + 2 * size_t(t) : 2 * size_t(t);
-    double* o0 = slot + 3 * size_t(spos[s0]);
-    double* o1 = slot + 3 * size_t(spos[s0 + 1]);
+    double* o0 = slot + 3 * size_t(p0);
+    double* o1 = slot + 3 * size_t(p1);
     for (int m = 0; m < 3; ++m) o0[m] = -fj[m], o1[m] = fj[m];
     if (kE && ok) {
       A[0] += en;
@@ -115,6 +117,8 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
   } else if (g >= nb && g < nb + na) {
     const int t = g - nb;
     const int32_t ia = aijk[3 * t], ja = aijk[3 * t + 1], ka = aijk[3 * t + 2];
+    const size_t s0 = 2 * size_t(nb) + 3 * size_t(t);
+    const int32_t p0 = spos[s0], p1 = spos[s0 + 1], p2 = spos[s0 + 2];
     double u[3] = {x[3 * ia] - x[3 * ja], x[3 * ia + 1] - x[3 * ja + 1], x[3 * ia + 2] - x[3 * ja + 2]};
     double v[3] = {x[3 * ka] - x[3 * ja], x[3 * ka + 1] - x[3 * ja + 1], x[3 * ka + 2] - x[3 * ja + 2]};
     min_image(C, u);
@@ -130,10 +134,9 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
       fi[m] = (v[m] * (1 / (lu * lv)) - u[m] * (cs / (lu * lu))) * gg;
       fk[m] = (u[m] * (1 / (lu * lv)) - v[m] * (cs / (lv * lv))) * gg;
     }
-    const size_t s0 = 2 * size_t(nb) + 3 * size_t(t);
-    double* o0 = slot + 3 * size_t(spos[s0]);
-    double* o1 = slot + 3 * size_t(spos[s0 + 1]);
-    double* o2 = slot + 3 * size_t(spos[s0 + 2]);
+    double* o0 = slot + 3 * size_t(p0);
+    double* o1 = slot + 3 * size_t(p1);
+    double* o2 = slot + 3 * size_t(p2);
     for (int m = 0; m < 3; ++m) o0[m] = fi[m], o1[m] = -(fi[m] + fk[m]), o2[m] = fk[m];
     if (kE) {
       A[1] += kt * dth * dth;
