@@ -134,3 +134,9 @@ def test_learned_respa_runs_and_judges_itself():
                          evaluator=ev)
     print("tight tolerance: learned", r2.learned_steps, "full", r2.full_steps, "fallbacks", r2.fallbacks, flush=True)
     assert r2.fallbacks >= 1 and r2.full_steps > r2.learned_steps
+    # after each failed judgment the run waits 4, 8, 16, ... plain steps, takes a reference full step and judges k steps
+    # later: four judgments in these 60 steps (k + 4, + 8, + 12, + 20 after the first full step; 14 without the wait)
+    vals = [row["learned_rmse"] for row in r2.thermo]
+    changes = sum(1 for a, b in zip(vals, vals[1:]) if b != a)
+    print("tight tolerance: judgments", changes, flush=True)
+    assert changes == 4
