@@ -67,7 +67,8 @@ class Thermo(C.Structure):
 
 class DReport(C.Structure):
     _fields_ = [("steps", C.c_int64), ("seconds", C.c_double), ("ns_per_day", C.c_double), ("list_builds", C.c_int32),
-                ("prunes", C.c_int32), ("learned_steps", C.c_int32), ("full_steps", C.c_int32), ("fallbacks", C.c_int32)]
+                ("prunes", C.c_int32), ("learned_steps", C.c_int32), ("full_steps", C.c_int32), ("fallbacks", C.c_int32),
+                ("judgments", C.c_int32)]
 
 
 class BOpts(C.Structure):
@@ -442,7 +443,8 @@ class DynamicsReport:
     prunes: int
     learned_steps: int = 0
     full_steps: int = 0
-    fallbacks: int = 0
+    fallbacks: int = 0               # learned r-RESPA: judgments that missed the tolerance
+    judgments: int = 0               # learned r-RESPA: full steps that judged a k-step prediction
 
 
 _THERMOSTATS = {"none": 0, "nve": 0, "bussi": 1, "csvr": 1, "v-rescale": 1, "langevin": 2, "baoab": 2}
@@ -475,7 +477,7 @@ def run_dynamics(system, o, evaluator=None):
     x, v, _ = ev.get_state()
     system.x, system.v = x, v
     return DynamicsReport([_row(rows[k]) for k in range(min(nr.value, cap))], rep.steps, rep.seconds, rep.ns_per_day,
-                          rep.list_builds, rep.prunes, rep.learned_steps, rep.full_steps, rep.fallbacks)
+                          rep.list_builds, rep.prunes, rep.learned_steps, rep.full_steps, rep.fallbacks, rep.judgments)
 
 
 @dataclass

@@ -630,7 +630,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     scal.upload(one, s);
     const int every = std::max(1, o->thermo_every);
     int32_t nr = 0;
-    int64_t learned_steps = 0, full_steps = 0, fallbacks = 0;
+    int64_t learned_steps = 0, full_steps = 0;
     double last_rmse = 0;
     bool last_learned = false;
     Learned* LR = nullptr;
@@ -745,7 +745,6 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
         last_learned = r == 1;
         if (r == 1) ++learned_steps;
         else ++full_steps;
-        if (r == 2) ++fallbacks;
         last_rmse = learned_rmse(*LR);
       } else {
         evaluate(*c, thermo, t, true);
@@ -780,7 +779,8 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       rep->prunes = c->prunes;
       rep->learned_steps = int32_t(learned_steps);
       rep->full_steps = int32_t(full_steps);
-      rep->fallbacks = int32_t(fallbacks);
+      rep->fallbacks = LR ? int32_t(learned_failures(*LR)) : 0;
+      rep->judgments = LR ? int32_t(learned_judgments(*LR)) : 0;
     }
   });
 }

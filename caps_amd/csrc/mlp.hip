@@ -562,6 +562,8 @@ struct Learned {
   int k = 4, train_frames = 2000, frames = 0, samples = 8192, iters = 4;
   double tol = 1e-3, rmse = 0, rmse_stale = 0, e_scale = 1.0;
   int64_t last_full = -1;
+  // this run's judgments (full steps that compared the k-step prediction with PME) and how many missed the tolerance
+  int64_t judgments = 0, failures = 0;
   // after a failed judgment: plain steps for `wait` steps (doubling per failure, at most kMaxWait), then a reference
   // full step at ref_at and the judging full step k steps later
   int wait = 0;
@@ -939,6 +941,10 @@ Learned* learned_begin(capsgpu_ctx& c, const capsgpu_dyn_opts& o) {
   L.ring_n = 0;
   L.wait = 0;
   L.ref_at = -1;
+  // every run starts unjudged: its first interval runs plain PME and is decided by this run's own first judgment
+  L.learned_mode = false;
+  L.rmse = L.rmse_stale = 0;
+  L.judgments = L.failures = 0;
   (void)desc_geom(c);   // the cell must hold 5 cells of 4 Å per axis
   const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
   L.f0.resize(n3);
@@ -991,7 +997,8 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
     evaluate(c, true, ts, true, 4, L.fslow.p);
     std::memcpy(L.t_slow, ts, sizeof ts);
     // judge the prediction made from the last full step, and the stale force, against this one (after training)
-    if (L.last_full >= 0 && !training && !ref_only) {
+    const bool judged = L.last_full >= 0 && !training && !ref_only;
+    if (judged) {
       estimate(c, L);
       HIPCHECK(hipMemcpyAsync(L.ftot.p, c.f.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
       kt_begin(c, "k_axpy");
@@ -1000,6 +1007,8 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
       kt_end(c);
       L.rmse = rel_rmse(c, L, L.fest.p, L.fslow.p, L.ftot.p);
       L.rmse_stale = rel_rmse(c, L, L.f0.p, L.fslow.p, L.ftot.p);
+      ++L.judgments;
+      if (!(L.rmse <= L.tol)) ++L.failures;
     }
     // train on the pair (k steps ago, now): from the ring while every step is full, else from the last full step
     if (training) {
@@ -1015,18 +1024,20 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
     // the reference point of the next interval (the current descriptors become D0; Dcur is rewritten every step)
     std::swap(L.D0, L.Dcur);
     HIPCHECK(hipMemcpyAsync(L.f0.p, L.fslow.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    const bool was = L.learned_mode;
-    L.learned_mode = L.frames >= L.train_frames && L.scaled && (training || (!ref_only && L.rmse <= L.tol));
-    if (L.learned_mode || training) {
+    // the next interval is predicted only after a judgment of this run that met the tolerance; a full step with no
+    // judgment (training, the run's first full step, a reference point after a wait) is followed by plain steps up to
+    // the judging full step k steps later
+    L.learned_mode = L.frames >= L.train_frames && L.scaled && judged && L.rmse <= L.tol;
+    if (L.learned_mode) {
       L.wait = 0, L.ref_at = -1;
-    } else if (ref_only) {
-      L.ref_at = -1;   // judged k steps from here
+    } else if (training || !judged) {
+      L.ref_at = -1;   // plain steps, then the judging full step k steps from here
     } else {
       L.wait = std::min(std::max(L.k, 2 * L.wait), kMaxWait);
       L.ref_at = step + L.wait;
     }
     L.last_full = step;
-    result = (was && !L.learned_mode) ? 2 : 0;
+    result = 0;
     kt_begin(c, "k_axpy");
     hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n3, 256)), dim3(256), 0, s, int64_t(n3), L.fslow.p, c.f.p);
     LAUNCHCHECK();
@@ -1045,6 +1056,8 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
 }
 
 double learned_rmse(const Learned& L) { return L.rmse; }
+int64_t learned_judgments(const Learned& L) { return L.judgments; }
+int64_t learned_failures(const Learned& L) { return L.failures; }
 double learned_rmse_stale(const Learned& L) { return L.rmse_stale; }
 void learned_end(capsgpu_ctx&, Learned*) {}
 void learned_free(Learned* L) { delete L; }

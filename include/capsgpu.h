@@ -144,7 +144,9 @@ typedef struct capsgpu_dyn_report {
   int64_t steps;
   double seconds, ns_per_day;
   int32_t list_builds, prunes;
-  int32_t learned_steps, full_steps, fallbacks;
+  int32_t learned_steps, full_steps;
+  int32_t fallbacks;            /* learned r-RESPA: judgments that missed learned_tol (each sends the run back to plain PME) */
+  int32_t judgments;            /* learned r-RESPA: full steps that judged a k-step prediction against PME */
 } capsgpu_dyn_report;
 
 /* run_dynamics (dynamics.cpp:71-815) from the set state: velocity Verlet + Bussi, Langevin BAOAB, NVE, r-RESPA; the

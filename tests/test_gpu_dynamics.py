@@ -59,6 +59,20 @@ def test_langevin_is_reproducible_and_holds_the_temperature():
     assert abs(T[len(T) // 2:].mean() - 300) < 30
 
 
+@pytest.mark.parametrize("thermostat", ["bussi", "langevin"])
+def test_thermostats_hold_the_temperature(thermostat):
+    # tests/test_dynamics.cpp:120-135: 400 K, tau_t 50 fs, 4000 steps, thermo every 10; the mean temperature past row 50
+    # within 16 K of the target. Bonded terms + PME's reciprocal part on a 1,024-atom melt.
+    print("case thermostat %s 400 K" % thermostat, flush=True)
+    s = H.melt_case(chains=8, seed=21)
+    op = ca.EnergyOptions(cutoff=8.0, coulomb=True, electrostatics="pme", pme_order=4, pme_spacing=1.0, tail=False, parts=5)
+    r = ca.run_dynamics(s, ca.DynamicsOptions(dt=0.5, steps=4000, temperature=400, thermostat=thermostat, tau_t=50, seed=11,
+                                              thermo_every=10, new_velocities=True, energy=op))
+    T = np.array([row["temperature"] for row in r.thermo])
+    print("%s: mean T past row 50 = %.2f K over %d rows" % (thermostat, T[50:].mean(), len(T) - 50), flush=True)
+    assert abs(T[50:].mean() - 400.0) < 16.0
+
+
 def test_state_round_trip():
     s = H.melt_case(chains=2)
     print("case state round trip", flush=True)
