@@ -112,6 +112,7 @@ def lib():
         L.capsgpu_create.argtypes = [C.POINTER(FF), C.POINTER(EOpts), C.POINTER(C.c_void_p)]
         L.capsgpu_destroy.argtypes = [C.c_void_p]
         L.capsgpu_compute.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+        L.capsgpu_compute_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.capsgpu_list_builds.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
         L.capsgpu_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.capsgpu_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]

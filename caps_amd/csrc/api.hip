@@ -6,6 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 
@@ -17,6 +20,10 @@
 #include "step.hpp"
 
 using namespace capsgpu;
+
+// the allocation-free evaluation's device scratch (one copy per device, held until the process ends)
+constexpr size_t kScratchBytes = size_t(96) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 thread_local std::string g_error;
@@ -267,6 +274,91 @@ void upload_cell(capsgpu_ctx& c, const double* cell) {
   if (!(cell_volume(cell) > 1e-9)) throw Error(CAPSGPU_EINVAL, "the cell must have a volume (periodic along a, b and c)");
 }
 
+// Evaluator::compute on the device state: forces and terms at the host positions x (field.cpp:1249-2539)
+void compute_body(capsgpu_ctx* c, const double* x, const double cell[12], double* f, double* terms, int32_t want) {
+  upload_cell(*c, cell);
+  for (size_t k = 0; k < 3 * size_t(c->n); ++k)
+    if (!std::isfinite(x[k])) throw Error(CAPSGPU_EFIELD, "positions are not finite numbers");
+  c->x.upload(x, 3 * size_t(c->n), c->stream);
+  c->state_set = true;
+  c->timing = true;
+  c->flags.zero(c->stream);
+  std::memset(c->h_flags, 0, 4 * sizeof(uint32_t));
+  double t[CAPSGPU_NTERMS];
+  evaluate(*c, want != 0 || terms != nullptr, t, false);
+  read_flags(*c);
+  if (c->h_flags[2]) {
+    c->flags.zero(c->stream);
+    throw Error(CAPSGPU_EFIELD, "a FENE bond is stretched to R0 or past; lower the time step or relax the structure first");
+  }
+  if (f) c->f.download(f, 3 * size_t(c->n), c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms[6];
+  for (int k = 0; k < 6; ++k) HIPCHECK(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
+  for (int k = 0; k < 6; ++k) c->stage_ms[k] = ms[k];
+  c->timing = false;
+  if (terms) std::memcpy(terms, t, sizeof t);
+}
+
+// The allocation-free evaluation's device memory: one block per device, holding the per-evaluation scratch of the
+// bonded terms and PME's reciprocal part (mesh, spectrum, influence function, spline and tile arrays, member slots,
+// partial sums). Its device address is looked up once per device; calls on one device share it, so they are serialised
+// here, and calls from elsewhere that overlap one of them on the same device race.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// the scratch block's address on device dev (that device made current for the lookup, the caller's restored)
+unsigned char* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw Error(CAPSGPU_EDEVICE, "device index out of range");
+  if (!g_scratch[dev]) {
+    int cur = 0;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHECK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHECK(hipSetDevice(cur));
+    HIPCHECK(e);
+    if (!p) throw Error(CAPSGPU_EDEVICE, "no device address for the scratch block");
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch[dev];
+}
+
+// For one evaluation: the context's per-evaluation arrays handed to the arena (their sizes kept, their contents not:
+// the cached PME tables are rebuilt and the tile counters cleared), and given back afterwards, the PME set-up then
+// redone for the owned arrays too.
+class ScratchBinding {
+ public:
+  ScratchBinding(capsgpu_ctx& c, Arena& a) : c_(c) {
+    Pme& P = c.pme;
+    bind(P.grid, a), bind(P.half, a), bind(P.gtab, a), bind(P.bmod, a), bind(P.theta, a), bind(P.base, a), bind(P.home, a);
+    bind(P.tile, a), bind(P.tcount, a), bind(P.tfill, a), bind(P.tatoms, a), bind(P.wsort, a), bind(P.bsort, a);
+    bind(P.tstart, a), bind(P.fpme, a), bind(P.tw, a), bind(P.epart, a);
+    bind(c.bonded.slot, a), bind(c.bonded.epart, a), bind(c.partials, a);
+    for (int d = 0; d < 4; ++d) bind(c.scan.tsum[d], a), bind(c.scan.toff[d], a);
+    P.green_ok = false, P.counts_clear = false;
+  }
+  ~ScratchBinding() {
+    for (auto it = undo_.rbegin(); it != undo_.rend(); ++it) (*it)();
+    c_.pme.green_ok = false, c_.pme.counts_clear = false;
+  }
+  ScratchBinding(const ScratchBinding&) = delete;
+  ScratchBinding& operator=(const ScratchBinding&) = delete;
+
+ private:
+  template <class T>
+  void bind(DevBuf<T>& d, Arena& a) {
+    auto owned = std::make_shared<DevBuf<T>>();
+    owned->swap(d);
+    d.arena = &a;
+    d.resize(owned->n);
+    undo_.push_back([&d, owned] { d.swap(*owned); });
+  }
+  capsgpu_ctx& c_;
+  std::vector<std::function<void()>> undo_;
+};
+
 }  // namespace
 
 capsgpu_ctx::~capsgpu_ctx() {
@@ -301,6 +393,7 @@ int32_t capsgpu_create(const capsgpu_ff* ff, const capsgpu_energy_opts* o, capsg
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(CAPSGPU_EDEVICE, "no GPU visible to HIP");
     auto c = std::make_unique<capsgpu_ctx>();
     c->n = n, c->nt = nt, c->opt = *o;
+    HIPCHECK(hipGetDevice(&c->device));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flags), 4 * sizeof(uint32_t), hipHostMallocDefault));
@@ -426,30 +519,40 @@ int32_t capsgpu_get_forces(capsgpu_ctx* c, double* f) {
 int32_t capsgpu_compute(capsgpu_ctx* c, const double* x, const double cell[12], double* f, double* terms, int32_t want) {
   return guard([&] {
     if (!c || !x || !cell) throw Error(CAPSGPU_EINVAL, "null argument");
-    upload_cell(*c, cell);
-    for (size_t k = 0; k < 3 * size_t(c->n); ++k)
-      if (!std::isfinite(x[k])) throw Error(CAPSGPU_EFIELD, "positions are not finite numbers");
-    c->x.upload(x, 3 * size_t(c->n), c->stream);
-    c->state_set = true;
-    c->timing = true;
-    c->flags.zero(c->stream);
-    std::memset(c->h_flags, 0, 4 * sizeof(uint32_t));
-    double t[CAPSGPU_NTERMS];
-    evaluate(*c, want != 0 || terms != nullptr, t, false);
-    read_flags(*c);
-    if (c->h_flags[2]) {
-      c->flags.zero(c->stream);
-      throw Error(CAPSGPU_EFIELD, "a FENE bond is stretched to R0 or past; lower the time step or relax the structure first");
-    }
-    if (f) c->f.download(f, 3 * size_t(c->n), c->stream);
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    float ms[6];
-    for (int k = 0; k < 6; ++k) HIPCHECK(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
-    for (int k = 0; k < 6; ++k) c->stage_ms[k] = ms[k];
-    c->timing = false;
-    if (terms) std::memcpy(terms, t, sizeof t);
+    compute_body(c, x, cell, f, terms, want);
   });
 }
+
+int32_t capsgpu_compute_noalloc(capsgpu_ctx* c, const double* x, const double cell[12], double* f, double* terms, int32_t want) {
+  return guard([&] {
+    if (!c || !x || !cell) throw Error(CAPSGPU_EINVAL, "null argument");
+    if (c->opt.parts & 2)
+      throw Error(CAPSGPU_EINVAL, "the allocation-free evaluation takes the bonded terms and PME's reciprocal part (parts 1, 4 or 5)");
+    if (!(cell_volume(cell) > 1e-9)) throw Error(CAPSGPU_EINVAL, "the cell must have a volume (periodic along a, b and c)");
+    // the scratch this evaluation needs at most, checked before any work: mesh, spectrum and influence function, the
+    // per-atom spline, tile and force arrays, the bonded terms' member slots, the reductions
+    const size_t n = size_t(c->n);
+    size_t need = 3 * size_t(c->bonded.nb * 2 + c->bonded.na * 3 + c->bonded.nf * 2 + 1) * sizeof(double) + 64 * 1024;
+    if (pme_on(*c)) {
+      int k[3];
+      pme_grid_dims(*c, cell, k);
+      const size_t k3 = size_t(k[0]) * k[1] * k[2], nh = size_t(k[0] / 2 + 1) * k[1] * k[2];
+      const size_t tiles = size_t((k[0] + 3) / 4) * ((k[1] + 3) / 4) * ((k[2] + 3) / 4);
+      need += 4 * k3 + 12 * nh + 16 * size_t(k[0] + k[1] + k[2]) + 12 * tiles + 60 * std::max({k[0], k[1], k[2]}) + 84 * n;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    unsigned char* base = scratch_on(c->device);
+    if (need > kScratchBytes)
+      throw Error(CAPSGPU_EINVAL, "the allocation-free evaluation holds " + std::to_string(kScratchBytes >> 20) + " MiB of device scratch; this "
+                                  "system needs about " + std::to_string((need + (1u << 20) - 1) >> 20) + " MiB: use capsgpu_compute");
+    Arena arena{base, kScratchBytes, 0};
+    {
+      ScratchBinding bind(*c, arena);
+      compute_body(c, x, cell, f, terms, want);
+    }
+  });
+}
+
 
 int32_t capsgpu_list_builds(capsgpu_ctx* c, int32_t* builds) {
   return guard([&] {

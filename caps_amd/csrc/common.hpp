@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace capsgpu {
@@ -42,25 +43,50 @@ __global__ void k_zero_bytes(unsigned char* __restrict__ p, size_t nbytes) {
 }
 }  // namespace
 
-// a device array that owns its memory and grows (never shrinks) on demand
+// A fixed block of device memory handed out front to back (256-byte aligned pieces), never freed piecewise: the
+// memory of evaluations that may not allocate. Taking more than it holds is an error that names the limit.
+struct Arena {
+  unsigned char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~size_t(255);
+    if (at > cap || bytes > cap - at)
+      throw Error(-1, "the allocation-free evaluation needs more than its " + std::to_string(cap >> 20) +
+                          " MiB of device scratch memory (" + std::to_string((at + bytes + (1u << 20) - 1) >> 20) +
+                          " MiB asked so far): use a smaller system or the allocating entry point");
+    used = at + bytes;
+    return base + at;
+  }
+};
+
+// a device array that owns its memory and grows (never shrinks) on demand; bound to an Arena it takes its memory
+// from there instead and frees nothing
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0, cap = 0;
+  Arena* arena = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), cap(o.cap) { o.p = nullptr, o.n = o.cap = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), cap(o.cap), arena(o.arena) { o.p = nullptr, o.n = o.cap = 0, o.arena = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
-      p = o.p, n = o.n, cap = o.cap;
-      o.p = nullptr, o.n = o.cap = 0;
+      p = o.p, n = o.n, cap = o.cap, arena = o.arena;
+      o.p = nullptr, o.n = o.cap = 0, o.arena = nullptr;
     }
     return *this;
   }
   ~DevBuf() { release(); }
+  void swap(DevBuf& o) noexcept {
+    std::swap(p, o.p), std::swap(n, o.n), std::swap(cap, o.cap), std::swap(arena, o.arena);
+  }
   void release() {
+    if (arena) {
+      p = nullptr, n = cap = 0;
+      return;
+    }
 #ifdef CAPSGPU_DEBUG_GUARD
     if (p) guard_free(p);
 #else
@@ -71,7 +97,10 @@ struct DevBuf {
   }
   // size n; contents are kept only when no reallocation happens
   void resize(size_t m) {
-    if (m > cap) {
+    if (m > cap && arena) {
+      p = static_cast<T*>(arena->take(m * sizeof(T)));
+      cap = m;
+    } else if (m > cap) {
       release();
 #ifdef CAPSGPU_DEBUG_GUARD
       const size_t c = m ? m : 1;

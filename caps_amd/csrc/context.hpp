@@ -122,6 +122,7 @@ struct Bonded {
 
 struct capsgpu_ctx {
   int n = 0, nt = 0;
+  int device = 0;                       // the HIP device current at creation: the stream's and every array's
   capsgpu_energy_opts opt{};
   hipStream_t stream = nullptr;
   // force field
@@ -185,6 +186,8 @@ void list_prune(capsgpu_ctx& c, double radius);
 void pair_forces(capsgpu_ctx& c, bool want);
 // pme.hip: reciprocal forces into pme.fpme (atom order); energy and virial partials when want
 void pme_setup(capsgpu_ctx& c);
+// the mesh pme_setup picks for a cell (points per axis), without setting anything up
+void pme_grid_dims(const capsgpu_ctx& c, const double cell[12], int k[3]);
 void pme_forces(capsgpu_ctx& c, bool want);
 // picks the transforms' tile shape by timing (out: line pairs along a, lines along b and c per workgroup)
 void pme_tune(capsgpu_ctx& c, int out[2]);

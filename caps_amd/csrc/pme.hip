@@ -920,6 +920,26 @@ FftPlan plan_of(const Pme& P) {
 
 }  // namespace
 
+// the smallest 2-3-5-7 size of at least n points (kspace.cpp:45-52, f90:23-36)
+static int good_fft_size(int n) {
+  int m = std::max(n, 1);
+  for (;;) {
+    int k = m;
+    for (int f : {2, 3, 5, 7})
+      while (k % f == 0) k /= f;
+    if (k == 1) return m;
+    ++m;
+  }
+}
+
+void pme_grid_dims(const capsgpu_ctx& c, const double cell[12], int k[3]) {
+  for (int d = 0; d < 3; ++d) {
+    const double* e = cell + 3 + 3 * d;
+    const double len = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    k[d] = good_fft_size(std::max(c.opt.pme_order + 1, int(std::ceil(len / c.opt.pme_spacing))));
+  }
+}
+
 // the grid, its tables and the influence function for the current cell (kept while the cell stays the same)
 void pme_setup(capsgpu_ctx& c) {
   Pme& P = c.pme;
@@ -943,23 +963,10 @@ void pme_setup(capsgpu_ctx& c) {
   P.hi[2][1] = (h[0][1] * h[2][0] - h[0][0] * h[2][1]) / v;
   P.hi[2][2] = (h[0][0] * h[1][1] - h[0][1] * h[1][0]) / v;
   P.vol = std::fabs(v);
-  // grid: the smallest 2-3-5-7 sizes with spacing at most pme_spacing (kspace.cpp:45-52, f90:23-36)
-  auto good = [](int n) {
-    int m = std::max(n, 1);
-    for (;;) {
-      int k = m;
-      for (int f : {2, 3, 5, 7})
-        while (k % f == 0) k /= f;
-      if (k == 1) return m;
-      ++m;
-    }
-  };
-  for (int d = 0; d < 3; ++d) {
-    const double* e = cell + 3 + 3 * d;
-    const double len = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-    P.k[d] = good(std::max(c.opt.pme_order + 1, int(std::ceil(len / c.opt.pme_spacing))));
+  // grid: the smallest 2-3-5-7 sizes with spacing at most pme_spacing
+  pme_grid_dims(c, cell, P.k);
+  for (int d = 0; d < 3; ++d)
     if (P.k[d] > kMaxFft) throw Error(-1, "PME grid larger than 512 points along an axis");
-  }
   P.order = c.opt.pme_order;
   if (P.order < 3 || P.order > kMaxOrder) throw Error(-1, "PME order must be 3 … 8");
   P.h1 = P.k[0] / 2 + 1;

@@ -96,6 +96,18 @@ int32_t capsgpu_destroy(capsgpu_ctx* ctx);
 int32_t capsgpu_compute(capsgpu_ctx* ctx, const double* x, const double cell[12], double* f, double* terms,
                         int32_t want_energies);
 
+/* capsgpu_compute without any device allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc). The per-evaluation scratch (PME mesh,
+ * spectrum and influence function, spline and tile arrays, bonded member slots, partial sums) comes from one 96 MiB
+ * block of device memory per GPU, held from the first call on that GPU until the process ends; the context's own
+ * state (positions, forces, force field), allocated by capsgpu_create, is used as capsgpu_compute uses it. Takes
+ * contexts with the bonded terms and PME's reciprocal part (parts 1, 4 or 5) and rejects, with CAPSGPU_EINVAL and a
+ * message and before any work, a system whose scratch would exceed the block. Calls on one GPU share
+ * the block: this library serialises its own calls, and calls that overlap them from elsewhere on the same GPU race.
+ * Results are bit-identical to capsgpu_compute's. */
+int32_t capsgpu_compute_noalloc(capsgpu_ctx* ctx, const double* x, const double cell[12], double* f, double* terms,
+                                int32_t want_energies);
+
 /* Evaluator::list_builds() (field.hpp:338) */
 int32_t capsgpu_list_builds(capsgpu_ctx* ctx, int32_t* builds);
 
