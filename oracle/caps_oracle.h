@@ -9,9 +9,9 @@
  * forms (Madelung constant, DFT, the tail integral), finite differences (pair, bonded and reciprocal forces, the virial
  * as the strain derivative, the MLP gradients), invariants (every pair within r_c listed once, momentum, temperature,
  * KG bond lengths and angles) and published known-answer values (mt19937_64's 10000th output).
- * Parity unpinned: the exact draw order of orc_initial_velocities (dynamics.cpp:236-256) and of orc_kremer_grest
- * (kremer_grest.cpp:15-83) — their distributions' streams are pinned, but the reference functions that consume them
- * link the whole core and do not build here, so only their invariants are checked.
+ * Draw orders: orc_kremer_grest is bit-exact against the reference's builder (kremer_grest.cpp:15-83, built alone by
+ * this directory's Makefile; tests/golden/kg.npz); orc_initial_velocities consumes the reference's own normal stream
+ * (tests/golden/rng.npz) in the order of dynamics.cpp:231-247 (that function links the whole core and is not built).
  *
  * Conventions: a cell is double[12] = origin(3), a(3), b(3), c(3), fully periodic; positions are 3N doubles (Å);
  * forces 3N doubles (kcal/mol/Å); energies kcal/mol; the virial tensor w[6] = xx yy zz xy xz yz (Σ r_a f_b, symmetrised).

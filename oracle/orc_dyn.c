@@ -6,8 +6,9 @@
  *   spare kept, Marsaglia–Tsang gamma). Pinned by the standard's known answer (the 10000th output of the default seed)
  *   and by moments of the distributions.
  * orc_kinetic_energy, orc_initial_velocities, orc_bussi_alpha: core/src/dynamics.cpp:65-69, 236-256, 466-477.
- * orc_kremer_grest: core/src/kremer_grest.cpp:15-83. Pinned by the reference's own invariants
- *   (tests/test_provenance.cpp:315-345): every bond 0.97, no fold past 120°, box ∛(MN/ρ).
+ * orc_kremer_grest: core/src/kremer_grest.cpp:15-83. Pinned bit for bit against the reference builder's positions
+ *   (tests/golden/kg.npz) and by its own invariants (tests/test_provenance.cpp:315-345): every bond 0.97, no fold past
+ *   120°, box ∛(MN/ρ).
  * orc_mlp_*: the 96-128-128-3 SiLU network of the learned r-RESPA (no reference counterpart), pinned by central finite
  *   differences of its loss.
  */

@@ -4,6 +4,7 @@ ref, which compiles the sources where they lie under /root/reference). Inputs an
     python tests/golden/make_golden.py
 """
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -50,6 +51,14 @@ def main():
     bus = np.zeros(2 * 100)
     O.ref().ref_rng_bussi(77, 3.0 * 512 - 3, 100, bus)
     np.savez_compressed(os.path.join(HERE, "rng.npz"), normal_seed=123, normal=nrm, bussi_seed=77, bussi_ndof=3.0 * 512 - 3, bussi=bus)
+    # the Kremer-Grest builder's bead positions (kremer_grest.cpp:15-83): flexible and with bending stiffness
+    kg = {}
+    for tag, args in (("flexible", (20, 50, 0.85, 0.0, 0.97, 1)), ("stiff", (10, 30, 0.85, 1.5, 0.97, 3))):
+        raw = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "kg_golden")] + [str(a) for a in args], capture_output=True,
+                             check=True).stdout
+        kg[tag + "_args"] = np.array(args, np.float64)
+        kg[tag + "_x"] = np.frombuffer(raw, np.float64).reshape(-1, 3)
+    np.savez_compressed(os.path.join(HERE, "kg.npz"), **kg)
     print("golden vectors written to", HERE)
 
 

@@ -68,3 +68,35 @@ def test_random_streams_match_the_golden_vectors():
     for _ in range(len(d["bussi"]) // 2):
         got += [r.normal(), r.gamma((nd - 1) / 2, 2.0)]
     assert np.array_equal(np.array(got), d["bussi"])
+
+
+@pytest.mark.parametrize("tag", ["flexible", "stiff"])
+def test_kremer_grest_chains_match_the_golden_vectors(tag):
+    # the reference builder's bead positions (kremer_grest.cpp:15-83, built alone by oracle/Makefile): the oracle's
+    # restatement and the library's host builder draw the same mt19937_64 stream in the same order, bit for bit
+    d = _load("kg.npz")
+    chains, beads, density, k_theta, bond, seed = d[tag + "_args"]
+    ref = d[tag + "_x"]
+    x = np.zeros(ref.size)
+    O.orc().orc_kremer_grest(int(chains), int(beads), density, k_theta, bond, int(seed), x)
+    assert np.array_equal(x.reshape(-1, 3), ref)
+    s = ca.kremer_grest(chains=int(chains), beads=int(beads), density=density, k_theta=k_theta, bond=bond, seed=int(seed))
+    assert np.array_equal(s.x, ref)
+
+
+def test_initial_velocities_draw_the_reference_stream_in_atom_order():
+    # dynamics.cpp:207-208, 231-247: v[3i + k] = sd_i · gauss(rng) for i, then k, from the run's mt19937_64(seed) and
+    # caps::Normal; the centre-of-mass momentum removed; the whole set scaled to exactly T over ndof = 3N - 3. The draws are
+    # the reference's own normal stream of seed 123 (golden), so the order of consumption is what is pinned here.
+    d = _load("rng.npz")
+    g = d["normal"]
+    n, T = 600, 300.0
+    m = 1.0 + 15.0 * (np.arange(n) % 5) / 4.0
+    kB, kAcc = 0.0019872041, 4.184e-4
+    v = np.sqrt(kB * T * kAcc / m)[:, None] * g[: 3 * n].reshape(n, 3)
+    v -= (m[:, None] * v).sum(0) / m.sum()
+    t0 = (m[:, None] * v * v).sum() / kAcc / ((3 * n - 3) * kB)
+    v *= np.sqrt(T / t0)
+    out = np.zeros(3 * n)
+    O.orc().orc_initial_velocities(O.Rng(int(d["normal_seed"])).ptr(), m, n, T, out)
+    assert np.allclose(out.reshape(n, 3), v, rtol=1e-12, atol=1e-15)
