@@ -55,14 +55,15 @@ class DOpts(C.Structure):
     _fields_ = [("dt", C.c_double), ("steps", C.c_int64), ("temperature", C.c_double), ("thermostat", C.c_int32),
                 ("tau_t", C.c_double), ("seed", C.c_uint64), ("thermo_every", C.c_int32), ("new_velocities", C.c_int32),
                 ("respa", C.c_int32), ("step_offset", C.c_int64), ("learned_k", C.c_int32), ("learned_tol", C.c_double),
-                ("learned_train", C.c_int32)]
+                ("learned_train", C.c_int32), ("barostat", C.c_int32), ("pressure", C.c_double), ("tau_p", C.c_double),
+                ("compressibility", C.c_double), ("barostat_every", C.c_int32)]
 
 
 class Thermo(C.Structure):
     _fields_ = [("step", C.c_int64), ("time_ps", C.c_double), ("temperature", C.c_double), ("kinetic", C.c_double),
                 ("potential", C.c_double), ("total", C.c_double), ("pressure", C.c_double), ("conserved", C.c_double),
                 ("terms", C.c_double * NTERMS), ("learned", C.c_int32), ("learned_rmse", C.c_double),
-                ("learned_rmse_stale", C.c_double)]
+                ("learned_rmse_stale", C.c_double), ("volume", C.c_double), ("density", C.c_double), ("p", C.c_double * 6)]
 
 
 class DReport(C.Structure):
@@ -432,6 +433,11 @@ class DynamicsOptions:
     learned_k: int = 0                # learned r-RESPA: a full PME evaluation every k steps (0: off)
     learned_tol: float = 1e-3         # relative force RMSE above which the next interval runs plain PME
     learned_train: int = 2000         # PME frames the network trains on before it predicts
+    barostat: str = "none"            # none, berendsen, c-rescale (isotropic; dynamics.cpp:701-723)
+    pressure: float = 1.0             # atm
+    tau_p: float = 1000.0             # fs
+    compressibility: float = 4.5e-5   # 1/atm
+    barostat_every: int = 10          # steps between cell updates
 
 
 @dataclass
@@ -449,6 +455,7 @@ class DynamicsReport:
 
 
 _THERMOSTATS = {"none": 0, "nve": 0, "bussi": 1, "csvr": 1, "v-rescale": 1, "langevin": 2, "baoab": 2}
+_BAROSTATS = {"none": 0, "berendsen": 1, "c-rescale": 2, "crescale": 2}
 
 
 def run_dynamics(system, o, evaluator=None):
@@ -464,6 +471,10 @@ def run_dynamics(system, o, evaluator=None):
     d.thermostat, d.tau_t, d.seed = _THERMOSTATS[o.thermostat.lower()], o.tau_t, o.seed
     d.thermo_every, d.new_velocities, d.respa, d.step_offset = o.thermo_every, int(not has_v), o.respa, o.step_offset
     d.learned_k, d.learned_tol, d.learned_train = o.learned_k, o.learned_tol, o.learned_train
+    if o.barostat.lower() not in _BAROSTATS:
+        raise ValueError("unknown barostat '%s' (none, berendsen, c-rescale)" % o.barostat)
+    d.barostat, d.pressure, d.tau_p = _BAROSTATS[o.barostat.lower()], o.pressure, o.tau_p
+    d.compressibility, d.barostat_every = o.compressibility, o.barostat_every
     cap = int(o.steps // max(1, o.thermo_every) + 2)
     rows = (Thermo * cap)()
     nr = C.c_int32()
@@ -475,8 +486,8 @@ def run_dynamics(system, o, evaluator=None):
         cb = PROGRESS_FN(_cb)
     _check(lib().capsgpu_run_dynamics(ev._h, C.byref(d), C.cast(cb, C.c_void_p) if cb else None, None, C.cast(rows, C.c_void_p), cap,
                                       C.byref(nr), C.byref(rep)))
-    x, v, _ = ev.get_state()
-    system.x, system.v = x, v
+    x, v, cell = ev.get_state()
+    system.x, system.v, system.cell = x, v, cell
     return DynamicsReport([_row(rows[k]) for k in range(min(nr.value, cap))], rep.steps, rep.seconds, rep.ns_per_day,
                           rep.list_builds, rep.prunes, rep.learned_steps, rep.full_steps, rep.fallbacks, rep.judgments)
 
@@ -564,7 +575,8 @@ class Batch:
 def _row(r):
     return dict(step=r.step, time_ps=r.time_ps, temperature=r.temperature, kinetic=r.kinetic, potential=r.potential, total=r.total,
                 pressure=r.pressure, conserved=r.conserved, terms=EnergyTerms.from_array(r.terms), learned=r.learned,
-                learned_rmse=r.learned_rmse, learned_rmse_stale=r.learned_rmse_stale)
+                learned_rmse=r.learned_rmse, learned_rmse_stale=r.learned_rmse_stale, volume=r.volume, density=r.density,
+                p=tuple(r.p))
 
 
 # ---- synthetic systems ------------------------------------------------------------------------------------------------

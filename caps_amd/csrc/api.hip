@@ -698,6 +698,11 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     if (o->thermostat < 0 || o->thermostat > 2) throw Error(CAPSGPU_EINVAL, "thermostat must be 0 (none), 1 (Bussi) or 2 (Langevin)");
     if (o->learned_k > 1 && (o->respa > 1)) throw Error(CAPSGPU_EINVAL, "learned r-RESPA replaces the bonded/non-bonded split");
     if (o->learned_k > 1 && !pme_on(*c)) throw Error(CAPSGPU_EINVAL, "learned r-RESPA needs PME");
+    const int baro = o->barostat;
+    if (baro < 0 || baro > 2) throw Error(CAPSGPU_EINVAL, "barostat must be 0 (none), 1 (Berendsen) or 2 (C-rescale)");
+    if (baro && o->learned_k > 1) throw Error(CAPSGPU_EINVAL, "learned r-RESPA runs at a fixed cell: no barostat");
+    if (baro && !(o->tau_p > 0 && o->compressibility > 0)) throw Error(CAPSGPU_EINVAL, "the barostat needs a positive tau_p and compressibility");
+    const int bevery = o->barostat_every > 0 ? o->barostat_every : 10;
     if (nrows) *nrows = 0;
     const int n = c->n;
     hipStream_t s = c->stream;
@@ -732,6 +737,9 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     std::vector<double> one = {1.0, 0.0};
     scal.upload(one, s);
     const int every = std::max(1, o->thermo_every);
+    double mtot = 0;
+    for (double m : c->mass) mtot += m;
+    double baro_bath = 0;   // the energy the stochastic cell rescaling took out of the kinetic energy (the conserved sum)
     int32_t nr = 0;
     int64_t learned_steps = 0, full_steps = 0;
     double last_rmse = 0;
@@ -755,7 +763,10 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       r.total = r.potential + r.kinetic;
       const double V = cell_volume(c->cell);
       r.pressure = (2 * r.kinetic + t[CAPSGPU_T_VIRIAL]) / (3 * V) * kAtm;
-      r.conserved = r.total + hb[1];
+      r.conserved = r.total + hb[1] + baro_bath;
+      r.volume = V;
+      r.density = mtot / 6.02214076e23 / (V * 1e-24);
+      for (int q = 0; q < 6; ++q) r.p[q] = (c->h_terms[41 + q] / kAcc + t[CAPSGPU_T_WXX + q]) / V * kAtm;
       r.learned = last_learned ? 1 : 0;
       r.learned_rmse = last_rmse;
       r.learned_rmse_stale = LR ? learned_rmse_stale(*LR) : 0.0;
@@ -791,6 +802,31 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       if (c->h_flags[2]) throw Error(CAPSGPU_EFIELD, "a FENE bond is stretched to R0 or past; lower the time step or relax the structure first");
     };
     double t[CAPSGPU_NTERMS];
+    // the barostat after a step (dynamics.cpp:701-723): the pressure from the kinetic energy after the step and the
+    // virial of the step's forces; the cell and the positions (and, for stochastic cell rescaling, the velocities)
+    // scaled about the cell origin; then the forces at the new positions and cell
+    auto barostat_update = [&](bool thermo) {
+      kick(*c, 0.0, true, kpart.p);
+      colsum(*c, kpart.p, nb, 7, c->terms.p + 48);
+      double kin = 0;
+      HIPCHECK(hipMemcpyAsync(&kin, c->terms.p + 48, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      const double V = cell_volume(c->cell), hp = dt * bevery;
+      const double p = (2 * kin + t[CAPSGPU_T_VIRIAL]) / (3 * V) * kAtm;
+      double deps = -o->compressibility / o->tau_p * (o->pressure - p) * hp;
+      if (baro == 2) {
+        const double ktv = kBoltz * o->temperature / V * kAtm;
+        deps += o->compressibility / o->tau_p * ktv * hp + std::sqrt(2 * ktv * o->compressibility * hp / o->tau_p) * rng.normal();
+      }
+      deps = std::clamp(deps, -0.01, 0.01);
+      const double mu = std::exp(deps / 3);
+      scale_xv(*c, c->cell, mu, baro == 2 ? 1.0 / mu : 1.0);
+      if (baro == 2) baro_bath -= kin / (mu * mu) - kin;
+      for (int k = 3; k < 12; ++k) c->cell[k] *= mu;
+      c->L.built = false;
+      full_eval(thermo, t, false);
+      if (thermo) kick(*c, 0.0, true, kpart.p);
+    };
     c->h_flags[0] = c->h_flags[1] = 0;
     if (!listed) c->flags.zero(s);
     full_eval(true, t, false);
@@ -802,6 +838,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     bool alpha_pending = false;
     for (int64_t step = 1; step <= o->steps; ++step) {
       const bool thermo = step % every == 0 || step == o->steps;
+      const bool bstep = baro && step % bevery == 0;
       if (listed) c->flags.zero(s);
       if (R > 1) {
         const double h = dt / R, hkh = 0.5 * h * kAcc;
@@ -815,7 +852,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
               read_flags(*c);
               if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable at step " + std::to_string(step));
             }
-            full_eval(thermo, t, true);
+            full_eval(thermo || bstep, t, true);
           }
           kick(*c, hkh, false, kpart.p, ffast.p);
         }
@@ -827,6 +864,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
           scale_v(*c, scal.p);
           if (thermo) kick(*c, 0.0, true, kpart.p);
         }
+        if (bstep) barostat_update(thermo);
         if (thermo) {
           if (!listed) check_flags(step);
           emit(step, t);
@@ -850,18 +888,19 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
         else ++full_steps;
         last_rmse = learned_rmse(*LR);
       } else {
-        evaluate(*c, thermo, t, true);
+        evaluate(*c, thermo || bstep, t, true);
       }
       kick(*c, hk, thermo || o->thermostat == 1, kpart.p);
       if (o->thermostat == 1) {
         const double r1 = rng.normal();
         const double sum = ndof > 1 ? rng.gamma((ndof - 1) / 2, 2.0) : 0.0;
         bussi(*c, kpart.p, kt_target, ndof, std::exp(-dt / o->tau_t), r1, sum, scal.p, scal.p + 1);
-        if (thermo) scale_v(*c, scal.p);
+        if (thermo || bstep) scale_v(*c, scal.p);
         else alpha_pending = true;
       }
+      if (bstep) barostat_update(thermo);
       if (thermo) {
-        if (o->thermostat == 1) kick(*c, 0.0, true, kpart.p);
+        if (o->thermostat == 1 && !bstep) kick(*c, 0.0, true, kpart.p);
         if (o->thermostat == 2) bath_add(*c, bpart.p, scal.p + 1);
         if (!listed) check_flags(step);
         emit(step, t);

@@ -426,6 +426,17 @@ __global__ void k_bussi(const double* __restrict__ part, int64_t rows, double kt
   }
 }
 
+// the barostat's isotropic scaling: positions about the cell origin by mu, velocities by vf (dynamics.cpp:558-564, 716-719)
+__global__ void k_scale_xv(int n, double o0, double o1, double o2, double mu, double vf, double* __restrict__ x, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double o[3] = {o0, o1, o2};
+  for (int k = 0; k < 3; ++k) {
+    x[3 * i + k] = o[k] + mu * (x[3 * i + k] - o[k]);
+    if (vf != 1.0) v[3 * i + k] *= vf;
+  }
+}
+
 __global__ void k_scale_v(int n, const double* __restrict__ alpha, double* __restrict__ v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 3 * n) v[i] *= *alpha;
@@ -605,6 +616,14 @@ void bussi(capsgpu_ctx& c, const double* part, double kt_target, double ndof, do
 void scale_v(capsgpu_ctx& c, const double* alpha) {
   kt_begin(c, "k_scale_v");
   hipLaunchKernelGGL(k_scale_v, dim3(blocks_for(3 * size_t(c.n), 256)), dim3(256), 0, c.stream, c.n, alpha, c.v.p);
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+void scale_xv(capsgpu_ctx& c, const double origin[3], double mu, double vf) {
+  kt_begin(c, "k_scale_xv");
+  hipLaunchKernelGGL(k_scale_xv, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, origin[0], origin[1], origin[2], mu, vf, c.x.p,
+                     c.v.p);
   LAUNCHCHECK();
   kt_end(c);
 }

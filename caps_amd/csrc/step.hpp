@@ -17,6 +17,8 @@ void step_advance(capsgpu_ctx& c, uint64_t* stepc, uint64_t by);
 void kick(capsgpu_ctx& c, double hk, bool ke, double* part, const double* f = nullptr);
 void bussi(capsgpu_ctx& c, const double* part, double kt_target, double ndof, double cc, double r1, double sum, double* alpha, double* bath);
 void scale_v(capsgpu_ctx& c, const double* alpha);
+// positions scaled by mu about origin, velocities by vf (the barostat's isotropic update)
+void scale_xv(capsgpu_ctx& c, const double origin[3], double mu, double vf);
 // *bath −= Σ part over the rows of c's atom blocks; the rows are cleared
 void bath_add(capsgpu_ctx& c, double* part, double* bath);
 // many contexts' Langevin step, one launch per kernel: per replica its constants and device arrays

@@ -137,6 +137,14 @@ typedef struct capsgpu_dyn_opts {
   double learned_tol;
   int32_t learned_train;        /* learned r-RESPA: PME frames (plain steps) the network trains on before it predicts;
                                    < 0: 2000. The network persists in the context across runs. */
+  /* pressure coupling (DynamicsOptions::barostat ... barostat_every, dynamics.hpp:80-87; dynamics.cpp:701-723):
+   * isotropic, every barostat_every steps after the step: 0 none, 1 Berendsen, 2 stochastic cell rescaling
+   * (Bernetti & Bussi 2020), the cell and positions scaled about the cell origin, at most 1 % in volume per update */
+  int32_t barostat;
+  double pressure;              /* atm */
+  double tau_p;                 /* fs */
+  double compressibility;       /* atm⁻¹ */
+  int32_t barostat_every;       /* steps between cell updates (< 1: 10) */
 } capsgpu_dyn_opts;
 
 /* caps::ThermoRow (core/include/caps/dynamics.hpp:28-43), the fields this path fills */
@@ -147,6 +155,8 @@ typedef struct capsgpu_thermo {
   int32_t learned;              /* learned r-RESPA: 1 this row's forces came from the MLP path */
   double learned_rmse;          /* learned r-RESPA: the last judged relative force RMSE (0 before the first) */
   double learned_rmse_stale;    /* learned r-RESPA: the same for the stale slow force held since the last full step */
+  double volume, density;       /* Å³, g/cm³ */
+  double p[6];                  /* pressure tensor, atm (kinetic + virial): xx yy zz xy xz yz */
 } capsgpu_thermo;
 
 /* Progress callback with each thermo row: return 0 to cancel (caps::DynamicsOptions::progress, dynamics.hpp:~141) */
