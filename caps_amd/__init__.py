@@ -82,6 +82,25 @@ class BReport(C.Structure):
                 ("mode", C.c_int32), ("branches", C.c_int32), ("graph_steps", C.c_int32), ("tune_seconds", C.c_double)]
 
 
+class COpts(C.Structure):
+    _fields_ = [("t_start", C.c_double), ("t_end", C.c_double), ("t_step", C.c_double), ("ps_per_step", C.c_double),
+                ("equilibrate_ps", C.c_double), ("average_from", C.c_double), ("dt", C.c_double), ("tau_t", C.c_double),
+                ("tau_p", C.c_double), ("pressure", C.c_double), ("compressibility", C.c_double), ("barostat", C.c_int32),
+                ("barostat_every", C.c_int32), ("seed", C.c_uint64), ("new_velocities", C.c_int32), ("property", C.c_int32),
+                ("fit", C.c_int32), ("glassy_max", C.c_double), ("rubbery_min", C.c_double)]
+
+
+class CPoint(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("density", C.c_double), ("density_err", C.c_double), ("specific_volume", C.c_double),
+                ("potential", C.c_double)]
+
+
+class CFit(C.Structure):
+    _fields_ = [("ok", C.c_int32), ("note", C.c_int32), ("tg", C.c_double), ("tg_err", C.c_double), ("slope_low", C.c_double),
+                ("slope_high", C.c_double), ("value_at_tg", C.c_double), ("alpha_low", C.c_double), ("alpha_high", C.c_double),
+                ("rss", C.c_double)]
+
+
 class Sys(C.Structure):
     _fields_ = [("n_atoms", C.c_int32), ("n_types", C.c_int32), ("cell", C.c_double * 12), ("x", C.POINTER(C.c_double)),
                 ("type", C.POINTER(C.c_int32)), ("charge", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
@@ -93,6 +112,7 @@ class Sys(C.Structure):
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(Thermo), C.c_void_p)
+COOLING_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(Thermo), C.c_int32, C.c_int32, C.c_void_p)
 
 
 def lib():
@@ -137,6 +157,11 @@ def lib():
         L.capsgpu_batch_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
         L.capsgpu_batch_run.argtypes = [C.c_void_p, C.POINTER(BOpts), C.c_void_p, C.c_void_p, C.POINTER(BReport)]
         L.capsgpu_batch_destroy.argtypes = [C.c_void_p]
+        L.capsgpu_run_cooling.argtypes = [C.c_void_p, C.POINTER(COpts), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                          C.POINTER(C.c_int32), C.POINTER(CFit)]
+        L.capsgpu_batch_cooling.argtypes = [C.c_void_p, C.POINTER(COpts), C.c_int32, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                            C.c_void_p]
+        L.capsgpu_tg_fit_points.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.POINTER(CFit)]
         L.capsgpu_build_melt.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_build_kg.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_system_free.argtypes = [C.POINTER(Sys)]
@@ -645,3 +670,144 @@ def minimise(ev, system, steps=200, max_move=0.1, ftol=0.0, verbose=False):
             print("minimise", it, e, fmax, step, flush=True)
     system.x = x
     return t
+
+
+# ---- the glass-transition scan (caps::run_cooling, core/src/mechanics.cpp:1381-1487) --------------------------------
+@dataclass
+class CoolingOptions:
+    """caps::CoolingOptions (core/include/caps/mechanics.hpp:323-343), the fields this path reads"""
+    t_start: float = 500.0
+    t_end: float = 200.0
+    t_step: float = 20.0
+    ps_per_step: float = 100.0
+    equilibrate_ps: float = -1.0      # < 0: one hold
+    average_from: float = 0.5
+    dt: float = 1.0
+    tau_t: float = 100.0
+    tau_p: float = 1000.0
+    pressure: float = 1.0
+    compressibility: float = 4.5e-5
+    barostat: str = "c-rescale"
+    barostat_every: int = 10
+    seed: int = 1
+    new_velocities: bool = False
+    property: str = "volume"          # volume (specific volume) or potential (potential energy per atom)
+    fit: str = "bilinear"             # bilinear (hinge free) or ranges (glassy and rubbery lines)
+    glassy_max: float = 0.0
+    rubbery_min: float = 0.0
+    energy: EnergyOptions = field(default_factory=EnergyOptions)
+    progress: object = None           # callable(row, index, total) -> bool
+
+
+TG_NOTES = {0: "", 1: "too few temperatures for the fit", 2: "the two slopes differ by less than 10 %: no clear transition in this range",
+            3: "the hinge sits at the edge of the range: extend the temperatures", 4: "the glassy range must end below where the rubbery range starts",
+            5: "each range needs at least two temperatures", 6: "the two lines are parallel: no crossing",
+            7: "a range with two temperatures fixes its line exactly: no error estimate", 8: "the lines cross outside the gap between the ranges: no Tg"}
+
+
+@dataclass
+class TgFit:
+    """caps::BilinearFit (core/include/caps/mechanics.hpp:305-313)"""
+    ok: bool
+    tg: float
+    tg_err: float
+    slope_low: float
+    slope_high: float
+    value_at_tg: float
+    alpha_low: float
+    alpha_high: float
+    rss: float
+    note: str
+
+
+@dataclass
+class CoolingResult:
+    points: list                      # dicts: temperature, density, density_err, specific_volume, potential
+    fit: TgFit
+
+
+def _tgfit(f):
+    return TgFit(bool(f.ok), f.tg, f.tg_err, f.slope_low, f.slope_high, f.value_at_tg, f.alpha_low, f.alpha_high, f.rss, TG_NOTES.get(f.note, ""))
+
+
+def _points(arr, n):
+    return [dict(temperature=p.temperature, density=p.density, density_err=p.density_err, specific_volume=p.specific_volume,
+                 potential=p.potential) for p in arr[:n]]
+
+
+def _copts(o):
+    c = COpts()
+    c.t_start, c.t_end, c.t_step, c.ps_per_step = o.t_start, o.t_end, o.t_step, o.ps_per_step
+    c.equilibrate_ps, c.average_from, c.dt, c.tau_t = o.equilibrate_ps, o.average_from, o.dt, o.tau_t
+    c.tau_p, c.pressure, c.compressibility = o.tau_p, o.pressure, o.compressibility
+    if o.barostat.lower() not in _BAROSTATS:
+        raise ValueError("unknown barostat '%s' (none, berendsen, c-rescale)" % o.barostat)
+    c.barostat, c.barostat_every, c.seed = _BAROSTATS[o.barostat.lower()], o.barostat_every, o.seed
+    c.new_velocities = int(o.new_velocities)
+    c.property = {"volume": 0, "potential": 1}[o.property]
+    c.fit = {"bilinear": 0, "ranges": 1}[o.fit]
+    c.glassy_max, c.rubbery_min = o.glassy_max, o.rubbery_min
+    return c
+
+
+def _ladder_len(o):
+    import math
+    return int(math.floor(abs(o.t_end - o.t_start) / o.t_step + 1e-9)) + 1
+
+
+def run_cooling(system, o, evaluator=None):
+    """caps::run_cooling (core/src/mechanics.cpp:1381-1487) on the GPU: stepwise cooling of one system (its state left
+    at the end of the scan) and the Tg fit"""
+    ev = evaluator or Evaluator(system.ff, o.energy)
+    ev.set_state(system.x, None if o.new_velocities or system.v is None else system.v, system.cell)
+    c = _copts(o)
+    if system.v is None:
+        c.new_velocities = 1
+    cap = _ladder_len(o) + 2
+    pts = (CPoint * cap)()
+    n = C.c_int32()
+    f = CFit()
+    cb = None
+    if o.progress is not None:
+        def _cb(row, index, total, user):
+            return 1 if o.progress(_row(row.contents), index, total) else 0
+        cb = COOLING_FN(_cb)
+    _check(lib().capsgpu_run_cooling(ev._h, C.byref(c), C.cast(cb, C.c_void_p) if cb else None, None, C.cast(pts, C.c_void_p), cap,
+                                     C.byref(n), C.byref(f)))
+    x, v, cell = ev.get_state()
+    system.x, system.v, system.cell = x, v, cell
+    return CoolingResult(_points(pts, n.value), _tgfit(f))
+
+
+def fit_tg(T, y, fit="bilinear", glassy_max=0.0, rubbery_min=0.0):
+    """caps::fit_bilinear / fit_two_ranges (core/src/mechanics.cpp:1218-1379) of given points"""
+    T = np.ascontiguousarray(T, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    f = CFit()
+    _check(lib().capsgpu_tg_fit_points(_p(T), _p(y), len(T), {"bilinear": 0, "ranges": 1}[fit], glassy_max, rubbery_min, C.byref(f)))
+    return _tgfit(f)
+
+
+def batch_cooling(batch, systems, o, samples=5):
+    """many independent replicas scanned together through the replica batch (Langevin at constant volume; Tg from the
+    potential energy per atom): one CoolingResult per replica; the systems' states are updated"""
+    nrep = len(batch.evaluators)
+    if len(systems) != nrep:
+        raise ValueError("one system per replica")
+    for ev, s in zip(batch.evaluators, systems):
+        ev.set_state(s.x, None if o.new_velocities or s.v is None else s.v, s.cell)
+    c = _copts(o)
+    if any(s.v is None for s in systems):
+        c.new_velocities = 1
+    cap = _ladder_len(o) + 2
+    pts = (CPoint * (cap * nrep))()
+    fits = (CFit * nrep)()
+    n = C.c_int32()
+    _check(lib().capsgpu_batch_cooling(batch._h, C.byref(c), samples, C.cast(pts, C.c_void_p), cap, C.byref(n), C.cast(fits, C.c_void_p)))
+    out = []
+    for r, (ev, s) in enumerate(zip(batch.evaluators, systems)):
+        x, v, cell = ev.get_state()
+        s.x, s.v, s.cell = x, v, cell
+        out.append(CoolingResult(_points(pts[r * cap:(r + 1) * cap], n.value), _tgfit(fits[r])))
+    return out
+

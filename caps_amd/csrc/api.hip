@@ -187,6 +187,9 @@ void ev_record(capsgpu_ctx& c, int k) {
 }  // namespace
 
 namespace capsgpu {
+// the calling thread's error message for capsgpu_last_error (entry points defined in other files)
+void set_last_error(const std::string& m) { g_error = m; }
+
 // parts: the terms to evaluate (-1: the options'; bit 8 leaves PME's reciprocal part out of the non-bonded terms),
 // fdst: where the forces go (nullptr: c.f)
 void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts, double* fdst) {

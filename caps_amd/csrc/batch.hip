@@ -308,6 +308,8 @@ capsgpu_batch* batch_create(capsgpu_ctx* const* replicas, int n) {
 }
 
 void batch_destroy(capsgpu_batch* b) { delete b; }
+int batch_size(const capsgpu_batch& b) { return int(b.r.size()); }
+capsgpu_ctx* batch_replica(capsgpu_batch& b, int i) { return b.r.at(size_t(i)).c; }
 
 void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temperatures, double* temperature_out, capsgpu_batch_report* rep) {
   const int n = int(b.r.size());

@@ -227,6 +227,58 @@ int32_t capsgpu_batch_run(capsgpu_batch* batch, const capsgpu_batch_opts* opts, 
                           capsgpu_batch_report* report);
 int32_t capsgpu_batch_destroy(capsgpu_batch* batch);
 
+/* ---- the glass-transition scan -------------------------------------------------------------------------------------
+ * caps::run_cooling (core/src/mechanics.cpp:1381-1487, core/include/caps/mechanics.hpp:320-353): an unsampled hold at
+ * t_start (equilibrate_ps; < 0: one hold; Bussi, seed + 1000), then a hold of ps_per_step at each temperature of the
+ * ladder t_start, t_start ± t_step, … to t_end (Bussi τ_t, the barostat given, seed + k), the density and potential
+ * energy averaged over each hold's rows from average_from on (the density's error from five blocks), and Tg fitted to the
+ * specific volume (property 0) or the potential energy per atom (property 1): the hinge of a continuous two-line fit
+ * (fit 0, mechanics.cpp:1218-1308) or the crossing of lines through a glassy and a rubbery range (fit 1, :1310-1379;
+ * glassy_max / rubbery_min 0: the lowest and highest third), errors by a bootstrap of residuals. */
+typedef struct capsgpu_cooling_opts {
+  double t_start, t_end, t_step;   /* K */
+  double ps_per_step;              /* hold at each temperature, ps */
+  double equilibrate_ps;           /* < 0: one hold */
+  double average_from;             /* fraction of each hold left out of the averages */
+  double dt, tau_t, tau_p, pressure, compressibility;
+  int32_t barostat;                /* as capsgpu_dyn_opts (2: stochastic cell rescaling, the reference's default) */
+  int32_t barostat_every;
+  uint64_t seed;
+  int32_t new_velocities;
+  int32_t property, fit;
+  double glassy_max, rubbery_min;
+} capsgpu_cooling_opts;
+
+typedef struct capsgpu_cooling_point {
+  double temperature, density, density_err, specific_volume, potential;   /* K, g/cm³, g/cm³, cm³/g, kcal/mol */
+} capsgpu_cooling_point;
+
+/* caps::BilinearFit (mechanics.hpp:305-313); note: 0 none, 1 too few temperatures, 2 the slopes differ by under 10 %,
+ * 3 the hinge at the edge of the range, 4 the ranges overlap, 5 a range with under two temperatures, 6 parallel lines,
+ * 7 a range of two temperatures (no error estimate), 8 the lines cross outside the gap between the ranges (no Tg) */
+typedef struct capsgpu_tg_fit {
+  int32_t ok, note;
+  double tg, tg_err;               /* K */
+  double slope_low, slope_high, value_at_tg, alpha_low, alpha_high, rss;
+} capsgpu_tg_fit;
+
+/* progress with each thermo row: index the hold (−1 the equilibration) of total; return 0 to cancel */
+typedef int32_t (*capsgpu_cooling_progress_fn)(const capsgpu_thermo* row, int32_t index, int32_t total, void* user);
+
+/* the scan of one context from its set state (left at the end of the scan); points: up to cap, *npoints the ladder's
+ * length; fit may be NULL */
+int32_t capsgpu_run_cooling(capsgpu_ctx* ctx, const capsgpu_cooling_opts* opts, capsgpu_cooling_progress_fn progress, void* user,
+                            capsgpu_cooling_point* points, int32_t cap, int32_t* npoints, capsgpu_tg_fit* fit);
+/* Many independent replicas scanned together (no reference counterpart: the reference scans one system at a time): the
+ * holds run through the replica batch (Langevin τ_t at constant volume, so barostat 0 and property 1), every replica
+ * through the same ladder, its potential energy sampled at `samples` evenly spaced points of the hold's averaged part.
+ * points: [replicas][cap]; fits: one per replica (may be NULL). */
+int32_t capsgpu_batch_cooling(capsgpu_batch* batch, const capsgpu_cooling_opts* opts, int32_t samples, capsgpu_cooling_point* points,
+                              int32_t cap, int32_t* npoints, capsgpu_tg_fit* fits);
+/* the Tg fit of given points (fit 0 or 1 as above) */
+int32_t capsgpu_tg_fit_points(const double* T, const double* y, int32_t n, int32_t fit, double glassy_max, double rubbery_min,
+                              capsgpu_tg_fit* out);
+
 /* ---- inspection, for the parity checks (no counterpart in the reference's interfaces) ----------------------------- */
 /* sizes of the current cluster list: atoms, clusters, slots, outer entries, image codes, columns nu, nv, pruned full
  * entries */
