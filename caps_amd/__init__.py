@@ -101,6 +101,15 @@ class CFit(C.Structure):
                 ("rss", C.c_double)]
 
 
+class ROpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("ftol", C.c_double), ("etol", C.c_double), ("max_iterations", C.c_int32)]
+
+
+class RStage(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("evaluations", C.c_int32), ("stopped_by", C.c_int32), ("energy", C.c_double),
+                ("fmax", C.c_double)]
+
+
 class Sys(C.Structure):
     _fields_ = [("n_atoms", C.c_int32), ("n_types", C.c_int32), ("cell", C.c_double * 12), ("x", C.POINTER(C.c_double)),
                 ("type", C.POINTER(C.c_int32)), ("charge", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
@@ -157,6 +166,7 @@ def lib():
         L.capsgpu_batch_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
         L.capsgpu_batch_run.argtypes = [C.c_void_p, C.POINTER(BOpts), C.c_void_p, C.c_void_p, C.POINTER(BReport)]
         L.capsgpu_batch_destroy.argtypes = [C.c_void_p]
+        L.capsgpu_minimise.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ROpts), C.c_void_p, C.POINTER(RStage)]
         L.capsgpu_run_cooling.argtypes = [C.c_void_p, C.POINTER(COpts), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           C.POINTER(C.c_int32), C.POINTER(CFit)]
         L.capsgpu_batch_cooling.argtypes = [C.c_void_p, C.POINTER(COpts), C.c_int32, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
@@ -648,28 +658,44 @@ def kremer_grest(chains=200, beads=50, density=0.85, k_theta=0.0, bond=0.97, see
     return s
 
 
-def minimise(ev, system, steps=200, max_move=0.1, ftol=0.0, verbose=False):
-    """steepest descent with the largest atomic move capped (the push-off a grown melt needs before dynamics);
-    returns the final energy terms"""
-    x = np.array(system.x, np.float64)
-    step = max_move
-    t, f = ev.compute(x, system.cell)
-    e = t.total()
-    for it in range(steps):
-        fmax = float(np.sqrt((f * f).sum(1)).max())
-        if fmax <= ftol:
-            break
-        xn = x + f * (step / fmax)
-        tn, fn = ev.compute(xn, system.cell)
-        if tn.total() < e:
-            x, f, e, t = xn, fn, tn.total(), tn
-            step = min(max_move, step * 1.2)
-        else:
-            step *= 0.5
-        if verbose and it % 20 == 0:
-            print("minimise", it, e, fmax, step, flush=True)
-    system.x = x
-    return t
+_MINIMISERS = {"sd": 0, "steepest": 0, "cg": 1, "lbfgs": 2, "l-bfgs": 2, "fire": 3}
+_STOPS = {0: "iterations", 1: "force", 2: "energy", 3: "line search"}
+
+
+@dataclass
+class RelaxOptions:
+    """caps::RelaxOptions (core/include/caps/relax.hpp:28-41), the minimiser's fields"""
+    method: str = "lbfgs"             # sd, cg, lbfgs, fire
+    ftol: float = 0.5                 # kcal/mol/A
+    etol: float = 1e-8
+    max_iterations: int = 5000
+    fixed: np.ndarray = None          # per atom: held in place
+
+
+@dataclass
+class RelaxStage:
+    """caps::RelaxStage (core/include/caps/relax.hpp:86-91)"""
+    iterations: int
+    evaluations: int
+    energy: float
+    fmax: float
+    stopped_by: str
+
+
+def minimise(ev, system, o=None):
+    """caps::minimise (core/src/relax.cpp:65-282) on the GPU with the evaluator's terms: system.x is updated"""
+    o = o or RelaxOptions()
+    if o.method.lower() not in _MINIMISERS:
+        raise ValueError("unknown minimiser '%s' (sd, cg, lbfgs, fire)" % o.method)
+    r = ROpts()
+    r.method, r.ftol, r.etol, r.max_iterations = _MINIMISERS[o.method.lower()], o.ftol, o.etol, o.max_iterations
+    x = np.ascontiguousarray(system.x, np.float64).reshape(-1).copy()
+    cell = np.ascontiguousarray(system.cell, np.float64)
+    fixed = None if o.fixed is None else np.ascontiguousarray(o.fixed, np.uint8)
+    st = RStage()
+    _check(lib().capsgpu_minimise(ev._h, _p(x), _p(cell), C.byref(r), _p(fixed), C.byref(st)))
+    system.x = x.reshape(-1, 3)
+    return RelaxStage(st.iterations, st.evaluations, st.energy, st.fmax, _STOPS.get(st.stopped_by, "?"))
 
 
 # ---- the glass-transition scan (caps::run_cooling, core/src/mechanics.cpp:1381-1487) --------------------------------

@@ -118,6 +118,28 @@ int32_t capsgpu_get_state(capsgpu_ctx* ctx, double* x, double* v, double* cell);
 /* the forces of the last evaluation (3N) */
 int32_t capsgpu_get_forces(capsgpu_ctx* ctx, double* f);
 
+/* ---- energy minimisation -------------------------------------------------------------------------------------------
+ * caps::minimise (core/src/relax.cpp:65-282; options core/include/caps/relax.hpp:28-32, stage :86-91): steepest descent,
+ * Polak–Ribière CG (with its parabola step), L-BFGS (m = 10) or FIRE on the context's evaluation, every vector on the
+ * device. Stops when the largest atomic force is below ftol, or the relative energy change below etol, or after
+ * max_iterations, or when the line search fails twice. */
+typedef struct capsgpu_relax_opts {
+  int32_t method;               /* 0 steepest descent, 1 conjugate gradients, 2 L-BFGS, 3 FIRE (caps::Minimiser's order) */
+  double ftol;                  /* kcal/mol/Å */
+  double etol;
+  int32_t max_iterations;
+} capsgpu_relax_opts;
+
+typedef struct capsgpu_relax_stage {
+  int32_t iterations, evaluations;
+  int32_t stopped_by;           /* 0 iterations, 1 force, 2 energy, 3 line search */
+  double energy, fmax;          /* kcal/mol, kcal/mol/Å at the end */
+} capsgpu_relax_stage;
+
+/* x: 3N in, the minimised positions out (also the context's state); fixed: per atom 1 = held in place (NULL: none) */
+int32_t capsgpu_minimise(capsgpu_ctx* ctx, double* x, const double cell[12], const capsgpu_relax_opts* opts, const uint8_t* fixed,
+                         capsgpu_relax_stage* out);
+
 /* caps::DynamicsOptions (core/include/caps/dynamics.hpp:60-154), the fields this path reads */
 typedef struct capsgpu_dyn_opts {
   double dt;                    /* fs */
