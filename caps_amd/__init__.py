@@ -56,7 +56,7 @@ class DOpts(C.Structure):
                 ("tau_t", C.c_double), ("seed", C.c_uint64), ("thermo_every", C.c_int32), ("new_velocities", C.c_int32),
                 ("respa", C.c_int32), ("step_offset", C.c_int64), ("learned_k", C.c_int32), ("learned_tol", C.c_double),
                 ("learned_train", C.c_int32), ("barostat", C.c_int32), ("pressure", C.c_double), ("tau_p", C.c_double),
-                ("compressibility", C.c_double), ("barostat_every", C.c_int32)]
+                ("compressibility", C.c_double), ("barostat_every", C.c_int32), ("temperature_end", C.c_double)]
 
 
 class Thermo(C.Structure):
@@ -108,6 +108,27 @@ class ROpts(C.Structure):
 class RStage(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("evaluations", C.c_int32), ("stopped_by", C.c_int32), ("energy", C.c_double),
                 ("fmax", C.c_double)]
+
+
+class EqStage(C.Structure):
+    _fields_ = [("ensemble", C.c_int32), ("ps", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double), ("pressure", C.c_double)]
+
+
+class EqOpts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("tau_t", C.c_double), ("tau_p", C.c_double), ("compressibility", C.c_double),
+                ("thermostat", C.c_int32), ("barostat", C.c_int32), ("barostat_every", C.c_int32), ("seed", C.c_uint64),
+                ("thermo_ps", C.c_double)]
+
+
+class EqSummary(C.Structure):
+    _fields_ = [("ps", C.c_double), ("temperature", C.c_double), ("pressure", C.c_double), ("density", C.c_double),
+                ("potential", C.c_double), ("density_end", C.c_double)]
+
+
+class ProtoParams(C.Structure):
+    _fields_ = [("t_final", C.c_double), ("t_max", C.c_double), ("p_final", C.c_double), ("p_max", C.c_double),
+                ("time_scale", C.c_double), ("cycles", C.c_int32), ("t_low", C.c_double), ("t_high", C.c_double),
+                ("ramp_ps", C.c_double), ("hold_ps", C.c_double)]
 
 
 class Sys(C.Structure):
@@ -167,6 +188,8 @@ def lib():
         L.capsgpu_batch_run.argtypes = [C.c_void_p, C.POINTER(BOpts), C.c_void_p, C.c_void_p, C.POINTER(BReport)]
         L.capsgpu_batch_destroy.argtypes = [C.c_void_p]
         L.capsgpu_minimise.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ROpts), C.c_void_p, C.POINTER(RStage)]
+        L.capsgpu_equilibrate.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(EqOpts), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.capsgpu_protocol.argtypes = [C.c_char_p, C.POINTER(ProtoParams), C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
         L.capsgpu_run_cooling.argtypes = [C.c_void_p, C.POINTER(COpts), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           C.POINTER(C.c_int32), C.POINTER(CFit)]
         L.capsgpu_batch_cooling.argtypes = [C.c_void_p, C.POINTER(COpts), C.c_int32, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
@@ -473,6 +496,7 @@ class DynamicsOptions:
     tau_p: float = 1000.0             # fs
     compressibility: float = 4.5e-5   # 1/atm
     barostat_every: int = 10          # steps between cell updates
+    temperature_end: float = -1.0     # K; > 0 ramps the thermostat target over the run
 
 
 @dataclass
@@ -510,6 +534,7 @@ def run_dynamics(system, o, evaluator=None):
         raise ValueError("unknown barostat '%s' (none, berendsen, c-rescale)" % o.barostat)
     d.barostat, d.pressure, d.tau_p = _BAROSTATS[o.barostat.lower()], o.pressure, o.tau_p
     d.compressibility, d.barostat_every = o.compressibility, o.barostat_every
+    d.temperature_end = o.temperature_end
     cap = int(o.steps // max(1, o.thermo_every) + 2)
     rows = (Thermo * cap)()
     nr = C.c_int32()
@@ -836,4 +861,90 @@ def batch_cooling(batch, systems, o, samples=5):
         s.x, s.v, s.cell = x, v, cell
         out.append(CoolingResult(_points(pts[r * cap:(r + 1) * cap], n.value), _tgfit(fits[r])))
     return out
+
+
+# ---- equilibration protocols (caps::equilibrate, core/src/equilibrate.cpp) -----------------------------------------
+_ENSEMBLES = {"nvt": 0, "npt": 1, "nve": 2}
+
+
+@dataclass
+class Stage:
+    """caps::Stage (core/include/caps/equilibrate.hpp:15-22), without the push-off force cap"""
+    ensemble: str = "nvt"
+    ps: float = 10.0
+    t_start: float = 300.0
+    t_end: float = -1.0
+    pressure: float = 1.0
+    label: str = ""
+
+
+@dataclass
+class ProtocolParams:
+    """caps::ProtocolParams (core/include/caps/equilibrate.hpp:24-37)"""
+    t_final: float = 300.0
+    t_max: float = 600.0
+    p_final: float = 1.0
+    p_max: float = 49346.2
+    time_scale: float = 1.0
+    cycles: int = 3
+    t_low: float = 300.0
+    t_high: float = 600.0
+    ramp_ps: float = 50.0
+    hold_ps: float = 50.0
+
+
+def protocol(name, params=None):
+    """the stages of "larsen21" or "annealing" (core/src/equilibrate.cpp:43-75)"""
+    p = params or ProtocolParams()
+    pp = ProtoParams(p.t_final, p.t_max, p.p_final, p.p_max, p.time_scale, p.cycles, p.t_low, p.t_high, p.ramp_ps, p.hold_ps)
+    n = C.c_int32()
+    _check(lib().capsgpu_protocol(name.encode(), C.byref(pp), None, 0, C.byref(n)))
+    arr = (EqStage * max(1, n.value))()
+    _check(lib().capsgpu_protocol(name.encode(), C.byref(pp), C.cast(arr, C.c_void_p), n.value, C.byref(n)))
+    names = {0: "nvt", 1: "npt", 2: "nve"}
+    return [Stage(names[a.ensemble], a.ps, a.t_start, a.t_end, a.pressure, "%d" % (k + 1)) for k, a in enumerate(arr[:n.value])]
+
+
+@dataclass
+class EquilibrateOptions:
+    """caps::EquilibrateOptions (core/include/caps/equilibrate.hpp:65-89), the staged part"""
+    stages: list = field(default_factory=list)
+    dt: float = 1.0
+    tau_t: float = 100.0
+    tau_p: float = 1000.0
+    compressibility: float = 4.5e-5
+    thermostat: str = "bussi"
+    barostat: str = "c-rescale"
+    barostat_every: int = 10
+    seed: int = 1
+    thermo_ps: float = 0.0
+    energy: EnergyOptions = field(default_factory=EnergyOptions)
+    progress: object = None           # callable(stage_index, stages, row) -> bool
+
+
+EQ_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(Thermo), C.c_int32, C.c_int32, C.c_void_p)
+
+
+def equilibrate(system, o, evaluator=None):
+    """caps::equilibrate (core/src/equilibrate.cpp:188-262) on the GPU: the stages one after another; returns one summary
+    dict per stage (means over its second half) and leaves the system at the end"""
+    ev = evaluator or Evaluator(system.ff, o.energy)
+    ev.set_state(system.x, system.v, system.cell)
+    st = (EqStage * len(o.stages))()
+    for k, s in enumerate(o.stages):
+        st[k] = EqStage(_ENSEMBLES[s.ensemble.lower()], s.ps, s.t_start, s.t_end, s.pressure)
+    eo = EqOpts(o.dt, o.tau_t, o.tau_p, o.compressibility, _THERMOSTATS[o.thermostat.lower()], _BAROSTATS[o.barostat.lower()],
+                o.barostat_every, o.seed, o.thermo_ps)
+    out = (EqSummary * len(o.stages))()
+    cb = None
+    if o.progress is not None:
+        def _cb(row, index, total, user):
+            return 1 if o.progress(index, total, _row(row.contents)) else 0
+        cb = EQ_FN(_cb)
+    _check(lib().capsgpu_equilibrate(ev._h, C.cast(st, C.c_void_p), len(o.stages), C.byref(eo), C.cast(cb, C.c_void_p) if cb else None,
+                                     None, C.cast(out, C.c_void_p)))
+    x, v, cell = ev.get_state()
+    system.x, system.v, system.cell = x, v, cell
+    return [dict(label=s.label, ps=r.ps, temperature=r.temperature, pressure=r.pressure, density=r.density, potential=r.potential,
+                 density_end=r.density_end) for s, r in zip(o.stages, out)]
 

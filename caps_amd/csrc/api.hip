@@ -724,7 +724,15 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     const bool listed = (c->opt.parts & 2) != 0;
     const double lim_b = listed ? 0.25 * c->opt.skin * c->opt.skin : -1.0, lim_p = listed ? 0.25 * bskin * bskin : -1.0;
     const double c1 = std::exp(-dt / std::max(o->tau_t, 1e-9)), c2 = std::sqrt(1 - c1 * c1);
-    const double kt_target = 0.5 * ndof * kBoltz * o->temperature;
+    double kt_target = 0.5 * ndof * kBoltz * o->temperature;
+    // a ramped thermostat target (DynamicsOptions::temperature_end, dynamics.cpp:342-345, 608-612): T(step) from the
+    // starting temperature to temperature_end over the run; Langevin's noise scales with √(T(step)/T(0))
+    const bool ramp = o->temperature_end > 0 && o->steps > 0;
+    if (ramp && !(o->temperature > 0)) throw Error(CAPSGPU_EINVAL, "a temperature ramp needs a positive starting temperature");
+    double t_now = o->temperature, c2_now = c2;
+    auto target_t = [&](int64_t step) {
+      return ramp ? o->temperature + (o->temperature_end - o->temperature) * double(step) / double(o->steps) : o->temperature;
+    };
     DevBuf<double> sdv, kpart, bpart, scal;   // scal: [0] α, [1] bath
     if (o->thermostat == 2) {
       std::vector<double> sd(static_cast<size_t>(n));
@@ -818,7 +826,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       const double p = (2 * kin + t[CAPSGPU_T_VIRIAL]) / (3 * V) * kAtm;
       double deps = -o->compressibility / o->tau_p * (o->pressure - p) * hp;
       if (baro == 2) {
-        const double ktv = kBoltz * o->temperature / V * kAtm;
+        const double ktv = kBoltz * t_now / V * kAtm;
         deps += o->compressibility / o->tau_p * ktv * hp + std::sqrt(2 * ktv * o->compressibility * hp / o->tau_p) * rng.normal();
       }
       deps = std::clamp(deps, -0.01, 0.01);
@@ -841,6 +849,11 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     bool alpha_pending = false;
     for (int64_t step = 1; step <= o->steps; ++step) {
       const bool thermo = step % every == 0 || step == o->steps;
+      if (ramp) {
+        t_now = target_t(step);
+        kt_target = 0.5 * ndof * kBoltz * t_now;
+        c2_now = c2 * std::sqrt(t_now / o->temperature);
+      }
       const bool bstep = baro && step % bevery == 0;
       if (listed) c->flags.zero(s);
       if (R > 1) {
@@ -875,7 +888,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
         continue;
       }
       if (o->thermostat == 2) {
-        baoab_open(*c, hk, 0.5 * dt, c1, c2, sdv.p, o->seed, nullptr, uint64_t(step), lim_b, lim_p, bpart.p);
+        baoab_open(*c, hk, 0.5 * dt, c1, c2_now, sdv.p, o->seed, nullptr, uint64_t(step), lim_b, lim_p, bpart.p);
       } else {
         vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p);
         alpha_pending = false;

@@ -1,7 +1,9 @@
-// The glass-transition scan on the GPU: stepwise cooling (core/src/mechanics.cpp:1381-1487, caps::run_cooling) of one
-// context through capsgpu_run_dynamics, and of many replicas at once through the replica batch; the Tg fits of
-// mechanics.cpp:1218-1379 (a continuous two-line fit with the hinge free, or two lines through a glassy and a rubbery
-// range), with their bootstrap errors drawn as the reference draws them (std::mt19937_64(12345), caps::UniformInt).
+// Protocols on the GPU dynamics. The glass-transition scan: stepwise cooling (core/src/mechanics.cpp:1381-1487,
+// caps::run_cooling) of one context through capsgpu_run_dynamics, and of many replicas at once through the replica batch;
+// the Tg fits of mechanics.cpp:1218-1379 (a continuous two-line fit with the hinge free, or two lines through a glassy
+// and a rubbery range), with their bootstrap errors drawn as the reference draws them (std::mt19937_64(12345),
+// caps::UniformInt). Equilibration: chained NVT / NPT / NVE stages (core/src/equilibrate.cpp:188-262, caps::equilibrate)
+// and the 21-step and annealing protocols (equilibrate.cpp:43-75).
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -322,6 +324,7 @@ int32_t capsgpu_run_cooling(capsgpu_ctx* c, const capsgpu_cooling_opts* o, capsg
       d.learned_train = -1;
       d.barostat = o->barostat, d.pressure = o->pressure, d.tau_p = o->tau_p, d.compressibility = o->compressibility;
       d.barostat_every = o->barostat_every;
+      d.temperature_end = -1;
       rows.assign(size_t(nsteps / std::max<int64_t>(1, every) + 2), capsgpu_thermo{});
       int32_t nr = 0;
       rethrow(capsgpu_run_dynamics(c, &d, relay, &pc, rows.data(), int32_t(rows.size()), &nr, nullptr));
@@ -408,6 +411,113 @@ int32_t capsgpu_batch_cooling(capsgpu_batch* b, const capsgpu_cooling_opts* o, i
       if (fits) fits[r] = cooling_fit(*o, pts[size_t(r)], double(std::max(1, batch_replica(*b, r)->n)));
     }
     if (npoints) *npoints = int32_t(temps.size());
+  });
+}
+
+int32_t capsgpu_protocol(const char* name, const capsgpu_protocol_params* p, capsgpu_eq_stage* out, int32_t cap, int32_t* n) {
+  return guard_cool([&] {
+    if (!name || !p || !n) throw Error(CAPSGPU_EINVAL, "null argument");
+    if (!(p->time_scale > 0)) throw Error(CAPSGPU_EINVAL, "the time scale must be above 0");
+    if (!(p->ramp_ps >= 0) || !(p->hold_ps >= 0)) throw Error(CAPSGPU_EINVAL, "stage lengths must be 0 ps or more");
+    if (p->cycles < 0) throw Error(CAPSGPU_EINVAL, "the number of annealing cycles must be 0 or more");
+    std::vector<capsgpu_eq_stage> st;
+    auto add = [&](int ens, double ps, double t0, double t1, double pr) {
+      capsgpu_eq_stage s{};
+      s.ensemble = ens, s.ps = ps * p->time_scale, s.t_start = t0, s.t_end = t1, s.pressure = pr;
+      st.push_back(s);
+    };
+    const std::string nm = name;
+    if (nm == "larsen21" || nm == "21-step") {
+      // Larsen, Lin, Hart and Colina, Macromolecules 44, 6944 (2011), Table 1
+      const double tm = p->t_max, tf = p->t_final, pm = p->p_max;
+      const double fr[6] = {0.02, 0.6, 1.0, 0.5, 0.1, 0.01};
+      const double heat[6] = {50, 50, 50, 50, 5, 5}, cool[6] = {50, 100, 100, 100, 10, 10}, comp[6] = {50, 50, 50, 5, 5, 5};
+      for (int k = 0; k < 6; ++k) {
+        add(0, heat[k], tm, -1, 1.0);
+        add(0, cool[k], tf, -1, 1.0);
+        add(1, comp[k], tf, -1, fr[k] * pm);
+      }
+      // steps 19 and 20 (heat, cool) and the final NPT stage
+      add(0, 5, tm, -1, 1.0);
+      add(0, 10, tf, -1, 1.0);
+      add(1, 800, tf, -1, p->p_final);
+    } else if (nm == "annealing" || nm == "anneal") {
+      add(1, p->hold_ps, p->t_low, -1, p->p_final);
+      for (int c = 1; c <= p->cycles; ++c) {
+        add(1, p->ramp_ps, p->t_low, p->t_high, p->p_final);
+        add(1, p->hold_ps, p->t_high, -1, p->p_final);
+        add(1, p->ramp_ps, p->t_high, p->t_low, p->p_final);
+        add(1, p->hold_ps, p->t_low, -1, p->p_final);
+      }
+    } else {
+      throw Error(CAPSGPU_EINVAL, "unknown protocol '" + nm + "' (larsen21, annealing)");
+    }
+    *n = int32_t(st.size());
+    if (out)
+      for (size_t k = 0; k < st.size() && int32_t(k) < cap; ++k) out[k] = st[k];
+  });
+}
+
+int32_t capsgpu_equilibrate(capsgpu_ctx* c, const capsgpu_eq_stage* stages, int32_t nstages, const capsgpu_eq_opts* o,
+                            capsgpu_eq_progress_fn progress, void* user, capsgpu_eq_summary* out) {
+  return guard_cool([&] {
+    if (!c || !stages || !o || nstages < 1) throw Error(CAPSGPU_EINVAL, "the protocol has no stages");
+    if (!(o->dt > 0) || o->dt > 5) throw Error(CAPSGPU_EINVAL, "time step must be in (0, 5] fs");
+    for (int k = 0; k < nstages; ++k) {
+      if (stages[k].ensemble < 0 || stages[k].ensemble > 2) throw Error(CAPSGPU_EINVAL, "a stage's ensemble is 0 (NVT), 1 (NPT) or 2 (NVE)");
+      if (!(stages[k].ps > 0)) throw Error(CAPSGPU_EINVAL, "every stage needs a positive duration");
+    }
+    int64_t offset = 0;
+    int64_t last_step = -1;
+    struct Ctx {
+      capsgpu_eq_progress_fn fn;
+      void* user;
+      int index, total;
+      int64_t* last;
+      std::vector<capsgpu_thermo>* rows;
+    };
+    std::vector<capsgpu_thermo> kept;
+    Ctx pc{progress, user, 0, nstages, &last_step, &kept};
+    auto relay = [](const capsgpu_thermo* r, void* u) -> int32_t {
+      Ctx* p = static_cast<Ctx*>(u);
+      // the first row of a stage repeats the last row of the previous one
+      if (*p->last == r->step) return 1;
+      *p->last = r->step;
+      p->rows->push_back(*r);
+      return p->fn ? p->fn(r, p->index, p->total, p->user) : 1;
+    };
+    for (int k = 0; k < nstages; ++k) {
+      const capsgpu_eq_stage& st = stages[k];
+      capsgpu_dyn_opts d{};
+      const int64_t steps = std::max<int64_t>(1, std::llround(st.ps * 1000.0 / o->dt));
+      d.dt = o->dt, d.steps = steps, d.temperature = st.t_start, d.temperature_end = st.t_end;
+      d.thermostat = st.ensemble == 2 ? 0 : (o->thermostat == 0 ? 1 : o->thermostat);
+      d.tau_t = o->tau_t;
+      d.barostat = st.ensemble == 1 ? (o->barostat == 0 ? 2 : o->barostat) : 0;
+      d.pressure = st.pressure, d.tau_p = o->tau_p, d.compressibility = o->compressibility, d.barostat_every = o->barostat_every;
+      d.seed = o->seed + 1000003ull * uint64_t(k);
+      d.step_offset = offset;
+      const double tps = o->thermo_ps > 0 ? o->thermo_ps : std::clamp(st.ps / 200.0, 10.0 * o->dt / 1000.0, 0.5);
+      d.thermo_every = std::max(1, int(std::lround(tps * 1000.0 / o->dt)));
+      d.new_velocities = 0, d.respa = 1, d.learned_train = -1;
+      pc.index = k;
+      const size_t first = kept.size();
+      std::vector<capsgpu_thermo> rows(size_t(steps / d.thermo_every + 2));
+      int32_t nr = 0;
+      rethrow(capsgpu_run_dynamics(c, &d, relay, &pc, rows.data(), int32_t(rows.size()), &nr, nullptr));
+      offset += steps;
+      capsgpu_eq_summary sum{};
+      sum.ps = double(steps) * o->dt / 1000.0;
+      const size_t a = first + (kept.size() - first) / 2, m = kept.size() - a;
+      for (size_t q = a; q < kept.size(); ++q) {
+        sum.temperature += kept[q].temperature / double(m);
+        sum.pressure += kept[q].pressure / double(m);
+        sum.density += kept[q].density / double(m);
+        sum.potential += kept[q].potential / double(m);
+      }
+      sum.density_end = kept.empty() ? 0 : kept.back().density;
+      if (out) out[k] = sum;
+    }
   });
 }
 

@@ -167,6 +167,8 @@ typedef struct capsgpu_dyn_opts {
   double tau_p;                 /* fs */
   double compressibility;       /* atm⁻¹ */
   int32_t barostat_every;       /* steps between cell updates (< 1: 10) */
+  double temperature_end;       /* K; > 0 ramps the thermostat target linearly from temperature to this over the run
+                                   (DynamicsOptions::temperature_end, dynamics.hpp:80); <= 0 no ramp (a zeroed struct) */
 } capsgpu_dyn_opts;
 
 /* caps::ThermoRow (core/include/caps/dynamics.hpp:28-43), the fields this path fills */
@@ -300,6 +302,45 @@ int32_t capsgpu_batch_cooling(capsgpu_batch* batch, const capsgpu_cooling_opts* 
 /* the Tg fit of given points (fit 0 or 1 as above) */
 int32_t capsgpu_tg_fit_points(const double* T, const double* y, int32_t n, int32_t fit, double glassy_max, double rubbery_min,
                               capsgpu_tg_fit* out);
+
+/* ---- equilibration protocols ---------------------------------------------------------------------------------------
+ * caps::equilibrate (core/src/equilibrate.cpp:188-262; core/include/caps/equilibrate.hpp:15-37, 65-106) without the
+ * convergence blocks: the stages run one after another from the context's state (no new velocities), stage k with seed
+ * + 1000003 k, NVT and NPT stages under the thermostat given (none: Bussi), NPT ones under the barostat given (none:
+ * stochastic cell rescaling), a thermo row every thermo_ps (0: about 200 per stage, between 10 steps and 0.5 ps); each
+ * stage summarised by the means over the second half of its rows. Push-off stages (capped pair forces) are not built. */
+typedef struct capsgpu_eq_stage {
+  int32_t ensemble;             /* 0 NVT, 1 NPT, 2 NVE */
+  double ps;                    /* duration */
+  double t_start, t_end;        /* K; t_end > 0 ramps the thermostat target over the stage */
+  double pressure;              /* atm (NPT) */
+} capsgpu_eq_stage;
+
+typedef struct capsgpu_eq_opts {
+  double dt, tau_t, tau_p, compressibility;
+  int32_t thermostat, barostat, barostat_every;   /* as capsgpu_dyn_opts */
+  uint64_t seed;
+  double thermo_ps;
+} capsgpu_eq_opts;
+
+typedef struct capsgpu_eq_summary {
+  double ps, temperature, pressure, density, potential, density_end;
+} capsgpu_eq_summary;
+
+typedef struct capsgpu_protocol_params {      /* caps::ProtocolParams (equilibrate.hpp:24-37) */
+  double t_final, t_max, p_final, p_max, time_scale;
+  int32_t cycles;
+  double t_low, t_high, ramp_ps, hold_ps;
+} capsgpu_protocol_params;
+
+/* progress with each thermo row of stage index (of total); return 0 to cancel */
+typedef int32_t (*capsgpu_eq_progress_fn)(const capsgpu_thermo* row, int32_t index, int32_t total, void* user);
+
+int32_t capsgpu_equilibrate(capsgpu_ctx* ctx, const capsgpu_eq_stage* stages, int32_t nstages, const capsgpu_eq_opts* opts,
+                            capsgpu_eq_progress_fn progress, void* user, capsgpu_eq_summary* out);
+/* the stages of a named protocol (equilibrate.cpp:43-75): "larsen21" (the 21-step compression and decompression of
+ * Larsen et al. 2011) or "annealing" (NPT heat / hold / cool / hold cycles); *n the count, up to cap written to out */
+int32_t capsgpu_protocol(const char* name, const capsgpu_protocol_params* params, capsgpu_eq_stage* out, int32_t cap, int32_t* n);
 
 /* ---- inspection, for the parity checks (no counterpart in the reference's interfaces) ----------------------------- */
 /* sizes of the current cluster list: atoms, clusters, slots, outer entries, image codes, columns nu, nv, pruned full
