@@ -56,7 +56,8 @@ class DOpts(C.Structure):
                 ("tau_t", C.c_double), ("seed", C.c_uint64), ("thermo_every", C.c_int32), ("new_velocities", C.c_int32),
                 ("respa", C.c_int32), ("step_offset", C.c_int64), ("learned_k", C.c_int32), ("learned_tol", C.c_double),
                 ("learned_train", C.c_int32), ("barostat", C.c_int32), ("pressure", C.c_double), ("tau_p", C.c_double),
-                ("compressibility", C.c_double), ("barostat_every", C.c_int32), ("temperature_end", C.c_double)]
+                ("compressibility", C.c_double), ("barostat_every", C.c_int32), ("temperature_end", C.c_double),
+                ("checkpoint_every", C.c_int64), ("checkpoint", C.c_void_p), ("checkpoint_user", C.c_void_p)]
 
 
 class Thermo(C.Structure):
@@ -138,10 +139,11 @@ class Sys(C.Structure):
                 ("bond_atoms", C.POINTER(C.c_int32)), ("bond_param", C.POINTER(C.c_double)), ("n_angles", C.c_int32),
                 ("angle_atoms", C.POINTER(C.c_int32)), ("angle_param", C.POINTER(C.c_double)), ("n_fene", C.c_int32),
                 ("fene_atoms", C.POINTER(C.c_int32)), ("fene_param", C.POINTER(C.c_double)), ("excl_offset", C.POINTER(C.c_int32)),
-                ("excl_index", C.POINTER(C.c_int32)), ("opaque", C.c_void_p)]
+                ("excl_index", C.POINTER(C.c_int32)), ("v", C.POINTER(C.c_double)), ("opaque", C.c_void_p)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(Thermo), C.c_void_p)
+CHECKPOINT_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64, C.c_void_p)
 COOLING_FN = C.CFUNCTYPE(C.c_int32, C.POINTER(Thermo), C.c_int32, C.c_int32, C.c_void_p)
 
 
@@ -198,6 +200,8 @@ def lib():
         L.capsgpu_build_melt.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_build_kg.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.POINTER(C.POINTER(Sys))]
         L.capsgpu_system_free.argtypes = [C.POINTER(Sys)]
+        L.capsgpu_read_lammps.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.POINTER(Sys))]
+        L.capsgpu_write_lammps.argtypes = [C.c_char_p, C.POINTER(Sys), C.c_void_p, C.c_char_p]
         _lib = L
     return _lib
 
@@ -258,6 +262,7 @@ class System:
     cell: np.ndarray           # (12,) origin, a, b, c
     ff: ForceField
     v: np.ndarray = None       # (N, 3) Å/fs
+    title: str = ""            # a data file's first line
 
 
 @dataclass
@@ -497,6 +502,8 @@ class DynamicsOptions:
     compressibility: float = 4.5e-5   # 1/atm
     barostat_every: int = 10          # steps between cell updates
     temperature_end: float = -1.0     # K; > 0 ramps the thermostat target over the run
+    checkpoint_every: int = 0         # steps between checkpoints (0: none; dynamics.hpp:143-146)
+    checkpoint: object = None         # a path (a LAMMPS data file with velocities, written whole) or callable(x, v, cell, step)
 
 
 @dataclass
@@ -539,13 +546,37 @@ def run_dynamics(system, o, evaluator=None):
     rows = (Thermo * cap)()
     nr = C.c_int32()
     rep = DReport()
+    failure = []
+    cpf = None
+    if o.checkpoint_every > 0 and o.checkpoint is not None:
+        n = system.ff.n
+        total = o.step_offset + o.steps
+
+        def _cp(xp, vp, cp, step, user):
+            try:
+                x = np.ctypeslib.as_array(xp, shape=(3 * n,)).reshape(-1, 3).copy()
+                v = np.ctypeslib.as_array(vp, shape=(3 * n,)).reshape(-1, 3).copy()
+                cell = np.ctypeslib.as_array(cp, shape=(12,)).copy()
+                if callable(o.checkpoint):
+                    o.checkpoint(x, v, cell, int(step))
+                else:
+                    write_checkpoint(o.checkpoint, System(x=x, cell=cell, ff=system.ff, v=v), int(step), total)
+                return 0
+            except Exception as e:   # noqa: BLE001 - handed back after the run
+                failure.append(e)
+                return 1
+        cpf = CHECKPOINT_FN(_cp)
+        d.checkpoint_every, d.checkpoint = o.checkpoint_every, C.cast(cpf, C.c_void_p)
     cb = None
     if o.progress is not None:
         def _cb(row, user):
             return 1 if o.progress(_row(row.contents)) else 0
         cb = PROGRESS_FN(_cb)
-    _check(lib().capsgpu_run_dynamics(ev._h, C.byref(d), C.cast(cb, C.c_void_p) if cb else None, None, C.cast(rows, C.c_void_p), cap,
-                                      C.byref(nr), C.byref(rep)))
+    rc = lib().capsgpu_run_dynamics(ev._h, C.byref(d), C.cast(cb, C.c_void_p) if cb else None, None, C.cast(rows, C.c_void_p), cap,
+                                    C.byref(nr), C.byref(rep))
+    if failure:
+        raise failure[0]
+    _check(rc)
     x, v, cell = ev.get_state()
     system.x, system.v, system.cell = x, v, cell
     return DynamicsReport([_row(rows[k]) for k in range(min(nr.value, cap))], rep.steps, rep.seconds, rep.ns_per_day,
@@ -659,8 +690,9 @@ def _system_from(ptr):
                     fene_atoms=a(s.fene_atoms, 2 * s.n_fene, np.int32).reshape(-1, 2),
                     fene_param=a(s.fene_param, 4 * s.n_fene, np.float64).reshape(-1, 4))
     cell = np.array(list(s.cell), np.float64)
+    v = a(s.v, 3 * n, np.float64).reshape(-1, 3) if s.v else None
     lib().capsgpu_system_free(ptr)
-    return System(x=x, cell=cell, ff=ff)
+    return System(x=x, cell=cell, ff=ff, v=v)
 
 
 def build_melt(chains, seed=1, density=1.0):
@@ -681,6 +713,108 @@ def kremer_grest(chains=200, beads=50, density=0.85, k_theta=0.0, bond=0.97, see
     s.ff.cutoff = 2.0 ** (1.0 / 6.0)
     s.ff.coul14 = 1.0
     return s
+
+
+# ---- LAMMPS data files and checkpoints --------------------------------------------------------------------------------
+def _settings_path(path):
+    """the run settings beside a data file: X.restart.data -> X.restart.settings.json (cli/main.cpp:2229-2231)"""
+    return os.path.splitext(path)[0] + ".settings.json"
+
+
+def _excludes_13(ff):
+    """whether the force field excludes the end atoms of its angles (the melt's 1-3 exclusions; Kremer-Grest has none)"""
+    if len(ff.angle_atoms) == 0:
+        return False
+    i, k = int(ff.angle_atoms[0][0]), int(ff.angle_atoms[0][2])
+    return k in ff.excl_index[ff.excl_offset[i]:ff.excl_offset[i + 1]]
+
+
+def read_lammps(path, exclude13=None):
+    """read_lammps_data (core/src/io_lammps.cpp:58-297) for atom_style full, units real, with this path's force field
+    from the coefficient sections. The run settings a data file cannot hold (LJ shifted at its own cut-off, 1-4 Coulomb
+    scale, 1-3 exclusions) are taken from the settings file a checkpoint writes beside it, when there is one."""
+    import json
+    settings = {}
+    sp = _settings_path(path)
+    if os.path.exists(sp):
+        with open(sp) as f:
+            settings = json.load(f)
+    if exclude13 is None:
+        exclude13 = settings.get("exclude13", None)
+    p = C.POINTER(Sys)()
+    _check(lib().capsgpu_read_lammps(os.fsencode(path), -1 if exclude13 is None else int(bool(exclude13)), C.byref(p)))
+    s = _system_from(p)
+    with open(path, errors="replace") as f:
+        s.title = f.readline().strip()
+    for k in ("lj_shift", "cutoff", "coul14", "lj96"):
+        if k in settings:
+            setattr(s.ff, k, type(getattr(s.ff, k))(settings[k]))
+    return s
+
+
+def write_lammps(path, system, title="", velocities=True):
+    """write_lammps_data (core/src/io_lammps.cpp:505-583) with the coefficient sections of this path's force field and
+    17 significant digits, so the file reads back to the same numbers; written to path.tmp, then renamed."""
+    ff = system.ff
+    keep = []
+
+    def arr(a, dt, ptr):
+        a = np.ascontiguousarray(a, dtype=dt).reshape(-1)
+        if a.size == 0:
+            a = np.zeros(1, dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(ptr))
+
+    S = Sys()
+    S.n_atoms, S.n_types = ff.n, ff.ntypes
+    S.cell = (C.c_double * 12)(*np.asarray(system.cell, np.float64).reshape(12))
+    S.x = arr(system.x, np.float64, C.c_double)
+    S.type, S.charge, S.mass = arr(ff.type, np.int32, C.c_int32), arr(ff.charge, np.float64, C.c_double), arr(ff.mass, np.float64, C.c_double)
+    S.eps, S.sigma = arr(ff.eps, np.float64, C.c_double), arr(ff.sigma, np.float64, C.c_double)
+    S.n_bonds, S.bond_atoms, S.bond_param = len(ff.bond_atoms), arr(ff.bond_atoms, np.int32, C.c_int32), arr(ff.bond_param, np.float64, C.c_double)
+    S.n_angles, S.angle_atoms = len(ff.angle_atoms), arr(ff.angle_atoms, np.int32, C.c_int32)
+    S.angle_param = arr(ff.angle_param, np.float64, C.c_double)
+    S.n_fene, S.fene_atoms, S.fene_param = len(ff.fene_atoms), arr(ff.fene_atoms, np.int32, C.c_int32), arr(ff.fene_param, np.float64, C.c_double)
+    v = None
+    if velocities and system.v is not None:
+        v = np.ascontiguousarray(system.v, np.float64).reshape(-1)
+    _check(lib().capsgpu_write_lammps(os.fsencode(path), C.byref(S), _p(v), title.encode()))
+
+
+def write_checkpoint(path, system, step, total):
+    """a checkpoint as caps md writes one (cli/main.cpp:7820-7846): the state with velocities as a LAMMPS data file whose
+    title says the step, and the run settings beside it, each written whole (a temporary file, then renamed)"""
+    import json
+    write_lammps(path, system, "checkpoint step %d of %d" % (step, total))
+    ff = system.ff
+    sp = _settings_path(path)
+    with open(sp + ".tmp", "w") as f:
+        json.dump({"step": int(step), "total": int(total), "lj_shift": bool(ff.lj_shift), "cutoff": float(ff.cutoff),
+                   "coul14": float(ff.coul14), "lj96": bool(ff.lj96), "exclude13": _excludes_13(ff)}, f)
+    os.replace(sp + ".tmp", sp)
+
+
+def checkpoint_step(system):
+    """the step a checkpoint was written at, from its title (cli/main.cpp:7810-7816); raises ValueError for another file
+    or one without velocities"""
+    at = system.title.find("checkpoint step ")
+    if at < 0:
+        raise ValueError("not a checkpoint: its title does not say the step")
+    if system.v is None or len(system.v) != system.ff.n:
+        raise ValueError("the checkpoint carries no velocities")
+    return int(system.title[at + 16:].split()[0])
+
+
+def resume_options(system, options, total_steps):
+    """the options that continue a run from a checkpoint read with read_lammps: total_steps is the run's whole length,
+    the checkpoint's step becomes the step offset and its velocities are kept (cli/main.cpp:7809-7820)"""
+    import copy
+    done = checkpoint_step(system)
+    if done >= total_steps:
+        raise ValueError("the checkpoint is at step %d: the total number of steps must be larger" % done)
+    o = copy.copy(options)
+    o.step_offset, o.steps, o.new_velocities = done, total_steps - done, False
+    return o
 
 
 _MINIMISERS = {"sd": 0, "steepest": 0, "cg": 1, "lbfgs": 2, "l-bfgs": 2, "fire": 3}

@@ -10,7 +10,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcapsgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["api.hip", "nbnxm_build.hip", "nbnxm_prune.hip", "nbnxm_pairs.hip", "pme.hip", "step.hip", "mlp.hip", "batch.hip", "cooling.cpp", "relax.hip", "builders.cpp"]
+SOURCES = ["api.hip", "nbnxm_build.hip", "nbnxm_prune.hip", "nbnxm_pairs.hip", "pme.hip", "step.hip", "mlp.hip", "batch.hip", "cooling.cpp", "relax.hip", "builders.cpp", "lammps_io.cpp"]
 # the list build and pruning reproduce the reference's rounding: no contraction of a·b + c into an FMA there
 EXACT = {"nbnxm_build.hip", "nbnxm_prune.hip"}
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -845,6 +845,19 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     kick(*c, 0.0, true, kpart.p);
     emit(0, t);
     HIPCHECK(hipStreamSynchronize(s));
+    // checkpoints of a whole state only: every coordinate and velocity finite (dynamics.cpp:758-763)
+    std::vector<double> cpx, cpv;
+    auto checkpoint_due = [&](int64_t step) { return o->checkpoint && o->checkpoint_every > 0 && step % o->checkpoint_every == 0 && step < o->steps; };
+    auto checkpoint = [&](int64_t step) {
+      cpx.resize(3 * size_t(n)), cpv.resize(3 * size_t(n));
+      c->x.download(cpx.data(), cpx.size(), s);
+      c->v.download(cpv.data(), cpv.size(), s);
+      HIPCHECK(hipStreamSynchronize(s));
+      for (size_t k = 0; k < cpx.size(); ++k)
+        if (!std::isfinite(cpx[k]) || !std::isfinite(cpv[k])) return;
+      if (o->checkpoint(cpx.data(), cpv.data(), c->cell, step + o->step_offset, o->checkpoint_user) != 0)
+        throw Error(CAPSGPU_EINVAL, "the checkpoint callback failed at step " + std::to_string(step + o->step_offset));
+    };
     const auto t0 = std::chrono::steady_clock::now();
     bool alpha_pending = false;
     for (int64_t step = 1; step <= o->steps; ++step) {
@@ -885,10 +898,11 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
           if (!listed) check_flags(step);
           emit(step, t);
         }
+        if (checkpoint_due(step)) checkpoint(step);
         continue;
       }
       if (o->thermostat == 2) {
-        baoab_open(*c, hk, 0.5 * dt, c1, c2_now, sdv.p, o->seed, nullptr, uint64_t(step), lim_b, lim_p, bpart.p);
+        baoab_open(*c, hk, 0.5 * dt, c1, c2_now, sdv.p, o->seed, nullptr, uint64_t(step + o->step_offset), lim_b, lim_p, bpart.p);
       } else {
         vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p);
         alpha_pending = false;
@@ -920,6 +934,10 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
         if (o->thermostat == 2) bath_add(*c, bpart.p, scal.p + 1);
         if (!listed) check_flags(step);
         emit(step, t);
+      }
+      if (checkpoint_due(step)) {
+        if (alpha_pending) scale_v(*c, scal.p), alpha_pending = false;   // the velocities of the step, whole
+        checkpoint(step);
       }
     }
     if (alpha_pending) scale_v(*c, scal.p);

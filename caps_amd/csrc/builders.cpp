@@ -13,7 +13,6 @@
 #include <cstdint>
 #include <cstring>
 #include <random>
-#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -21,8 +20,11 @@
 #include "../../include/capsgpu.h"
 #include "../../include/capsgpu_build.h"
 #include "host_rng.hpp"
+#include "system.hpp"
 
 namespace {
+using capsgpu::exclusions;
+using capsgpu::Sys;
 thread_local std::string g_err;
 
 struct V3 {
@@ -35,48 +37,6 @@ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 V3 unit(V3 a) { return a * (1.0 / std::sqrt(dot(a, a))); }
 
-struct Sys {
-  capsgpu_system s{};
-  std::vector<double> x, charge, mass, eps, sigma, bond_param, angle_param, fene_param;
-  std::vector<int32_t> type, bond_atoms, angle_atoms, fene_atoms, excl_offset, excl_index;
-  void publish() {
-    s.x = x.data(), s.type = type.data(), s.charge = charge.data(), s.mass = mass.data(), s.eps = eps.data(), s.sigma = sigma.data();
-    s.n_bonds = int32_t(bond_atoms.size() / 2), s.bond_atoms = bond_atoms.data(), s.bond_param = bond_param.data();
-    s.n_angles = int32_t(angle_atoms.size() / 3), s.angle_atoms = angle_atoms.data(), s.angle_param = angle_param.data();
-    s.n_fene = int32_t(fene_atoms.size() / 2), s.fene_atoms = fene_atoms.data(), s.fene_param = fene_param.data();
-    s.excl_offset = excl_offset.data(), s.excl_index = excl_index.data();
-    s.opaque = this;
-  }
-};
-
-// 1-2 (and 1-3 when asked) partners from the bonds, sorted, as CSR
-void exclusions(Sys& S, int n, bool with13) {
-  std::vector<std::vector<int32_t>> nb(static_cast<size_t>(n)), ex(static_cast<size_t>(n));
-  for (size_t b = 0; b + 1 < S.bond_atoms.size(); b += 2) {
-    const int32_t i = S.bond_atoms[b], j = S.bond_atoms[b + 1];
-    nb[size_t(i)].push_back(j), nb[size_t(j)].push_back(i);
-  }
-  for (size_t b = 0; b + 1 < S.fene_atoms.size(); b += 2) {
-    const int32_t i = S.fene_atoms[b], j = S.fene_atoms[b + 1];
-    nb[size_t(i)].push_back(j), nb[size_t(j)].push_back(i);
-  }
-  for (int i = 0; i < n; ++i) {
-    std::set<int32_t> p;
-    for (int32_t j : nb[size_t(i)]) {
-      p.insert(j);
-      if (with13)
-        for (int32_t k : nb[size_t(j)])
-          if (k != i) p.insert(k);
-    }
-    ex[size_t(i)].assign(p.begin(), p.end());
-  }
-  S.excl_offset.assign(size_t(n) + 1, 0);
-  for (int i = 0; i < n; ++i) {
-    S.excl_offset[size_t(i) + 1] = S.excl_offset[size_t(i)] + int32_t(ex[size_t(i)].size());
-    S.excl_index.insert(S.excl_index.end(), ex[size_t(i)].begin(), ex[size_t(i)].end());
-  }
-  if (S.excl_index.empty()) S.excl_index.push_back(0);
-}
 
 // a cubic grid of the placed backbone carbons, for the self-avoidance test
 struct Grid {

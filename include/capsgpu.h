@@ -141,6 +141,8 @@ int32_t capsgpu_minimise(capsgpu_ctx* ctx, double* x, const double cell[12], con
                          capsgpu_relax_stage* out);
 
 /* caps::DynamicsOptions (core/include/caps/dynamics.hpp:60-154), the fields this path reads */
+typedef int32_t (*capsgpu_checkpoint_fn)(const double* x, const double* v, const double cell[12], int64_t step, void* user);
+
 typedef struct capsgpu_dyn_opts {
   double dt;                    /* fs */
   int64_t steps;
@@ -151,7 +153,8 @@ typedef struct capsgpu_dyn_opts {
   int32_t thermo_every;         /* steps between thermo rows */
   int32_t new_velocities;       /* 1: Maxwell–Boltzmann at temperature (dynamics.cpp:236-256), else the set state */
   int32_t respa;                /* r-RESPA inner steps for the bonded forces (1: off), Bussi or NVE only */
-  int64_t step_offset;          /* added to reported steps */
+  int64_t step_offset;          /* added to reported steps; Langevin's noise is drawn for step + step_offset, so a run
+                                   resumed from a checkpoint draws what the uninterrupted run would have */
   /* learned r-RESPA (no reference counterpart): 0 off; k > 1 a full PME evaluation every k steps, the MLP's predicted
    * increment of the reciprocal-space force in between, falling back to plain PME when its force RMSE exceeds
    * learned_tol (relative to the full force) at a full step */
@@ -169,6 +172,13 @@ typedef struct capsgpu_dyn_opts {
   int32_t barostat_every;       /* steps between cell updates (< 1: 10) */
   double temperature_end;       /* K; > 0 ramps the thermostat target linearly from temperature to this over the run
                                    (DynamicsOptions::temperature_end, dynamics.hpp:80); <= 0 no ramp (a zeroed struct) */
+  /* checkpoints (DynamicsOptions::checkpoint_every / checkpoint, dynamics.hpp:143-146; dynamics.cpp:758-763): every
+   * checkpoint_every steps (0: none) before the last, after the step, the full state — positions (3N, Å, as held),
+   * velocities (3N, Å/fs), cell and step (with step_offset) — when every coordinate and velocity is finite. The callback
+   * returns 0 to go on; anything else ends the run with CAPSGPU_EINVAL. */
+  int64_t checkpoint_every;
+  capsgpu_checkpoint_fn checkpoint;
+  void* checkpoint_user;
 } capsgpu_dyn_opts;
 
 /* caps::ThermoRow (core/include/caps/dynamics.hpp:28-43), the fields this path fills */
