@@ -978,7 +978,8 @@ int32_t capsgpu_set_tiles(capsgpu_ctx* c, int32_t lines_x, int32_t lines_yz, int
   return guard([&] {
     if (!c) throw Error(CAPSGPU_EINVAL, "null argument");
     auto pow2 = [](int32_t v) { return v >= 1 && v <= 16 && (v & (v - 1)) == 0; };
-    if (!pow2(lines_x) || !pow2(lines_yz)) throw Error(CAPSGPU_EINVAL, "tile shapes are powers of two from 1 to 16 lines");
+    if ((lines_x != 0 && !pow2(lines_x)) || !pow2(lines_yz))
+      throw Error(CAPSGPU_EINVAL, "tile shapes are powers of two from 1 to 16 lines (lines_x 0: the plane transform)");
     c->pme.lines_x = lines_x, c->pme.lines_yz = lines_yz;
     if (tiles) tiles[0] = lines_x, tiles[1] = lines_yz;
   });

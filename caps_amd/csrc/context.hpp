@@ -102,7 +102,9 @@ struct Pme {
   DevBuf<float2> tw;                    // twiddles of the three axes
   DevBuf<double> epart;                 // energy and virial partials
   int64_t erows = 0;                    // rows of epart written by the last evaluation with energies
-  int lines_x = 8, lines_yz = 16;       // the transforms' tile shape (line pairs along a, lines along b and c)
+  int lines_x = 0, lines_yz = 16;       // the transforms' tile shape (line pairs along a, lines along b and c); lines_x 0:
+                                        // a c-plane per workgroup for a and b together when it fits in LDS (else 8 pairs)
+  bool plane_ok = false;                // the grid's c-planes fit the plane transform (even K2, LDS)
 };
 
 struct Bonded {

@@ -9,6 +9,8 @@
 //                 once, no atomics, a fixed order)
 //  k_fft_x_r2c    two real lines per complex transform along a (as the reference), the half spectrum written
 //  k_fft_y        complex lines along b, 16 lines of consecutive a-frequencies per workgroup (coalesced)
+//  k_fft_xy       a whole c-plane per workgroup when it fits in LDS: the two passes above (or k_fft_y and
+//                 k_fft_x_c2r for the inverse) with the plane kept in LDS between them, one read and one write of it
 //  k_fft_z_conv   lines along c: forward transform, × influence function (energy and virial on energy steps),
 //                 inverse transform — the convolution without a round trip through HBM
 //  k_fft_x_c2r    back to the real grid, two lines per complex transform
@@ -17,6 +19,7 @@
 // forward sign +1 as the reference; twiddles from a table computed in double on the host.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "context.hpp"
 #include "prof.hpp"
@@ -394,29 +397,56 @@ __host__ __device__ inline int line_stride(int n) { return n + (n >> 4) + 1; }
 
 // one Stockham stage of radix R over nl lines of length n (ls apart): butterfly i of a line takes
 // x_q = src[i + q n/R] × w^(q k), w = exp(sign 2πi / (pR)), k = i mod p, and writes its DFT to dst[(i − k) R + k + q p];
-// twl holds exp(2πi m/n)
-template <int R>
-__device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, int ls, int p, const float2* twl, int sign) {
+// twl holds exp(2πi m/n). The butterfly is one function for the stages with run-time sizes and the compile-time ones
+// below, so both compute a line with the same instructions.
+template <int R, int Sign>
+__device__ inline void butterfly(const float2* s, float2* d, int i, int k, int nb, int p, int step, const float2* twl) {
+  float2 v[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const float2 x = s[lds_at(i + q * nb)];
+    if (q == 0) {
+      v[q] = x;
+    } else {
+      float2 w = twl[q * k * step];   // q k step < n
+      if (Sign < 0) w.y = -w.y;
+      v[q] = cmul(x, w);
+    }
+  }
+  dft<R>(v, Sign);
+  const int j = (i - k) * R + k;
+#pragma unroll
+  for (int q = 0; q < R; ++q) d[lds_at(j + q * p)] = v[q];
+}
+
+template <int R, int Sign>
+__device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, int ls, int p, const float2* twl) {
   const int nb = n / R, step = n / (p * R);
   for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
     const int line = t / nb, i = t - line * nb;
-    const int k = i % p;
-    float2 v[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const float2 x = src[line * ls + lds_at(i + q * nb)];
-      if (q == 0) {
-        v[q] = x;
-      } else {
-        float2 w = twl[q * k * step];   // q k step < n
-        if (sign < 0) w.y = -w.y;
-        v[q] = cmul(x, w);
-      }
-    }
-    dft<R>(v, sign);
-    const int j = (i - k) * R + k;
-#pragma unroll
-    for (int q = 0; q < R; ++q) dst[line * ls + lds_at(j + q * p)] = v[q];
+    butterfly<R, Sign>(src + line * ls, dst + line * ls, i, i % p, nb, p, step, twl);
+  }
+}
+
+// power-of-two lengths with every size a compile-time constant (the index arithmetic shifts and masks); the stages are
+// the plan's (radix 8 while it divides, then 4, then 2), so a line's result is the run-time stages' bit for bit
+template <int R, int N, int P, int Sign>
+__device__ inline void fft_stage_n(const float2* src, float2* dst, int nl, const float2* twl) {
+  constexpr int nb = N / R, step = N / (P * R), LS = N + (N >> 4) + 1;
+  for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
+    const int line = t / nb, i = t % nb;
+    butterfly<R, Sign>(src + line * LS, dst + line * LS, i, i % P, nb, P, step, twl);
+  }
+}
+template <int N, int Sign, int P>
+__device__ inline float2* stages_n(float2* src, float2* dst, int nl, const float2* twl) {
+  if constexpr (P >= N) {
+    return src;
+  } else {
+    constexpr int M = N / P, R = M % 8 == 0 ? 8 : M % 4 == 0 ? 4 : 2;
+    fft_stage_n<R, N, P, Sign>(src, dst, nl, twl);
+    __syncthreads();
+    return stages_n<N, Sign, P * R>(dst, src, nl, twl);
   }
 }
 
@@ -445,7 +475,18 @@ __device__ inline void store_twiddles(float2* twl, const TwFetch& r, const Fft1&
 
 // nl lines of length n in A (laid out as lds_at and line_stride say), transformed by the whole workgroup; returns the
 // buffer with the result
-__device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const float2* twl, int sign) {
+// (the sign a compile-time constant in every kernel, so each butterfly compiles to the same instructions wherever it runs
+// and every tile shape transforms a line bit-identically)
+template <int Sign>
+__device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const float2* twl) {
+  switch (F.n) {
+    case 32: return stages_n<32, Sign, 1>(A, B, nl, twl);
+    case 64: return stages_n<64, Sign, 1>(A, B, nl, twl);
+    case 128: return stages_n<128, Sign, 1>(A, B, nl, twl);
+    case 256: return stages_n<256, Sign, 1>(A, B, nl, twl);
+    case 512: return stages_n<512, Sign, 1>(A, B, nl, twl);
+    default: break;
+  }
   float2* src = A;
   float2* dst = B;
   const int n = F.n, ls = line_stride(n);
@@ -453,12 +494,12 @@ __device__ float2* fft_lines(float2* A, float2* B, int nl, const Fft1& F, const 
   for (int s = 0; s < F.nr; ++s) {
     const int R = int((F.rad >> (4 * s)) & 15u);
     switch (R) {
-      case 2: fft_stage<2>(src, dst, nl, n, ls, p, twl, sign); break;
-      case 3: fft_stage<3>(src, dst, nl, n, ls, p, twl, sign); break;
-      case 4: fft_stage<4>(src, dst, nl, n, ls, p, twl, sign); break;
-      case 5: fft_stage<5>(src, dst, nl, n, ls, p, twl, sign); break;
-      case 7: fft_stage<7>(src, dst, nl, n, ls, p, twl, sign); break;
-      default: fft_stage<8>(src, dst, nl, n, ls, p, twl, sign); break;
+      case 2: fft_stage<2, Sign>(src, dst, nl, n, ls, p, twl); break;
+      case 3: fft_stage<3, Sign>(src, dst, nl, n, ls, p, twl); break;
+      case 4: fft_stage<4, Sign>(src, dst, nl, n, ls, p, twl); break;
+      case 5: fft_stage<5, Sign>(src, dst, nl, n, ls, p, twl); break;
+      case 7: fft_stage<7, Sign>(src, dst, nl, n, ls, p, twl); break;
+      default: fft_stage<8, Sign>(src, dst, nl, n, ls, p, twl); break;
     }
     __syncthreads();
     float2* tmp = src;
@@ -503,7 +544,7 @@ __device__ __forceinline__ void b_fft_x_r2c(int bx, int gx, GridArgs G, FftPlan 
   }
   store_twiddles(twl, twf, FP.ax[0]);
   __syncthreads();
-  const float2* Z = fft_lines(A, B, LP, FP.ax[0], twl, +1);
+  const float2* Z = fft_lines<+1>(A, B, LP, FP.ax[0], twl);
   for (int t = threadIdx.x; t < LP * h1; t += blockDim.x) {
     const int pl = t / h1, m = t - pl * h1, pr = pb + pl;
     if (pr >= npairs) continue;
@@ -558,7 +599,7 @@ __device__ __forceinline__ void b_fft_x_c2r(int bx, int gx, GridArgs G, FftPlan 
   }
   store_twiddles(twl, twf, FP.ax[0]);
   __syncthreads();
-  const float2* Z = fft_lines(A, B, LP, FP.ax[0], twl, -1);
+  const float2* Z = fft_lines<-1>(A, B, LP, FP.ax[0], twl);
   for (int t = threadIdx.x; t < LP * K1; t += blockDim.x) {
     const int pl = t / K1, m = t - pl * K1, pr = pb + pl;
     if (pr >= npairs) continue;
@@ -574,7 +615,8 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
 }
 
 // lines along b of the half spectrum: element (m, j, k) at m + h1 (j + K2 k)
-__device__ __forceinline__ void b_fft_y(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half, int sign) {
+template <int Sign>
+__device__ __forceinline__ void b_fft_y(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half) {
   extern __shared__ float2 lds[];
   const int h1 = G.h1, K2 = G.k[1];
   const int LY = G.ly, lg = G.ly_log2;   // lines (consecutive a-frequencies) per workgroup
@@ -600,14 +642,158 @@ __device__ __forceinline__ void b_fft_y(int bx, int gx, GridArgs G, FftPlan FP, 
   }
   store_twiddles(twl, twf, FP.ax[1]);
   __syncthreads();
-  const float2* Z = fft_lines(A, B, nl, FP.ax[1], twl, sign);
+  const float2* Z = fft_lines<Sign>(A, B, nl, FP.ax[1], twl);
   for (int t = threadIdx.x; t < LY * K2; t += blockDim.x) {
     const int j = t >> lg, li = t & (LY - 1);
     if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * ls + lds_at(j)];
   }
 }
-__global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half, int sign) {
-  b_fft_y(blockIdx.x, gridDim.x, G, FP, tw, half, sign);
+template <int Sign>
+__global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half) {
+  b_fft_y<Sign>(blockIdx.x, gridDim.x, G, FP, tw, half);
+}
+
+// ---- a c-plane per workgroup: the transforms along a and b with the plane in LDS --------------------------------------
+// The plane k's K2 real lines along a, paired (2l, 2l + 1) into one complex transform each as in k_fft_x_r2c, split into
+// their half spectra, then the h1 lines along b as in k_fft_y: every line's arithmetic is those kernels', only the half
+// spectrum stays in LDS between the passes (one read of the plane and one write instead of two each). The inverse runs
+// the same steps backwards (k_fft_y(−1), then k_fft_x_c2r). Used when K2 is even (the pairs stay within a plane) and the
+// plane fits: LDS holds two regions of plane_region() elements (the Stockham stages' source and destination), then the
+// twiddles of a and b.
+constexpr int kPlaneThreads = 1024, kPlaneBatch = 8;
+__host__ __device__ inline int plane_region(int K1, int K2) {
+  const int h1 = K1 / 2 + 1, a = ((K2 + 1) / 2) * line_stride(K1), b = h1 * line_stride(K2);
+  return a > b ? a : b;
+}
+
+__device__ inline void plane_twiddles(const float2* __restrict__ tw, const FftPlan& FP, float2* twa, float2* twb) {
+  for (int t = threadIdx.x; t < FP.ax[0].n; t += blockDim.x) twa[t] = tw[FP.ax[0].tw + t];
+  for (int t = threadIdx.x; t < FP.ax[1].n; t += blockDim.x) twb[t] = tw[FP.ax[1].tw + t];
+}
+
+// KA, KB: the plane's sizes as compile-time constants (0: the run-time G.k), so the common grids index with shifts
+template <int KA, int KB>
+__device__ __forceinline__ void b_fft_xy_fwd(int kz, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
+                                             float2* __restrict__ half) {
+  extern __shared__ float2 lds[];
+  const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, ls1 = line_stride(K1), ls2 = line_stride(K2), npair = K2 / 2;
+  const int S = plane_region(K1, K2);
+  float2* A = lds;
+  float2* B = lds + S;
+  float2* twa = lds + 2 * S;
+  float2* twb = twa + K1;
+  // the plane's rows, row r into component r & 1 of pair line r >> 1; loads batched ahead of the LDS stores
+  const float* src = grid + size_t(kz) * K2 * K1;
+  float* Af = reinterpret_cast<float*>(A);
+  const int total = K2 * K1;
+  for (int t0 = 0; t0 < total; t0 += kPlaneBatch * kPlaneThreads) {
+    float r[kPlaneBatch];
+#pragma unroll
+    for (int u = 0; u < kPlaneBatch; ++u) {
+      const int t = t0 + u * kPlaneThreads + threadIdx.x;
+      r[u] = t < total ? src[t] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kPlaneBatch; ++u) {
+      const int t = t0 + u * kPlaneThreads + threadIdx.x;
+      if (t < total) {
+        const int row = t / K1, x = t - row * K1;
+        Af[2 * ((row >> 1) * ls1 + lds_at(x)) + (row & 1)] = r[u];
+      }
+    }
+  }
+  plane_twiddles(tw, FP, twa, twb);
+  __syncthreads();
+  float2* Z = fft_lines<+1>(A, B, npair, FP.ax[0], twa);
+  float2* Y = Z == A ? B : A;
+  // the pairs' half spectra as the b-lines' elements: line m (a-frequency), element j (row)
+  for (int t = threadIdx.x; t < npair * h1; t += blockDim.x) {
+    const int pl = t / h1, m = t - pl * h1;
+    const float2 zm = Z[pl * ls1 + lds_at(m)], zc0 = Z[pl * ls1 + lds_at((K1 - m) % K1)];
+    const float2 zc = make_float2(zc0.x, -zc0.y);
+    const float2 d = make_float2(zm.x - zc.x, zm.y - zc.y);
+    Y[m * ls2 + lds_at(2 * pl)] = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
+    Y[m * ls2 + lds_at(2 * pl + 1)] = make_float2(0.5f * d.y, -0.5f * d.x);
+  }
+  __syncthreads();
+  const float2* W = fft_lines<+1>(Y, Z, h1, FP.ax[1], twb);
+  float2* dst = half + size_t(kz) * K2 * h1;
+  for (int t = threadIdx.x; t < K2 * h1; t += blockDim.x) {
+    const int j = t / h1, m = t - j * h1;
+    dst[t] = W[m * ls2 + lds_at(j)];
+  }
+}
+
+template <int KA, int KB>
+__device__ __forceinline__ void b_fft_xy_inv(int kz, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
+                                             float* __restrict__ grid) {
+  extern __shared__ float2 lds[];
+  const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, ls1 = line_stride(K1), ls2 = line_stride(K2), npair = K2 / 2;
+  const int S = plane_region(K1, K2);
+  float2* A = lds;
+  float2* B = lds + S;
+  float2* twa = lds + 2 * S;
+  float2* twb = twa + K1;
+  const float2* src = half + size_t(kz) * K2 * h1;
+  const int total = K2 * h1;
+  for (int t0 = 0; t0 < total; t0 += kPlaneBatch * kPlaneThreads) {
+    float2 r[kPlaneBatch];
+#pragma unroll
+    for (int u = 0; u < kPlaneBatch; ++u) {
+      const int t = t0 + u * kPlaneThreads + threadIdx.x;
+      r[u] = t < total ? src[t] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < kPlaneBatch; ++u) {
+      const int t = t0 + u * kPlaneThreads + threadIdx.x;
+      if (t < total) {
+        const int j = t / h1, m = t - j * h1;
+        A[m * ls2 + lds_at(j)] = r[u];
+      }
+    }
+  }
+  plane_twiddles(tw, FP, twa, twb);
+  __syncthreads();
+  float2* Y = fft_lines<-1>(A, B, h1, FP.ax[1], twb);
+  float2* X = Y == A ? B : A;
+  // two rows' half spectra rebuilt whole into one complex line, Z = A + iB
+  for (int t = threadIdx.x; t < npair * K1; t += blockDim.x) {
+    const int pl = t / K1, x = t - pl * K1;
+    const int m = x > K1 / 2 ? K1 - x : x;
+    float2 a = Y[m * ls2 + lds_at(2 * pl)], b = Y[m * ls2 + lds_at(2 * pl + 1)];
+    if (x > K1 / 2) a.y = -a.y, b.y = -b.y;
+    X[pl * ls1 + lds_at(x)] = make_float2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  const float2* Z = fft_lines<-1>(X, Y, npair, FP.ax[0], twa);
+  const float* Zf = reinterpret_cast<const float*>(Z);
+  float* dst = grid + size_t(kz) * K2 * K1;
+  for (int t = threadIdx.x; t < K2 * K1; t += blockDim.x) {
+    const int row = t / K1, x = t - row * K1;
+    dst[t] = Zf[2 * ((row >> 1) * ls1 + lds_at(x)) + (row & 1)];
+  }
+}
+
+__device__ __forceinline__ void b_fft_xy(int kz, const GridArgs& G, const FftPlan& FP, const float2* __restrict__ tw, float* __restrict__ grid,
+                                         float2* __restrict__ half, int forward) {
+  const int K1 = G.k[0], K2 = G.k[1];
+#define CAPSGPU_PLANE(A, B)                                      \
+  if (K1 == A && K2 == B) {                                      \
+    if (forward) b_fft_xy_fwd<A, B>(kz, G, FP, tw, grid, half);  \
+    else b_fft_xy_inv<A, B>(kz, G, FP, tw, half, grid);          \
+    return;                                                      \
+  }
+  CAPSGPU_PLANE(64, 64)
+  CAPSGPU_PLANE(128, 128)
+  CAPSGPU_PLANE(256, 256)
+#undef CAPSGPU_PLANE
+  if (forward) b_fft_xy_fwd<0, 0>(kz, G, FP, tw, grid, half);
+  else b_fft_xy_inv<0, 0>(kz, G, FP, tw, half, grid);
+}
+
+__global__ void __launch_bounds__(kPlaneThreads) k_fft_xy(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float* __restrict__ grid,
+                                                         float2* __restrict__ half, int forward) {
+  b_fft_xy(blockIdx.x, G, FP, tw, grid, half, forward);
 }
 
 // the influence function and its virial weights at half-spectrum index (a, b, c), in double (f90:518-564)
@@ -692,7 +878,7 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
   }
   store_twiddles(twl, twf, FP.ax[2]);
   __syncthreads();
-  float2* Z = fft_lines(A, B, nl, FP.ax[2], twl, +1);
+  float2* Z = fft_lines<+1>(A, B, nl, FP.ax[2], twl);
   double e = 0, w[6] = {0, 0, 0, 0, 0, 0};
   for (int t0 = 0; t0 < nl * K3; t0 += kFftBatch * kFftThreads) {
     float gb[kFftBatch];
@@ -722,7 +908,7 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
   }
   __syncthreads();
   float2* other = Z == A ? B : A;
-  const float2* R = fft_lines(Z, other, nl, FP.ax[2], twl, -1);
+  const float2* R = fft_lines<-1>(Z, other, nl, FP.ax[2], twl);
   for (int t = threadIdx.x; t < LY * K3; t += blockDim.x) {
     const int k = t >> lg, li = t & (LY - 1);
     if (li < nl) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
@@ -868,7 +1054,7 @@ GridArgs grid_args(const Pme& P) {
   G.ntm = std::max(G.nt[0], std::max(G.nt[1], G.nt[2]));
   G.p = P.order;
   G.h1 = P.h1;
-  G.lx = P.lines_x;
+  G.lx = P.lines_x > 0 ? P.lines_x : P.plane_ok ? 0 : 8;   // 0: the plane transform
   G.ly = P.lines_yz;
   G.ly_log2 = 0;
   while ((1 << G.ly_log2) < G.ly) ++G.ly_log2;
@@ -930,6 +1116,28 @@ static int good_fft_size(int n) {
     if (k == 1) return m;
     ++m;
   }
+}
+
+// the plane transform's dynamic LDS, and whether a grid's planes take it on this device: K2 even and the LDS within the
+// device's limit per workgroup (the kernel's limit raised to it once)
+size_t plane_lds(int K1, int K2) { return (2 * size_t(plane_region(K1, K2)) + size_t(K1) + size_t(K2)) * sizeof(float2); }
+
+bool plane_fits(int device, int K1, int K2) {
+  if (K2 % 2) return false;
+  static std::mutex mu;
+  static int limit[64];
+  static bool known[64];
+  if (device < 0 || device >= 64) return false;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!known[device]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) v = 0;
+    if (v > 0 && hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_xy), hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess)
+      v = 65536;
+    (void)hipGetLastError();
+    limit[device] = v, known[device] = true;
+  }
+  return plane_lds(K1, K2) <= size_t(limit[device]);
 }
 
 void pme_grid_dims(const capsgpu_ctx& c, const double cell[12], int k[3]) {
@@ -1010,6 +1218,7 @@ void pme_setup(capsgpu_ctx& c) {
   P.home.upload(home, s);
   P.grid.resize(size_t(P.k[0]) * P.k[1] * P.k[2]);
   P.half.resize(nh);
+  P.plane_ok = plane_fits(c.device, P.k[0], P.k[1]);
   for (int q = 0; q < 12; ++q) P.cell[q] = cell[q];
   P.green_ok = true;
   P.counts_clear = false;   // a new grid: its tile counters are cleared before the next spread
@@ -1038,7 +1247,8 @@ void launch_fft_y(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, int sign)
   const int mblocks = (P.h1 + G.ly - 1) / G.ly;
   const size_t lds_y = (2 * size_t(G.ly) * size_t(line_stride(P.k[1])) + P.k[1]) * sizeof(float2);
   kt_begin(c, "k_fft_y");
-  hipLaunchKernelGGL(k_fft_y, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, c.stream, G, F, P.tw.p, P.half.p, sign);
+  if (sign > 0) hipLaunchKernelGGL(k_fft_y<+1>, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, c.stream, G, F, P.tw.p, P.half.p);
+  else hipLaunchKernelGGL(k_fft_y<-1>, dim3(unsigned(mblocks * P.k[2])), dim3(256), lds_y, c.stream, G, F, P.tw.p, P.half.p);
   LAUNCHCHECK();
   kt_end(c);
 }
@@ -1063,19 +1273,40 @@ void launch_fft_z_conv(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool
   kt_end(c);
 }
 
+void launch_fft_xy(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool forward) {
+  Pme& P = c.pme;
+  kt_begin(c, forward ? "k_fft_xy_fwd" : "k_fft_xy_inv");
+  hipLaunchKernelGGL(k_fft_xy, dim3(unsigned(P.k[2])), dim3(kPlaneThreads), plane_lds(P.k[0], P.k[1]), c.stream, G, F, P.tw.p, P.grid.p,
+                     P.half.p, forward ? 1 : 0);
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+// the passes along a and b: one plane kernel (G.lx = 0) or the a-lines and b-lines kernels
+void fft_ab(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool forward) {
+  if (G.lx == 0) {
+    launch_fft_xy(c, G, F, forward);
+  } else if (forward) {
+    launch_fft_x(c, G, F, true);
+    launch_fft_y(c, G, F, +1);
+  } else {
+    launch_fft_y(c, G, F, -1);
+    launch_fft_x(c, G, F, false);
+  }
+}
+
 // mesh → half spectrum → × influence function → mesh, in place on P.grid
 void fft_forward_conv_inverse(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool want) {
-  launch_fft_x(c, G, F, true);
-  launch_fft_y(c, G, F, +1);
+  fft_ab(c, G, F, true);
   launch_fft_z_conv(c, G, F, want);
-  launch_fft_y(c, G, F, -1);
-  launch_fft_x(c, G, F, false);
+  fft_ab(c, G, F, false);
 }
 }  // namespace
 
-// the transforms' tile shape of this context, picked by timing on its own mesh: the forward and inverse passes along a
-// with 2, 4, 8 or 16 line pairs per workgroup, then the passes along b and c (with the convolution) with 4, 8 or 16
-// lines, each the best of three runs timed with HIP events on the context's stream. The passes run in place on the
+// the transforms' tile shape of this context, picked by timing on its own mesh: the passes along b and c (with the
+// convolution) with 4, 8 or 16 lines per workgroup, then the passes along a and b — the plane kernel when the planes fit
+// (lines_x 0), or the a-lines with 2, 4, 8 or 16 line pairs per workgroup and the b-lines at the chosen width — each the
+// best of three runs timed with HIP events on the context's stream. The passes run in place on the
 // mesh of the last evaluation, which the next evaluation overwrites; every shape transforms each line with the same
 // arithmetic, so the choice changes the time and not the forces.
 void pme_tune(capsgpu_ctx& c, int out[2]) {
@@ -1102,16 +1333,8 @@ void pme_tune(capsgpu_ctx& c, int out[2]) {
   };
   try {
     float tbest = 1e30f;
-    int pick = P.lines_x;
-    for (int lx : {2, 4, 8, 16}) {
-      P.lines_x = lx;
-      const GridArgs G = grid_args(P);
-      const float t = best_of([&] { launch_fft_x(c, G, F, true), launch_fft_x(c, G, F, false); });
-      if (t < tbest) tbest = t, pick = lx;
-    }
-    P.lines_x = pick;
-    tbest = 1e30f;
-    pick = P.lines_yz;
+    int pick = P.lines_yz;
+    P.lines_x = 8;
     for (int ly : {4, 8, 16}) {
       P.lines_yz = ly;
       const GridArgs G = grid_args(P);
@@ -1119,6 +1342,16 @@ void pme_tune(capsgpu_ctx& c, int out[2]) {
       if (t < tbest) tbest = t, pick = ly;
     }
     P.lines_yz = pick;
+    tbest = 1e30f;
+    pick = 8;
+    for (int lx : {0, 2, 4, 8, 16}) {
+      if (lx == 0 && !P.plane_ok) continue;
+      P.lines_x = lx;
+      const GridArgs G = grid_args(P);
+      const float t = best_of([&] { fft_ab(c, G, F, true), fft_ab(c, G, F, false); });
+      if (t < tbest) tbest = t, pick = lx;
+    }
+    P.lines_x = pick;
   } catch (...) {
     c.profile = prof;
     (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
@@ -1302,10 +1535,17 @@ __global__ void __launch_bounds__(256) kg_fft_x(const PmeRep* __restrict__ R, co
   else b_fft_x_c2r(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.F, a.tw, a.half, a.grid);
 }
 
-__global__ void __launch_bounds__(256) kg_fft_y(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr, int sign) {
+__global__ void __launch_bounds__(kPlaneThreads) kg_fft_xy(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr, int forward) {
   const int r = find_rep(off, nr, blockIdx.x);
   const PmeRep& a = R[r];
-  b_fft_y(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.F, a.tw, a.half, sign);
+  b_fft_xy(int(blockIdx.x) - off[r], a.G, a.F, a.tw, a.grid, a.half, forward);
+}
+
+template <int Sign>
+__global__ void __launch_bounds__(256) kg_fft_y(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const PmeRep& a = R[r];
+  b_fft_y<Sign>(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.F, a.tw, a.half);
 }
 
 __global__ void __launch_bounds__(256) kg_fft_z_conv(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
@@ -1319,9 +1559,9 @@ __global__ void __launch_bounds__(256) kg_fft_z_conv(const PmeRep* __restrict__ 
 struct PmeGroup {
   int nr = 0, order = 0;
   DevBuf<unsigned char> reps;                         // PmeRep[nr]
-  DevBuf<int32_t> off_atoms, off_tiles, off_x, off_yz, off_z;   // workgroup prefixes per launch
-  int h_atoms = 0, h_tiles = 0, h_x = 0, h_yz = 0, h_z = 0;       // the launches' workgroups
-  size_t lds_x = 0, lds_y = 0, lds_z = 0;
+  DevBuf<int32_t> off_atoms, off_tiles, off_x, off_yz, off_z, off_xy;   // workgroup prefixes per launch
+  int h_atoms = 0, h_tiles = 0, h_x = 0, h_yz = 0, h_z = 0, h_xy = 0;       // the launches' workgroups
+  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_xy = 0;
 };
 
 PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
@@ -1329,7 +1569,7 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
   g->nr = int(cs.size());
   if (g->nr < 1) throw Error(-1, "a PME group needs a replica");
   std::vector<PmeRep> R(cs.size());
-  std::vector<int32_t> oa(1, 0), ot(1, 0), ox(1, 0), oy(1, 0), oz(1, 0);
+  std::vector<int32_t> oa(1, 0), ot(1, 0), ox(1, 0), oy(1, 0), oz(1, 0), oxy(1, 0);
   for (size_t i = 0; i < cs.size(); ++i) {
     capsgpu_ctx& c = *cs[i];
     Pme& P = c.pme;
@@ -1348,25 +1588,34 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
     a.wsort = reinterpret_cast<float4*>(P.wsort.p), a.bsort = reinterpret_cast<int4*>(P.bsort.p);
     a.home = P.home.p, a.grid = P.grid.p, a.half = P.half.p, a.tw = P.tw.p, a.gtab = P.gtab.p, a.bmod = P.bmod.p;
     a.fac = kPi * kPi / (P.beta * P.beta), a.vol = P.vol, a.fpme = P.fpme.p;
+    // a replica on the plane transform (G.lx = 0) has workgroups in the plane launches only, the others in the a- and
+    // b-line launches only
     const int npairs = (P.k[1] * P.k[2] + 1) / 2, mblocks = (P.h1 + G.ly - 1) / G.ly;
+    const bool plane = G.lx == 0;
     oa.push_back(oa.back() + int(blocks_for(size_t(c.n), 256)));
     ot.push_back(ot.back() + a.ntiles);
-    ox.push_back(ox.back() + (npairs + G.lx - 1) / G.lx);
-    oy.push_back(oy.back() + mblocks * P.k[2]);
+    ox.push_back(ox.back() + (plane ? 0 : (npairs + G.lx - 1) / G.lx));
+    oy.push_back(oy.back() + (plane ? 0 : mblocks * P.k[2]));
     oz.push_back(oz.back() + mblocks * P.k[1]);
-    g->lds_x = std::max(g->lds_x, (2 * size_t(G.lx) * size_t(line_stride(P.k[0])) + P.k[0]) * sizeof(float2));
+    oxy.push_back(oxy.back() + (plane ? P.k[2] : 0));
+    if (plane) g->lds_xy = std::max(g->lds_xy, plane_lds(P.k[0], P.k[1]));
+    else g->lds_x = std::max(g->lds_x, (2 * size_t(G.lx) * size_t(line_stride(P.k[0])) + P.k[0]) * sizeof(float2));
     g->lds_y = std::max(g->lds_y, (2 * size_t(G.ly) * size_t(line_stride(P.k[1])) + P.k[1]) * sizeof(float2));
     g->lds_z = std::max(g->lds_z, (2 * size_t(G.ly) * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2));
   }
   g->reps.upload(reinterpret_cast<const unsigned char*>(R.data()), R.size() * sizeof(PmeRep), s);
   g->off_atoms.upload(oa, s), g->off_tiles.upload(ot, s), g->off_x.upload(ox, s), g->off_yz.upload(oy, s), g->off_z.upload(oz, s);
-  g->h_atoms = oa.back(), g->h_tiles = ot.back(), g->h_x = ox.back(), g->h_yz = oy.back(), g->h_z = oz.back();
+  g->off_xy.upload(oxy, s);
+  if (g->lds_xy > 65536)
+    HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_fft_xy), hipFuncAttributeMaxDynamicSharedMemorySize, int(g->lds_xy)));
+  g->h_atoms = oa.back(), g->h_tiles = ot.back(), g->h_x = ox.back(), g->h_yz = oy.back(), g->h_z = oz.back(), g->h_xy = oxy.back();
   return g.release();
 }
 
 void pme_group_destroy(PmeGroup* g) { delete g; }
 
-// the reciprocal forces of every replica into its pme.fpme: eleven launches for the whole group
+// the reciprocal forces of every replica into its pme.fpme: nine launches for the whole group when every replica is on
+// the plane transform, eleven when none is
 void pme_group_forces(PmeGroup& g, hipStream_t s) {
   const PmeRep* R = reinterpret_cast<const PmeRep*>(g.reps.p);
   const int nr = g.nr;
@@ -1387,16 +1636,31 @@ void pme_group_forces(PmeGroup& g, hipStream_t s) {
 #undef CAPSGPU_GSPREAD
   }
   LAUNCHCHECK();
-  hipLaunchKernelGGL(kg_fft_x, dim3(unsigned(nx)), dim3(256), g.lds_x, s, R, g.off_x.p, nr, 1);
-  LAUNCHCHECK();
-  hipLaunchKernelGGL(kg_fft_y, dim3(unsigned(ny)), dim3(256), g.lds_y, s, R, g.off_yz.p, nr, +1);
-  LAUNCHCHECK();
+  // the passes along a and b: the plane kernel for the replicas on it, the line kernels for the others (a launch with no
+  // workgroups is skipped)
+  const int nxy = g.h_xy;
+  if (nxy) {
+    hipLaunchKernelGGL(kg_fft_xy, dim3(unsigned(nxy)), dim3(kPlaneThreads), g.lds_xy, s, R, g.off_xy.p, nr, 1);
+    LAUNCHCHECK();
+  }
+  if (nx) {
+    hipLaunchKernelGGL(kg_fft_x, dim3(unsigned(nx)), dim3(256), g.lds_x, s, R, g.off_x.p, nr, 1);
+    LAUNCHCHECK();
+    hipLaunchKernelGGL(kg_fft_y<+1>, dim3(unsigned(ny)), dim3(256), g.lds_y, s, R, g.off_yz.p, nr);
+    LAUNCHCHECK();
+  }
   hipLaunchKernelGGL(kg_fft_z_conv, dim3(unsigned(nz)), dim3(256), g.lds_z, s, R, g.off_z.p, nr);
   LAUNCHCHECK();
-  hipLaunchKernelGGL(kg_fft_y, dim3(unsigned(ny)), dim3(256), g.lds_y, s, R, g.off_yz.p, nr, -1);
-  LAUNCHCHECK();
-  hipLaunchKernelGGL(kg_fft_x, dim3(unsigned(nx)), dim3(256), g.lds_x, s, R, g.off_x.p, nr, 0);
-  LAUNCHCHECK();
+  if (nx) {
+    hipLaunchKernelGGL(kg_fft_y<-1>, dim3(unsigned(ny)), dim3(256), g.lds_y, s, R, g.off_yz.p, nr);
+    LAUNCHCHECK();
+    hipLaunchKernelGGL(kg_fft_x, dim3(unsigned(nx)), dim3(256), g.lds_x, s, R, g.off_x.p, nr, 0);
+    LAUNCHCHECK();
+  }
+  if (nxy) {
+    hipLaunchKernelGGL(kg_fft_xy, dim3(unsigned(nxy)), dim3(kPlaneThreads), g.lds_xy, s, R, g.off_xy.p, nr, 0);
+    LAUNCHCHECK();
+  }
   switch (g.order) {
 #define CAPSGPU_GGATHER(Q) \
   case Q: hipLaunchKernelGGL(kg_gather<Q>, dim3(unsigned(nt)), dim3(64), lds_gather, s, R, g.off_tiles.p, nr); break;
