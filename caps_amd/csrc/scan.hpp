@@ -1,5 +1,6 @@
 // Internal: device-wide exclusive prefix sums (uint32 or int32 counts -> int64 offsets), deterministic.
-// Three levels at most: blocks of 2048 items scanned in LDS, the block totals scanned recursively, offsets added back.
+// Blocks of 2048 items scanned in LDS; up to 64 blocks, each block then adds the totals of the blocks before it (a wave
+// sums them); beyond, the block totals are scanned recursively (three levels at most) and the offsets added back.
 #pragma once
 
 #include "common.hpp"
@@ -59,6 +60,24 @@ static __global__ void __launch_bounds__(kScanThreads) k_scan_add(int64_t* __res
   }
 }
 
+// the offsets of at most 64 blocks summed by each block's first wave from the block totals, then added
+static __global__ void __launch_bounds__(kScanThreads) k_scan_add_small(int64_t* __restrict__ out, size_t n, const int64_t* __restrict__ sums) {
+  __shared__ int64_t off;
+  if (threadIdx.x < 64) {
+    int64_t v = int(threadIdx.x) < int(blockIdx.x) ? sums[threadIdx.x] : 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (threadIdx.x == 0) off = v;
+  }
+  __syncthreads();
+  const int64_t a = off;
+  const size_t i = size_t(blockIdx.x) * kScanTile + threadIdx.x;
+  for (int k = 0; k < kScanItems; ++k) {
+    const size_t j = i + size_t(k) * kScanThreads;
+    if (j < n) out[j] += a;
+  }
+}
+
 // out[i] = Σ_{k<i} in[k] for i ≤ n (out has n + 1 entries: out[n] is the total)
 struct Scanner {
   DevBuf<int64_t> tsum[4], toff[4];
@@ -78,7 +97,10 @@ struct Scanner {
     // scan n + 1 items, the last being 0, so out[n] is the total
     hipLaunchKernelGGL(k_scan_tiles<T>, dim3(unsigned(tiles)), dim3(kScanThreads), 0, s, in, out, n, sums.p);
     LAUNCHCHECK();
-    if (tiles > 1) {
+    if (tiles > 1 && tiles <= 64) {
+      hipLaunchKernelGGL(k_scan_add_small, dim3(unsigned(tiles)), dim3(kScanThreads), 0, s, out, n + 1, sums.p);
+      LAUNCHCHECK();
+    } else if (tiles > 1) {
       DevBuf<int64_t>& offs = toff[depth];
       offs.resize(tiles + 1);
       run<int64_t>(sums.p, offs.p, tiles, depth + 1, s);
