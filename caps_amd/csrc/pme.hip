@@ -4,9 +4,11 @@
 //
 //  k_splines      per atom: fractional grid coordinate, first grid point, B-spline weights and derivatives; home tile
 //  k_tile_*       atoms binned by the 4x4x4 grid tile of their first point (sorted by atom in each tile)
-//  k_spread       one 64-lane workgroup per tile: each point sums q·θa·θb·θc over the atoms whose support covers it,
-//                 staged in LDS, a lane taking a row of 4 points over a quarter of the atoms (every point written
-//                 once, no atomics, a fixed order)
+//  k_spread_blk   one 256-thread workgroup per block of 2×2×2 tiles (every K a multiple of 8): each atom of the home
+//                 tiles that reach the block staged once (splines, weight windows) in LDS, then each wave sums two
+//                 tiles, a lane taking a row of 4 points over a quarter of the atoms (every point written once, no
+//                 atomics, a fixed order)
+//  k_spread       the same per single tile (64 lanes) for the other grids
 //  k_fft_x_r2c    two real lines per complex transform along a (as the reference), the half spectrum written
 //  k_fft_y        complex lines along b, 16 lines of consecutive a-frequencies per workgroup (coalesced)
 //  k_fft_xy       a whole c-plane per workgroup when it fits in LDS: the two passes above (or k_fft_y and
@@ -302,6 +304,165 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
   const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
   if (tile >= ntiles) return;
   b_spread<P>(tile, G, home, tstart, wsort, bsort, grid);
+}
+
+// ---- the spread by blocks of 2×2×2 tiles (every K a multiple of 8) ---------------------------------------------------
+// k_spread stages each atom once per tile it reaches (eight tiles at order 5), its splines recomputed each time; here a
+// workgroup of four waves takes a block of eight tiles and stages each atom of the home tiles that reach the block
+// once: its splines, and per axis the NTP windows of four weights it has on the tiles past its home tile (θa, θb,
+// q·θc), kept in LDS by element so every read of the accumulation meets distinct banks or one broadcast. Each wave then
+// sums two tiles, lane = (row (j, k) of the tile, quarter of the candidates): per candidate one row weight θb·qθc and
+// one float4 of θa from LDS, four FMAs. A tile's candidates are walked home tile by home tile (each with fixed window
+// offsets), in (c, b, a) order; the quarters are added in a fixed order and every mesh point is written once.
+constexpr int kBlkThreads = 256, kBlkStage = 256, kBlkPitch = kBlkStage + 4;   // atoms staged per chunk: at order 5 the
+                                                                                 // block's 27 home tiles hold ~216
+
+// θ[idx − s] for a compile-time idx (the window element) and a first-point offset s ∈ [0, 4), zero outside [0, P)
+template <int P>
+__device__ __forceinline__ float shifted(const float* th, int idx, int s) {
+  float v = 0.0f;
+#pragma unroll
+  for (int ss = 0; ss < 4; ++ss) {
+    const int m = idx - ss;
+    if (m >= 0 && m < P) v = s == ss ? th[m] : v;
+  }
+  return v;
+}
+
+template <int P>
+__device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const int64_t* __restrict__ tstart, const float4* __restrict__ wsort,
+                                             const int4* __restrict__ bsort, float* __restrict__ grid) {
+  constexpr int NTP = (P + 6) / 4, NA = NTP + 1, NH = NA * NA * NA;   // home tiles per axis that reach the block, in all
+  static_assert(NH <= 64, "a lane per home tile");
+  __shared__ float4 sWa[2 * kBlkStage];          // θa of staged atom f on the block's tile column dx at dx·stage + f
+  __shared__ float sWbc[2 * 4 * NTP * kBlkPitch];   // element e = 4o + j of the θb windows at e·pitch + f, then q·θc's
+  __shared__ int s_ex[NH + 1], s_hb[NH];         // home tile h's first staged place and first atom
+  float* sWb = sWbc;
+  float* sWc = sWbc + 4 * NTP * kBlkPitch;
+  static_assert(sizeof(sWbc) >= 8 * 4 * 64 * sizeof(float), "the tiles' partial sums reuse the windows' space");
+  float(*s_part)[4][64] = reinterpret_cast<float(*)[4][64]>(sWbc);   // per tile: the quarters' sums by point (after the chunks)
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int nbx = G.nt[0] / 2, nby = G.nt[1] / 2;
+  const int bx = blk % nbx, by = (blk / nbx) % nby, bz = blk / (nbx * nby);
+  const int tx0 = 2 * bx, ty0 = 2 * by, tz0 = 2 * bz;
+  // home tile h = ix + NA (iy + NA iz): tiles t0 − NTP + 1 + i along each axis (periodic)
+  if (wave == 0) {
+    int cnt = 0, hb = 0;
+    if (lane < NH) {
+      const int ix = lane % NA, iy = (lane / NA) % NA, iz = lane / (NA * NA);
+      int hx = tx0 - NTP + 1 + ix, hy = ty0 - NTP + 1 + iy, hz = tz0 - NTP + 1 + iz;
+      hx += hx < 0 ? G.nt[0] : 0, hx -= hx >= G.nt[0] ? G.nt[0] : 0;
+      hy += hy < 0 ? G.nt[1] : 0, hy -= hy >= G.nt[1] ? G.nt[1] : 0;
+      hz += hz < 0 ? G.nt[2] : 0, hz -= hz >= G.nt[2] ? G.nt[2] : 0;
+      const int h = (hz * G.nt[1] + hy) * G.nt[0] + hx;
+      hb = int(tstart[h]);
+      cnt = int(tstart[h + 1] - tstart[h]);
+    }
+    const int own = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(cnt, d, 64);
+      if (lane >= d) cnt += o;
+    }
+    if (lane < NH) s_ex[lane] = cnt - own, s_hb[lane] = hb;
+    if (lane == NH - 1) s_ex[NH] = cnt;
+  }
+  __syncthreads();
+  const int m = s_ex[NH];
+  // this wave's two tiles: dx = 0, 1 at (dy, dz) = (wave & 1, wave >> 1); lane = (row j + 4k, quarter)
+  const int dy = wave & 1, dz = wave >> 1;
+  const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4;
+  float acc[2][4] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+  for (int c0 = 0; c0 < m; c0 += kBlkStage) {
+    const int f = c0 + tid;
+    if (tid < kBlkStage && f < m) {
+      // the home tile of staged place f: the last h with s_ex[h] <= f (empty tiles are passed over)
+      int lo = 0, hi = NH - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_ex[mid] <= f) lo = mid;
+        else hi = mid - 1;
+      }
+      const int at = s_hb[lo] + (f - s_ex[lo]), ix = lo % NA;
+      const float4 w = wsort[at];
+      const int4 b = bsort[at];
+      float ta[P], tb[P], tc[P], dd[P];
+      bspline<P>(w.x, ta, dd);
+      bspline<P>(w.y, tb, dd);
+      bspline<P>(w.z, tc, dd);
+#pragma unroll
+      for (int j = 0; j < P; ++j) tc[j] = w.w * tc[j];
+      const int sx = b.x & 3, sy = b.y & 3, sz = b.z & 3;
+      // θa on the tile columns dx = 0, 1 of the block: the window (dx + NTP − 1 − ix) past the home tile, or zero
+      // (two named values, not an array: a select into an array indexed by ix would go through scratch)
+      float4 w0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w1 = w0;
+      const int o0 = NTP - 1 - ix, o1 = NTP - ix;
+#pragma unroll
+      for (int o = 0; o < NTP; ++o) {
+        const float e0 = shifted<P>(ta, 4 * o, sx), e1 = shifted<P>(ta, 4 * o + 1, sx), e2 = shifted<P>(ta, 4 * o + 2, sx),
+                    e3 = shifted<P>(ta, 4 * o + 3, sx);
+        const bool h0 = o0 == o, h1 = o1 == o;
+        w0.x = h0 ? e0 : w0.x, w0.y = h0 ? e1 : w0.y, w0.z = h0 ? e2 : w0.z, w0.w = h0 ? e3 : w0.w;
+        w1.x = h1 ? e0 : w1.x, w1.y = h1 ? e1 : w1.y, w1.z = h1 ? e2 : w1.z, w1.w = h1 ? e3 : w1.w;
+      }
+      sWa[tid] = w0, sWa[kBlkStage + tid] = w1;
+#pragma unroll
+      for (int o = 0; o < NTP; ++o) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          sWb[(4 * o + u) * kBlkPitch + tid] = shifted<P>(tb, 4 * o + u, sy);
+          sWc[(4 * o + u) * kBlkPitch + tid] = shifted<P>(tc, 4 * o + u, sz);
+        }
+      }
+    }
+    __syncthreads();
+    const int c1 = min(m, c0 + kBlkStage);
+    // per tile column dx and home row (iy, iz): the home tiles dx … dx + NTP − 1 along a, consecutive in the staged
+    // order, each atom's θa on this column staged for it
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+      const float4* pa = sWa + dx * kBlkStage - c0;
+#pragma unroll
+      for (int iz = dz; iz < dz + NTP; ++iz) {
+#pragma unroll
+        for (int iy = dy; iy < dy + NTP; ++iy) {
+          const int h = dx + NA * (iy + NA * iz);
+          const int a = max(s_ex[h], c0), e = min(s_ex[h + NTP], c1);
+          const int oy = dy + NTP - 1 - iy, oz = dz + NTP - 1 - iz;
+          const float* pb = sWb + (4 * oy + gy) * kBlkPitch - c0;
+          const float* pc = sWc + (4 * oz + gz) * kBlkPitch - c0;
+          for (int g = a + quarter; g < e; g += 4) {
+            const float v = pb[g] * pc[g];
+            const float4 wa = pa[g];
+            acc[dx][0] += v * wa.x, acc[dx][1] += v * wa.y, acc[dx][2] += v * wa.z, acc[dx][3] += v * wa.w;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // the quarters' sums by point (i + 4 j + 16 k) of each tile, then each lane adds its point's four in order
+  const int prow = 4 * gy + 16 * gz;
+#pragma unroll
+  for (int dx = 0; dx < 2; ++dx) {
+    const int t = dx + 2 * dy + 4 * dz;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s_part[t][quarter][prow + i] = acc[dx][i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int dx = 0; dx < 2; ++dx) {
+    const int t = dx + 2 * dy + 4 * dz;
+    const float sum = ((s_part[t][0][lane] + s_part[t][1][lane]) + s_part[t][2][lane]) + s_part[t][3][lane];
+    const int gx = 4 * (tx0 + dx) + (lane & 3), gy_ = 4 * (ty0 + dy) + ((lane >> 2) & 3), gz_ = 4 * (tz0 + dz) + (lane >> 4);
+    grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
+  }
+}
+template <int P>
+__global__ void __launch_bounds__(kBlkThreads) k_spread_blk(GridArgs G, const int64_t* __restrict__ tstart, const float4* __restrict__ wsort,
+                                                            const int4* __restrict__ bsort, float* __restrict__ grid) {
+  const int nblk = (G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2);
+  b_spread_blk<P>(int(xcd_block(blockIdx.x, nblk)), G, tstart, wsort, bsort, grid);
 }
 
 // ---- FFT ------------------------------------------------------------------------------------------------------------
@@ -1048,6 +1209,15 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
   b_gather<P>(tile, G, tstart, tatoms, wsort, bsort, grid, f);
 }
 
+// the spread by blocks: every K a multiple of 8 (whole blocks of 2×2×2 tiles) and more tiles per axis than reach a
+// block's tile (so its home tiles are distinct)
+bool spread_blocks(const GridArgs& G) {
+  const int ntp = (G.p + 6) / 4;
+  for (int d = 0; d < 3; ++d)
+    if (G.k[d] % 8 != 0 || G.nt[d] < ntp + 1) return false;
+  return true;
+}
+
 GridArgs grid_args(const Pme& P) {
   GridArgs G{};
   for (int d = 0; d < 3; ++d) G.k[d] = P.k[d], G.nt[d] = (P.k[d] + 3) / 4;
@@ -1406,9 +1576,14 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   P.counts_clear = true;
   kt_end(c);
   kt_begin(c, "k_spread");
+  const bool blocks = spread_blocks(G);
+  const unsigned nblk = unsigned((G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2));
   switch (G.p) {
-#define CAPSGPU_SPREAD(Q) \
-  case Q: hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), 0, s, G, P.home.p, P.tstart.p, wsort, bsort, P.grid.p); break;
+#define CAPSGPU_SPREAD(Q)                                                                                                          \
+  case Q:                                                                                                                          \
+    if (blocks) hipLaunchKernelGGL(k_spread_blk<Q>, dim3(nblk), dim3(kBlkThreads), 0, s, G, P.tstart.p, wsort, bsort, P.grid.p);  \
+    else hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), 0, s, G, P.home.p, P.tstart.p, wsort, bsort, P.grid.p); \
+    break;
     CAPSGPU_SPREAD(3) CAPSGPU_SPREAD(4) CAPSGPU_SPREAD(5) CAPSGPU_SPREAD(6) CAPSGPU_SPREAD(7) CAPSGPU_SPREAD(8)
 #undef CAPSGPU_SPREAD
   }
@@ -1521,6 +1696,14 @@ __global__ void __launch_bounds__(64) kg_spread(const PmeRep* __restrict__ R, co
 }
 
 template <int P>
+__global__ void __launch_bounds__(kBlkThreads) kg_spread_blk(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int g = int(xcd_block(blockIdx.x, off[nr]));
+  const int r = find_rep(off, nr, g);
+  const PmeRep& a = R[r];
+  b_spread_blk<P>(g - off[r], a.G, a.tstart, a.wsort, a.bsort, a.grid);
+}
+
+template <int P>
 __global__ void __launch_bounds__(64) kg_gather(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
   const int g = int(xcd_block(blockIdx.x, off[nr]));
   const int r = find_rep(off, nr, g);
@@ -1559,8 +1742,8 @@ __global__ void __launch_bounds__(256) kg_fft_z_conv(const PmeRep* __restrict__ 
 struct PmeGroup {
   int nr = 0, order = 0;
   DevBuf<unsigned char> reps;                         // PmeRep[nr]
-  DevBuf<int32_t> off_atoms, off_tiles, off_x, off_yz, off_z, off_xy;   // workgroup prefixes per launch
-  int h_atoms = 0, h_tiles = 0, h_x = 0, h_yz = 0, h_z = 0, h_xy = 0;       // the launches' workgroups
+  DevBuf<int32_t> off_atoms, off_tiles, off_x, off_yz, off_z, off_xy, off_sp, off_blk;   // workgroup prefixes per launch
+  int h_atoms = 0, h_tiles = 0, h_x = 0, h_yz = 0, h_z = 0, h_xy = 0, h_sp = 0, h_blk = 0;  // the launches' workgroups
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_xy = 0;
 };
 
@@ -1569,7 +1752,7 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
   g->nr = int(cs.size());
   if (g->nr < 1) throw Error(-1, "a PME group needs a replica");
   std::vector<PmeRep> R(cs.size());
-  std::vector<int32_t> oa(1, 0), ot(1, 0), ox(1, 0), oy(1, 0), oz(1, 0), oxy(1, 0);
+  std::vector<int32_t> oa(1, 0), ot(1, 0), ox(1, 0), oy(1, 0), oz(1, 0), oxy(1, 0), osp(1, 0), oblk(1, 0);
   for (size_t i = 0; i < cs.size(); ++i) {
     capsgpu_ctx& c = *cs[i];
     Pme& P = c.pme;
@@ -1588,6 +1771,10 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
     a.wsort = reinterpret_cast<float4*>(P.wsort.p), a.bsort = reinterpret_cast<int4*>(P.bsort.p);
     a.home = P.home.p, a.grid = P.grid.p, a.half = P.half.p, a.tw = P.tw.p, a.gtab = P.gtab.p, a.bmod = P.bmod.p;
     a.fac = kPi * kPi / (P.beta * P.beta), a.vol = P.vol, a.fpme = P.fpme.p;
+    // the spread each replica takes alone: by blocks of 2×2×2 tiles or by tiles
+    const bool blocks = spread_blocks(G);
+    osp.push_back(osp.back() + (blocks ? 0 : a.ntiles));
+    oblk.push_back(oblk.back() + (blocks ? (G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2) : 0));
     // a replica on the plane transform (G.lx = 0) has workgroups in the plane launches only, the others in the a- and
     // b-line launches only
     const int npairs = (P.k[1] * P.k[2] + 1) / 2, mblocks = (P.h1 + G.ly - 1) / G.ly;
@@ -1605,7 +1792,8 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
   }
   g->reps.upload(reinterpret_cast<const unsigned char*>(R.data()), R.size() * sizeof(PmeRep), s);
   g->off_atoms.upload(oa, s), g->off_tiles.upload(ot, s), g->off_x.upload(ox, s), g->off_yz.upload(oy, s), g->off_z.upload(oz, s);
-  g->off_xy.upload(oxy, s);
+  g->off_xy.upload(oxy, s), g->off_sp.upload(osp, s), g->off_blk.upload(oblk, s);
+  g->h_sp = osp.back(), g->h_blk = oblk.back();
   if (g->lds_xy > 65536)
     HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_fft_xy), hipFuncAttributeMaxDynamicSharedMemorySize, int(g->lds_xy)));
   g->h_atoms = oa.back(), g->h_tiles = ot.back(), g->h_x = ox.back(), g->h_yz = oy.back(), g->h_z = oz.back(), g->h_xy = oxy.back();
@@ -1629,9 +1817,13 @@ void pme_group_forces(PmeGroup& g, hipStream_t s) {
   hipLaunchKernelGGL(kg_tile_rank, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
   LAUNCHCHECK();
   const size_t lds_gather = (size_t(g.order + 3) * (g.order + 3) * (g.order + 4) + 8 * 6 * size_t(g.order)) * sizeof(float);
+  // the spread: replicas by blocks and the others by tiles (a launch with no workgroups is skipped)
   switch (g.order) {
-#define CAPSGPU_GSPREAD(Q) \
-  case Q: hipLaunchKernelGGL(kg_spread<Q>, dim3(unsigned(nt)), dim3(64), 0, s, R, g.off_tiles.p, nr); break;
+#define CAPSGPU_GSPREAD(Q)                                                                                                     \
+  case Q:                                                                                                                      \
+    if (g.h_blk) hipLaunchKernelGGL(kg_spread_blk<Q>, dim3(unsigned(g.h_blk)), dim3(kBlkThreads), 0, s, R, g.off_blk.p, nr);  \
+    if (g.h_sp) hipLaunchKernelGGL(kg_spread<Q>, dim3(unsigned(g.h_sp)), dim3(64), 0, s, R, g.off_sp.p, nr);                   \
+    break;
     CAPSGPU_GSPREAD(3) CAPSGPU_GSPREAD(4) CAPSGPU_GSPREAD(5) CAPSGPU_GSPREAD(6) CAPSGPU_GSPREAD(7) CAPSGPU_GSPREAD(8)
 #undef CAPSGPU_GSPREAD
   }
