@@ -16,7 +16,7 @@
 //  k_fft_z_conv   lines along c: forward transform, × influence function (energy and virial on energy steps),
 //                 inverse transform — the convolution without a round trip through HBM
 //  k_fft_x_c2r    back to the real grid, two lines per complex transform
-//  k_gather       per atom: F = −q Σ_d K_d h⁻¹_d ∂_d over its p³ points
+//  k_gather       per tile: its mesh box in LDS, p lanes per atom: F = −q Σ_d K_d h⁻¹_d ∂_d over its p³ points
 // The FFT is a mixed-radix (2, 3, 4, 5, 7, 8) Stockham transform in LDS, X(m) = Σ x(k) exp(sign 2πi mk/n),
 // forward sign +1 as the reference; twiddles from a table computed in double on the host.
 #include <algorithm>
@@ -1096,26 +1096,26 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(GridArgs G, FftPlan FP, cons
 }
 
 // one 64-lane workgroup per 4x4x4 tile: the tile's box of (p + 3)³ mesh points in LDS (loaded row by row, wrapped),
-// then its atoms eight at a time, eight lanes per atom, each lane summing a share of the atom's p² rows; the eight
-// lanes' sums are added in a fixed order (butterfly) and the first lane writes the atom's force. Dynamic LDS: the box
-// (p + 3)² rows of p + 4 floats, then per atom slot 6p weights.
+// then its atoms ⌊64/p⌋ at a time, p lanes per atom: lane j of an atom takes its p rows (dy = j, dz = 0 … p − 1), so
+// every lane has whole rows and keeps its b-weights; the p lanes' sums are added in order by the first and it writes
+// the atom's force. Dynamic LDS: the box (p + 3)² rows of p + 4 floats, then per atom slot 6p weights.
 template <int P>
 __device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
                                                const float4* __restrict__ wsort, const int4* __restrict__ bsort,
                                                const float* __restrict__ grid, float* __restrict__ f) {
   extern __shared__ float2 lds[];
+  constexpr int APG = 64 / P;                            // atoms per group
   const int lane = threadIdx.x;
-  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
   const int64_t a0 = tstart[tile], na = tstart[tile + 1] - a0;
   if (na == 0) return;
-  constexpr int p = P, W = P + 3, WX = W + 1;          // rows padded to W + 1 floats: the eight rows an atom's lanes
-                                                         // read fall in different LDS banks
+  constexpr int W = P + 3, WX = W + 1;                   // rows padded to W + 1 floats (fewer bank conflicts)
   float* box = reinterpret_cast<float*>(lds);            // [W][W][WX] (c, b, a)
-  float* s_w = box + W * W * WX;                         // [8][6 P]: θa θb θc dθa dθb dθc
+  float* s_w = box + W * W * WX;                         // [APG][6 P]: θa θb θc dθa dθb dθc
   const int tx = tile % G.nt[0], ty = (tile / G.nt[0]) % G.nt[1], tz = tile / (G.nt[0] * G.nt[1]);
-  const int slot = lane >> 3, sub = lane & 7;
-  // the first eight atoms' data are requested before the box, so the two loads' latencies overlap
-  int64_t at = a0 + (slot < na ? slot : 0);
+  const int slot = lane / P, sub = lane - slot * P;
+  const bool in_group = slot < APG;
+  // the first group's atoms are requested before the box, so the two loads' latencies overlap
+  int64_t at = a0 + (in_group && slot < na ? slot : 0);
   uint32_t i_next = tatoms[at];
   float4 w_next = wsort[at];
   int4 b_next = bsort[at];
@@ -1144,56 +1144,60 @@ __device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __
     const int e = lane + 64 * it;
     if (e < kBox) box[(e / W) * WX + e % W] = bv[it];
   }
-  for (int64_t g0 = 0; g0 < na; g0 += 8) {
-    const bool has = g0 + slot < na;
+  for (int64_t g0 = 0; g0 < na; g0 += APG) {
+    const bool has = in_group && g0 + slot < na;
     const uint32_t i = i_next;
     const float4 w = w_next;
     const int4 b = b_next;
-    if (g0 + 8 < na) {
-      const int64_t ia = g0 + 8 + slot;
-      at = a0 + (ia < na ? ia : 0);
+    if (g0 + APG < na) {
+      const int64_t ia = g0 + APG + slot;
+      at = a0 + (in_group && ia < na ? ia : 0);
       i_next = tatoms[at], w_next = wsort[at], b_next = bsort[at];
     }
-    float* ws = s_w + slot * 6 * P;
-    if (sub < 3) {
-      // lane sub of the atom's eight: the splines of axis sub
+    float* ws = s_w + (in_group ? slot : 0) * 6 * P;
+    if (in_group && sub < 3) {
+      // lane sub of the atom's p: the splines of axis sub
       float th[P], dth[P];
       bspline<P>(sub == 0 ? w.x : sub == 1 ? w.y : w.z, th, dth);
 #pragma unroll
       for (int j = 0; j < P; ++j) ws[sub * P + j] = th[j], ws[(3 + sub) * P + j] = dth[j];
     }
     __syncthreads();
-    // the atom's first point relative to the tile's corner (its home tile: 0 … 3 per axis)
-    const int ox = b.x - 4 * tx, oy = b.y - 4 * ty, oz = b.z - 4 * tz;
-    float ta[P], da[P];
-#pragma unroll
-    for (int j = 0; j < P; ++j) ta[j] = ws[j], da[j] = ws[3 * P + j];
     float g0s = 0.0f, g1s = 0.0f, g2s = 0.0f;
-    if (has)
-      for (int r = sub; r < p * p; r += 8) {
-        const int dy = r % p, dz = r / p;
-        const float* row = box + ((oz + dz) * W + (oy + dy)) * WX + ox;
+    if (has) {
+      // the atom's first point relative to the tile's corner (its home tile: 0 … 3 per axis); this lane's rows dy = sub
+      const int ox = b.x - 4 * tx, oy = b.y - 4 * ty, oz = b.z - 4 * tz;
+      float ta[P], da[P];
+#pragma unroll
+      for (int j = 0; j < P; ++j) ta[j] = ws[j], da[j] = ws[3 * P + j];
+      const float tyv = ws[P + sub], dyv = ws[4 * P + sub];
+      const float* row = box + (oz * W + (oy + sub)) * WX + ox;
+#pragma unroll
+      for (int dz = 0; dz < P; ++dz) {
         float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
         for (int a = 0; a < P; ++a) {
-          const float v = row[a];
+          const float v = row[dz * W * WX + a];
           s0 += da[a] * v;
           s1 += ta[a] * v;
         }
-        const float tyv = ws[P + dy], dyv = ws[4 * P + dy], tzv = ws[2 * P + dz], dzv = ws[5 * P + dz];
+        const float tzv = ws[2 * P + dz], dzv = ws[5 * P + dz];
         g0s += s0 * tyv * tzv;
         g1s += s1 * dyv * tzv;
         g2s += s1 * tyv * dzv;
       }
+    }
+    // the atom's p lanes added in order by its first lane (every lane takes part in the shuffles)
+    const int base = (in_group ? slot : 0) * P;
+    float t0 = g0s, t1 = g1s, t2 = g2s;
 #pragma unroll
-    for (int d = 4; d >= 1; d >>= 1) {
-      g0s += __shfl_xor(g0s, d, 64);
-      g1s += __shfl_xor(g1s, d, 64);
-      g2s += __shfl_xor(g2s, d, 64);
+    for (int k = 1; k < P; ++k) {
+      const float u0 = __shfl(g0s, base + k, 64), u1 = __shfl(g1s, base + k, 64), u2 = __shfl(g2s, base + k, 64);
+      t0 += u0, t1 += u1, t2 += u2;
     }
     if (has && sub == 0)
       for (int m = 0; m < 3; ++m) {
-        const double grad = G.k[0] * double(g0s) * G.hi[0][m] + G.k[1] * double(g1s) * G.hi[1][m] + G.k[2] * double(g2s) * G.hi[2][m];
+        const double grad = G.k[0] * double(t0) * G.hi[0][m] + G.k[1] * double(t1) * G.hi[1][m] + G.k[2] * double(t2) * G.hi[2][m];
         f[3 * size_t(i) + m] = float(-double(w.w) * grad);
       }
     __syncthreads();
@@ -1592,7 +1596,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   fft_forward_conv_inverse(c, G, F, want);
   P.fpme.resize(3 * size_t(n));
   kt_begin(c, "k_gather");
-  const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 4) + 8 * 6 * size_t(G.p)) * sizeof(float);
+  const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 4) + 64 * 6) * sizeof(float);   // box + ⌊64/p⌋ slots of 6p
   switch (G.p) {
 #define CAPSGPU_GATHER(Q) \
   case Q: hipLaunchKernelGGL(k_gather<Q>, dim3(unsigned(ntiles)), dim3(64), lds_gather, s, G, P.tstart.p, P.tatoms.p, wsort, bsort, P.grid.p, P.fpme.p); break;
@@ -1816,7 +1820,7 @@ void pme_group_forces(PmeGroup& g, hipStream_t s) {
   LAUNCHCHECK();
   hipLaunchKernelGGL(kg_tile_rank, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
   LAUNCHCHECK();
-  const size_t lds_gather = (size_t(g.order + 3) * (g.order + 3) * (g.order + 4) + 8 * 6 * size_t(g.order)) * sizeof(float);
+  const size_t lds_gather = (size_t(g.order + 3) * (g.order + 3) * (g.order + 4) + 64 * 6) * sizeof(float);
   // the spread: replicas by blocks and the others by tiles (a launch with no workgroups is skipped)
   switch (g.order) {
 #define CAPSGPU_GSPREAD(Q)                                                                                                     \
