@@ -501,14 +501,49 @@ __device__ inline void dft4(float2& v0, float2& v1, float2& v2, float2& v3, int 
   const float2 a = cadd(v0, v2), b = csub(v0, v2), c = cadd(v1, v3), d = mul_si(csub(v1, v3), sign);
   v0 = cadd(a, c), v2 = csub(a, c), v1 = cadd(b, d), v3 = csub(b, d);
 }
+// × exp(sign 2πi m/16) for a compile-time m: cos and sin of π/8 steps as literals, the quarter turns exact
+template <int M>
+__device__ inline float2 tw16(float2 a, int sign) {
+  constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, h = 0.70710678118654752f;
+  constexpr int m = M % 16;
+  if constexpr (m == 0) {
+    return a;
+  } else if constexpr (m == 4) {
+    return mul_si(a, sign);
+  } else {
+    constexpr float C[16] = {1.0f, c1, h, s1, 0.0f, -s1, -h, -c1, -1.0f, -c1, -h, -s1, 0.0f, s1, h, c1};
+    constexpr float S[16] = {0.0f, s1, h, c1, 1.0f, c1, h, s1, 0.0f, -s1, -h, -c1, -1.0f, -c1, -h, -s1};
+    const float c = C[m], sn = sign * S[m];
+    return make_float2(a.x * c - a.y * sn, a.x * sn + a.y * c);
+  }
+}
+
 template <int R>
 __device__ inline void dft(float2* v, int sign) {
-  if (R == 2) {
+  if constexpr (R == 2) {
     const float2 a = v[0], b = v[1];
     v[0] = cadd(a, b), v[1] = csub(a, b);
-  } else if (R == 4) {
+  } else if constexpr (R == 4) {
     dft4(v[0], v[1], v[2], v[3], sign);
-  } else if (R == 8) {
+  } else if constexpr (R == 16) {
+    // n = n1 + 4 n2, k = k2 + 4 k1: four 4-point transforms over n2, × exp(sign 2πi n1 k2/16), four over n1
+    float2 y[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+      float2 a0 = v[n1], a1 = v[n1 + 4], a2 = v[n1 + 8], a3 = v[n1 + 12];
+      dft4(a0, a1, a2, a3, sign);
+      y[4 * n1] = a0, y[4 * n1 + 1] = a1, y[4 * n1 + 2] = a2, y[4 * n1 + 3] = a3;   // y[4 n1 + k2]
+    }
+    y[5] = tw16<1>(y[5], sign), y[6] = tw16<2>(y[6], sign), y[7] = tw16<3>(y[7], sign);
+    y[9] = tw16<2>(y[9], sign), y[10] = tw16<4>(y[10], sign), y[11] = tw16<6>(y[11], sign);
+    y[13] = tw16<3>(y[13], sign), y[14] = tw16<6>(y[14], sign), y[15] = tw16<9>(y[15], sign);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      float2 b0 = y[k2], b1 = y[4 + k2], b2 = y[8 + k2], b3 = y[12 + k2];
+      dft4(b0, b1, b2, b3, sign);
+      v[k2] = b0, v[k2 + 4] = b1, v[k2 + 8] = b2, v[k2 + 12] = b3;
+    }
+  } else if constexpr (R == 8) {
     float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     dft4(e0, e1, e2, e3, sign);
     dft4(o0, o1, o2, o3, sign);
@@ -522,6 +557,7 @@ __device__ inline void dft(float2* v, int sign) {
     v[2] = cadd(e2, t2), v[6] = csub(e2, t2);
     v[3] = cadd(e3, t3), v[7] = csub(e3, t3);
   } else {
+    static_assert(R == 3 || R == 5 || R == 7, "radix");
     // R = 3, 5, 7: cos and sin of 2πm/R
     constexpr float C3[3] = {1.0f, -0.5f, -0.5f}, S3[3] = {0.0f, 0.86602540378443865f, -0.86602540378443865f};
     constexpr float C5[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f, -0.80901699437494742f, 0.30901699437494742f};
@@ -589,8 +625,9 @@ __device__ inline void fft_stage(const float2* src, float2* dst, int nl, int n, 
   }
 }
 
-// power-of-two lengths with every size a compile-time constant (the index arithmetic shifts and masks); the stages are
-// the plan's (radix 8 while it divides, then 4, then 2), so a line's result is the run-time stages' bit for bit
+// power-of-two lengths with every size a compile-time constant (the index arithmetic shifts and masks) and radix 16
+// while it divides (then 8, 4, 2): 128 points in two stages. Every kernel transforms these lengths here, so a line's
+// result is the same wherever it is transformed.
 template <int R, int N, int P, int Sign>
 __device__ inline void fft_stage_n(const float2* src, float2* dst, int nl, const float2* twl) {
   constexpr int nb = N / R, step = N / (P * R), LS = N + (N >> 4) + 1;
@@ -604,7 +641,7 @@ __device__ inline float2* stages_n(float2* src, float2* dst, int nl, const float
   if constexpr (P >= N) {
     return src;
   } else {
-    constexpr int M = N / P, R = M % 8 == 0 ? 8 : M % 4 == 0 ? 4 : 2;
+    constexpr int M = N / P, R = M % 16 == 0 ? 16 : M % 8 == 0 ? 8 : M % 4 == 0 ? 4 : 2;
     fft_stage_n<R, N, P, Sign>(src, dst, nl, twl);
     __syncthreads();
     return stages_n<N, Sign, P * R>(dst, src, nl, twl);
