@@ -1514,10 +1514,10 @@ void fft_forward_conv_inverse(capsgpu_ctx& c, const GridArgs& G, const FftPlan& 
 }
 }  // namespace
 
-// the transforms' tile shape of this context, picked by timing on its own mesh: the passes along b and c (with the
-// convolution) with 4, 8 or 16 lines per workgroup, then the passes along a and b — the plane kernel when the planes fit
-// (lines_x 0), or the a-lines with 2, 4, 8 or 16 line pairs per workgroup and the b-lines at the chosen width — each the
-// best of three runs timed with HIP events on the context's stream. The passes run in place on the
+// the transforms' tile shape of this context, picked by timing on its own mesh: every pair of the passes along a and b
+// (the plane kernel when the planes fit, lines_x 0, or the a-lines with 2, 4, 8 or 16 line pairs per workgroup) and the
+// lines along b and c (4, 8 or 16 per workgroup), the whole transform with the convolution, each the best of three runs
+// timed with HIP events on the context's stream. The passes run in place on the
 // mesh of the last evaluation, which the next evaluation overwrites; every shape transforms each line with the same
 // arithmetic, so the choice changes the time and not the forces.
 void pme_tune(capsgpu_ctx& c, int out[2]) {
@@ -1543,26 +1543,20 @@ void pme_tune(capsgpu_ctx& c, int out[2]) {
     return best;
   };
   try {
+    // every pair of shapes over the whole transform (the b pass belongs to the plane kernel or to the line tiles, so the
+    // two choices are not independent)
     float tbest = 1e30f;
-    int pick = P.lines_yz;
-    P.lines_x = 8;
-    for (int ly : {4, 8, 16}) {
-      P.lines_yz = ly;
-      const GridArgs G = grid_args(P);
-      const float t = best_of([&] { launch_fft_y(c, G, F, +1), launch_fft_z_conv(c, G, F, false), launch_fft_y(c, G, F, -1); });
-      if (t < tbest) tbest = t, pick = ly;
-    }
-    P.lines_yz = pick;
-    tbest = 1e30f;
-    pick = 8;
+    int pick_x = 8, pick_yz = P.lines_yz;
     for (int lx : {0, 2, 4, 8, 16}) {
       if (lx == 0 && !P.plane_ok) continue;
-      P.lines_x = lx;
-      const GridArgs G = grid_args(P);
-      const float t = best_of([&] { fft_ab(c, G, F, true), fft_ab(c, G, F, false); });
-      if (t < tbest) tbest = t, pick = lx;
+      for (int ly : {4, 8, 16}) {
+        P.lines_x = lx, P.lines_yz = ly;
+        const GridArgs G = grid_args(P);
+        const float t = best_of([&] { fft_forward_conv_inverse(c, G, F, false); });
+        if (t < tbest) tbest = t, pick_x = lx, pick_yz = ly;
+      }
     }
-    P.lines_x = pick;
+    P.lines_x = pick_x, P.lines_yz = pick_yz;
   } catch (...) {
     c.profile = prof;
     (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
