@@ -309,39 +309,83 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
 // ---- the spread by blocks of 2×2×2 tiles (every K a multiple of 8) ---------------------------------------------------
 // k_spread stages each atom once per tile it reaches (eight tiles at order 5), its splines recomputed each time; here a
 // workgroup of four waves takes a block of eight tiles and stages each atom of the home tiles that reach the block
-// once: its splines, and per axis the NTP windows of four weights it has on the tiles past its home tile (θa, θb,
-// q·θc), kept in LDS by element so every read of the accumulation meets distinct banks or one broadcast. Each wave then
-// sums two tiles, lane = (row (j, k) of the tile, quarter of the candidates): per candidate one row weight θb·qθc and
-// one float4 of θa from LDS, four FMAs. A tile's candidates are walked home tile by home tile (each with fixed window
-// offsets), in (c, b, a) order; the quarters are added in a fixed order and every mesh point is written once.
-constexpr int kBlkThreads = 256, kBlkStage = 256, kBlkPitch = kBlkStage + 4;   // atoms staged per chunk: at order 5 the
+// once: its splines (the B-spline pieces as polynomials in w, by Horner), and per axis the NTP windows of four weights
+// it has on the tiles past its home tile (θa, θb, q·θc). A window is cut out by where the weights are written, not by
+// selects: each staging thread owns one column of the LDS rows (row e of column f holds element e of staged atom f),
+// writes its P weights into zeroed rows s … s + P − 1 (s its first point's place in its tile) and, for θa, reads its two
+// float4 windows back. Every read of the accumulation meets distinct banks or one broadcast. Each wave then sums two
+// tiles, lane = (rows (j, k) and (j + 2, k) of the tile, eighth of the candidates): per candidate two row weights θb·qθc
+// and one float4 of θa from LDS, eight FMAs, in wave-uniform rounds of eight candidates over each run of home tiles. A
+// tile's candidates are walked home tile by home tile (each with fixed window offsets), in (c, b, a) order; the eighths
+// are added in a fixed order and every mesh point is written once.
+constexpr int kBlkThreads = 256, kBlkStage = 256, kBlkPitch = kBlkStage + 8;   // atoms staged per chunk: at order 5 the
                                                                                  // block's 27 home tiles hold ~216
 
-// θ[idx − s] for a compile-time idx (the window element) and a first-point offset s ∈ [0, 4), zero outside [0, P)
+// The weights θ[j](w) of order P as polynomials, θ[j] = Σ_m c[j][m] w^m: bspline()'s recursion carried out on the
+// coefficients in double, then rounded to float.
 template <int P>
-__device__ __forceinline__ float shifted(const float* th, int idx, int s) {
-  float v = 0.0f;
-#pragma unroll
-  for (int ss = 0; ss < 4; ++ss) {
-    const int m = idx - ss;
-    if (m >= 0 && m < P) v = s == ss ? th[m] : v;
+struct BsplinePoly {
+  float c[P][P];
+};
+template <int P>
+constexpr BsplinePoly<P> bspline_poly() {
+  double t[P][P] = {};
+  t[0][0] = 1.0, t[0][1] = -1.0, t[1][1] = 1.0;   // θ[0] = 1 − w, θ[1] = w
+  for (int k = 3; k <= P; ++k) {
+    const double div = 1.0 / double(k - 1);
+    for (int m = P - 1; m >= 1; --m) t[k - 1][m] = div * t[k - 2][m - 1];   // θ[k−1] = w θ[k−2] / (k−1)
+    t[k - 1][0] = 0.0;
+    for (int j = 1; j <= k - 2; ++j) {   // θ[k−j−1] = ((w + j) θ[k−j−2] + (k − j − w) θ[k−j−1]) / (k−1)
+      double r[P] = {};
+      for (int m = 0; m < P; ++m) {
+        r[m] += double(j) * t[k - j - 2][m] + double(k - j) * t[k - j - 1][m];
+        if (m + 1 < P) r[m + 1] += t[k - j - 2][m] - t[k - j - 1][m];
+      }
+      for (int m = 0; m < P; ++m) t[k - j - 1][m] = div * r[m];
+    }
+    double r[P] = {};   // θ[0] = (1 − w) θ[0] / (k−1)
+    for (int m = 0; m < P; ++m) {
+      r[m] += t[0][m];
+      if (m + 1 < P) r[m + 1] -= t[0][m];
+    }
+    for (int m = 0; m < P; ++m) t[0][m] = div * r[m];
   }
-  return v;
+  BsplinePoly<P> out{};
+  for (int j = 0; j < P; ++j)
+    for (int m = 0; m < P; ++m) out.c[j][m] = float(t[j][m]);
+  return out;
+}
+
+// θ[j](w), j < P, by Horner in FP32: P − 1 FMAs each, within a few 1e-8 of the recursion's values for w in [0, 1)
+template <int P>
+__device__ __forceinline__ void bspline_horner(float w, float* th) {
+  constexpr BsplinePoly<P> C = bspline_poly<P>();
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    float p = C.c[j][P - 1];
+#pragma unroll
+    for (int m = P - 2; m >= 0; --m) p = __builtin_fmaf(p, w, C.c[j][m]);
+    th[j] = p;
+  }
 }
 
 template <int P>
 __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const int64_t* __restrict__ tstart, const float4* __restrict__ wsort,
                                              const int4* __restrict__ bsort, float* __restrict__ grid) {
   constexpr int NTP = (P + 6) / 4, NA = NTP + 1, NH = NA * NA * NA;   // home tiles per axis that reach the block, in all
-  static_assert(NH <= 64, "a lane per home tile");
-  __shared__ float4 sWa[2 * kBlkStage];          // θa of staged atom f on the block's tile column dx at dx·stage + f
-  __shared__ float sWbc[2 * 4 * NTP * kBlkPitch];   // element e = 4o + j of the θb windows at e·pitch + f, then q·θc's
-  __shared__ int s_ex[NH + 1], s_hb[NH];         // home tile h's first staged place and first atom
-  float* sWb = sWbc;
-  float* sWc = sWbc + 4 * NTP * kBlkPitch;
-  static_assert(sizeof(sWbc) >= 8 * 4 * 64 * sizeof(float), "the tiles' partial sums reuse the windows' space");
-  float(*s_part)[4][64] = reinterpret_cast<float(*)[4][64]>(sWbc);   // per tile: the quarters' sums by point (after the chunks)
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  constexpr int NR = 8 * NTP;                                          // LDS rows of a staged atom: θb's, then q·θc's
+  static_assert(NH <= 64 && NH < 256, "a lane per home tile, its index in a byte");
+  static_assert(4 * NTP + 8 <= NR && P + 7 <= NR && P + 2 < 4 * NTP, "θa's rows and the zero windows either side fit the rows");
+  static_assert(kBlkThreads == kBlkStage, "one staged atom per thread");
+  __shared__ float4 sWa[2 * kBlkStage];   // θa of staged atom f on the block's tile column dx at dx·stage + f
+  __shared__ float sW[NR * kBlkPitch];    // element e = 4o + j of the θb windows at row e (e·pitch + f), q·θc's at 4 NTP + e
+  __shared__ int s_ex[NH + 1], s_hb[NH];  // home tile h's first staged place and first atom
+  __shared__ uint8_t s_home[kBlkStage];   // the home tile of each staged place of the first chunk
+  float* sWb = sW;
+  float* sWc = sW + 4 * NTP * kBlkPitch;
+  static_assert(sizeof(sW) >= 8 * 8 * 64 * sizeof(float), "the tiles' partial sums reuse the rows' space");
+  float(*s_part)[8][64] = reinterpret_cast<float(*)[8][64]>(sW);   // per tile: the eighths' sums by point (after the chunks)
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int nbx = G.nt[0] / 2, nby = G.nt[1] / 2;
   const int bx = blk % nbx, by = (blk / nbx) % nby, bz = blk / (nbx * nby);
   const int tx0 = 2 * bx, ty0 = 2 * by, tz0 = 2 * bz;
@@ -364,56 +408,62 @@ __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const i
       const int o = __shfl_up(cnt, d, 64);
       if (lane >= d) cnt += o;
     }
-    if (lane < NH) s_ex[lane] = cnt - own, s_hb[lane] = hb;
+    if (lane < NH) {
+      s_ex[lane] = cnt - own, s_hb[lane] = hb;
+      const int q1 = cnt < kBlkStage ? cnt : kBlkStage;
+      for (int q = cnt - own; q < q1; ++q) s_home[q] = uint8_t(lane);
+    }
     if (lane == NH - 1) s_ex[NH] = cnt;
   }
   __syncthreads();
-  const int m = s_ex[NH];
-  // this wave's two tiles: dx = 0, 1 at (dy, dz) = (wave & 1, wave >> 1); lane = (row j + 4k, quarter)
+  const int m = __builtin_amdgcn_readfirstlane(s_ex[NH]);
+  // this wave's two tiles: dx = 0, 1 at (dy, dz) = (wave & 1, wave >> 1); lane = (rows j, j + 2 at k, eighth), j < 2.
+  // With the rows' pitch ≡ 8 (mod 32 banks), the eight eighths' reads of rows j, j + 2 or k meet distinct banks.
   const int dy = wave & 1, dz = wave >> 1;
-  const int gy = lane & 3, gz = (lane >> 2) & 3, quarter = lane >> 4;
-  float acc[2][4] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+  const int gy = lane & 1, gz = (lane >> 1) & 3, eighth = lane >> 3;
+  float acc[2][2][4] = {};   // [dx][row j, j + 2][i]
   for (int c0 = 0; c0 < m; c0 += kBlkStage) {
     const int f = c0 + tid;
-    if (tid < kBlkStage && f < m) {
-      // the home tile of staged place f: the last h with s_ex[h] <= f (empty tiles are passed over)
-      int lo = 0, hi = NH - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_ex[mid] <= f) lo = mid;
-        else hi = mid - 1;
+    if (f < m) {
+      int h;
+      if (c0 == 0) {
+        h = s_home[tid];
+      } else {   // the last h with s_ex[h] <= f (empty tiles are passed over)
+        int lo = 0, hi = NH - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_ex[mid] <= f) lo = mid;
+          else hi = mid - 1;
+        }
+        h = lo;
       }
-      const int at = s_hb[lo] + (f - s_ex[lo]), ix = lo % NA;
+      const int at = s_hb[h] + (f - s_ex[h]), ix = h % NA;
       const float4 w = wsort[at];
       const int4 b = bsort[at];
-      float ta[P], tb[P], tc[P], dd[P];
-      bspline<P>(w.x, ta, dd);
-      bspline<P>(w.y, tb, dd);
-      bspline<P>(w.z, tc, dd);
-#pragma unroll
-      for (int j = 0; j < P; ++j) tc[j] = w.w * tc[j];
+      float ta[P], tb[P], tc[P];
+      bspline_horner<P>(w.x, ta);
+      bspline_horner<P>(w.y, tb);
+      bspline_horner<P>(w.z, tc);
       const int sx = b.x & 3, sy = b.y & 3, sz = b.z & 3;
-      // θa on the tile columns dx = 0, 1 of the block: the window (dx + NTP − 1 − ix) past the home tile, or zero
-      // (two named values, not an array: a select into an array indexed by ix would go through scratch)
-      float4 w0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w1 = w0;
-      const int o0 = NTP - 1 - ix, o1 = NTP - ix;
+      float* col = sW + tid;
 #pragma unroll
-      for (int o = 0; o < NTP; ++o) {
-        const float e0 = shifted<P>(ta, 4 * o, sx), e1 = shifted<P>(ta, 4 * o + 1, sx), e2 = shifted<P>(ta, 4 * o + 2, sx),
-                    e3 = shifted<P>(ta, 4 * o + 3, sx);
-        const bool h0 = o0 == o, h1 = o1 == o;
-        w0.x = h0 ? e0 : w0.x, w0.y = h0 ? e1 : w0.y, w0.z = h0 ? e2 : w0.z, w0.w = h0 ? e3 : w0.w;
-        w1.x = h1 ? e0 : w1.x, w1.y = h1 ? e1 : w1.y, w1.z = h1 ? e2 : w1.z, w1.w = h1 ? e3 : w1.w;
-      }
-      sWa[tid] = w0, sWa[kBlkStage + tid] = w1;
+      for (int e = 0; e < NR; ++e) col[e * kBlkPitch] = 0.0f;
+      // θa at rows 4 + sx + j: window o (elements 4o … 4o + 3) at rows 4 + 4o …, zero windows at o = −1 and o = NTP.
+      // The block's tile column dx takes window dx + NTP − 1 − ix, at rows 4 (dx + NTP − ix) … .
+      float* ca = col + (4 + sx) * kBlkPitch;
 #pragma unroll
-      for (int o = 0; o < NTP; ++o) {
+      for (int j = 0; j < P; ++j) ca[j * kBlkPitch] = ta[j];
+      const float* r0 = col + 4 * (NTP - ix) * kBlkPitch;
+      const float* r1 = r0 + 4 * kBlkPitch;
+      sWa[tid] = make_float4(r0[0], r0[kBlkPitch], r0[2 * kBlkPitch], r0[3 * kBlkPitch]);
+      sWa[kBlkStage + tid] = make_float4(r1[0], r1[kBlkPitch], r1[2 * kBlkPitch], r1[3 * kBlkPitch]);
+      // θa's rows zeroed again, then θb at rows sy + j and q·θc at rows 4 NTP + sz + j
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          sWb[(4 * o + u) * kBlkPitch + tid] = shifted<P>(tb, 4 * o + u, sy);
-          sWc[(4 * o + u) * kBlkPitch + tid] = shifted<P>(tc, 4 * o + u, sz);
-        }
-      }
+      for (int e = 4; e < P + 7; ++e) col[e * kBlkPitch] = 0.0f;
+      float* cb = col + sy * kBlkPitch;
+      float* cc = col + (4 * NTP + sz) * kBlkPitch;
+#pragma unroll
+      for (int j = 0; j < P; ++j) cb[j * kBlkPitch] = tb[j], cc[j * kBlkPitch] = w.w * tc[j];
     }
     __syncthreads();
     const int c1 = min(m, c0 + kBlkStage);
@@ -421,39 +471,55 @@ __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const i
     // order, each atom's θa on this column staged for it
 #pragma unroll
     for (int dx = 0; dx < 2; ++dx) {
-      const float4* pa = sWa + dx * kBlkStage - c0;
 #pragma unroll
       for (int iz = dz; iz < dz + NTP; ++iz) {
 #pragma unroll
         for (int iy = dy; iy < dy + NTP; ++iy) {
           const int h = dx + NA * (iy + NA * iz);
-          const int a = max(s_ex[h], c0), e = min(s_ex[h + NTP], c1);
+          const int a = __builtin_amdgcn_readfirstlane(max(s_ex[h], c0));
+          const int n = __builtin_amdgcn_readfirstlane(min(s_ex[h + NTP], c1)) - a;
+          if (n <= 0) continue;
           const int oy = dy + NTP - 1 - iy, oz = dz + NTP - 1 - iz;
-          const float* pb = sWb + (4 * oy + gy) * kBlkPitch - c0;
-          const float* pc = sWc + (4 * oz + gz) * kBlkPitch - c0;
-          for (int g = a + quarter; g < e; g += 4) {
-            const float v = pb[g] * pc[g];
+          const float* pb = sWb + (4 * oy + gy) * kBlkPitch + (a - c0) + eighth;
+          const float* pc = sWc + (4 * oz + gz) * kBlkPitch + (a - c0) + eighth;
+          const float4* pa = sWa + dx * kBlkStage + (a - c0) + eighth;
+          auto add = [&](int g, float z) {
+            const float c = pc[g] * z, v0 = pb[g] * c, v1 = pb[g + 2 * kBlkPitch] * c;
             const float4 wa = pa[g];
-            acc[dx][0] += v * wa.x, acc[dx][1] += v * wa.y, acc[dx][2] += v * wa.z, acc[dx][3] += v * wa.w;
-          }
+            acc[dx][0][0] += v0 * wa.x, acc[dx][0][1] += v0 * wa.y, acc[dx][0][2] += v0 * wa.z, acc[dx][0][3] += v0 * wa.w;
+            acc[dx][1][0] += v1 * wa.x, acc[dx][1][1] += v1 * wa.y, acc[dx][1][2] += v1 * wa.z, acc[dx][1][3] += v1 * wa.w;
+          };
+          // rounds of eight candidates, one per eighth: every lane has one in each round but the last
+          const int T = (n + 7) >> 3;
+          int k = 0;
+          for (; k + 2 < T; k += 2) add(8 * k, 1.0f), add(8 * k + 8, 1.0f);
+          if (k + 1 < T) add(8 * k, 1.0f), ++k;
+          // the last round: a lane past the run takes the run's last candidate with a zero weight (finite products of
+          // zero, so its sums are unchanged)
+          const bool in = 8 * k + eighth < n;
+          add(in ? 8 * k : n - 1 - eighth, in ? 1.0f : 0.0f);
         }
       }
     }
     __syncthreads();
   }
-  // the quarters' sums by point (i + 4 j + 16 k) of each tile, then each lane adds its point's four in order
+  // the eighths' sums by point (i + 4 j + 16 k) of each tile, then each lane adds its point's eight in order
   const int prow = 4 * gy + 16 * gz;
 #pragma unroll
   for (int dx = 0; dx < 2; ++dx) {
     const int t = dx + 2 * dy + 4 * dz;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) s_part[t][quarter][prow + i] = acc[dx][i];
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) s_part[t][eighth][prow + 8 * r + i] = acc[dx][r][i];
   }
   __syncthreads();
 #pragma unroll
   for (int dx = 0; dx < 2; ++dx) {
     const int t = dx + 2 * dy + 4 * dz;
-    const float sum = ((s_part[t][0][lane] + s_part[t][1][lane]) + s_part[t][2][lane]) + s_part[t][3][lane];
+    float sum = s_part[t][0][lane];
+#pragma unroll
+    for (int e = 1; e < 8; ++e) sum += s_part[t][e][lane];
     const int gx = 4 * (tx0 + dx) + (lane & 3), gy_ = 4 * (ty0 + dy) + ((lane >> 2) & 3), gz_ = 4 * (tz0 + dz) + (lane >> 4);
     grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
   }
