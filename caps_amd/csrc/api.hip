@@ -236,9 +236,8 @@ void evaluate(capsgpu_ctx& c, bool want, double* t, bool flags_known, int parts,
   ev_record(c, 3);
   if (recip) pme_forces(c, want);
   ev_record(c, 4);
-  if (bonded) bonded_forces(c, want);
+  bonded_and_sum(c, want, nonb, recip, bonded, fdst);
   ev_record(c, 5);
-  sum_forces(c, nonb, recip, bonded, fdst);
   if (want) {
     Sums S{};
     double* out = c.terms.p;

@@ -115,8 +115,13 @@ struct Bonded {
   DevBuf<int32_t> aslot;                // a member's slot: bond t 2t + k, angle t 2nb + 3t + k, FENE t 2nb + 3na + 2t + k
   DevBuf<int32_t> spos;                 // per member slot: its place in the per-atom order (the CSR above)
   DevBuf<double> slot;                  // 3 per place: the term's force on that member, by atom
-  DevBuf<double> epart;                 // per block of terms: bond, angle, virial, w[6]
+  DevBuf<double> epart;                 // per block of terms (or per range): bond, angle, virial, w[6]
   int eblocks = 0;                      // rows of epart
+  // ranges of consecutive atoms whose slots fit a workgroup's LDS (the bonded terms and force sum in one launch)
+  bool ranged = false;
+  int nranges = 0;
+  DevBuf<int32_t> rstart, roff;         // range r: atoms rstart[r] … rstart[r+1] − 1; its terms rterm[roff[r]] …
+  DevBuf<uint32_t> rterm;               // term (bonds, angles, FENE bonds in one numbering); top bit: its energy counted here
 };
 
 
@@ -203,6 +208,8 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff);
 void bonded_forces(capsgpu_ctx& c, bool want);
 // integrate.hip
 void sum_forces(capsgpu_ctx& c, bool nb, bool rec, bool bond, double* fdst);
+// the bonded terms (when bond) and the force sum into fdst: one launch by atom ranges, or k_bonded + k_sum_forces
+void bonded_and_sum(capsgpu_ctx& c, bool want, bool nb, bool rec, bool bond, double* fdst);
 void check_moves(capsgpu_ctx& c, double lim_build, double lim_prune);
 // partial sums: slot k of c.partials; reduce_terms folds them (fixed order) into c.terms
 enum Part { P_VDW = 0, P_COUL, P_W0, P_BOND = 8, P_ANGLE, P_BVIR, P_BW0, P_RECE = 16, P_RW0, NPART = 24 };

@@ -5,6 +5,7 @@
 //               (:2150-2180).
 //  k_sum_forces per atom: the pair, reciprocal and bonded forces; its bonded slots (contiguous, in the order of its
 //               terms) added first (no atomics, a fixed order)
+//  k_bonded_sum the two above in one launch by ranges of consecutive atoms, the slots kept in LDS (the same sums)
 //  k_sum_forces the total force per atom: pair forces (by slot, float) + reciprocal (float) + bonded (double)
 //  k_vv_open    velocity Verlet's opening half kick and drift (core/src/dynamics.cpp:405-431, 657-666), Bussi's
 //               scaling of the previous step folded in
@@ -26,8 +27,10 @@ struct CellArgs {
   int ortho;
 };
 
+// kOrtho: 1 the cell is orthorhombic, 0 it is not, −1 C.ortho says (a kernel for one kind of cell keeps fewer registers)
+template <int kOrtho = -1>
 __device__ inline void min_image(const CellArgs& C, double* d) {
-  if (C.ortho) {
+  if (kOrtho == 1 || (kOrtho == -1 && C.ortho)) {
     const double L[3] = {C.a[0], C.b[1], C.c[2]};
     for (int k = 0; k < 3; ++k) d[k] -= L[k] * round(d[k] / L[k]);
     return;
@@ -58,21 +61,18 @@ __device__ inline void vir_add(double* A, const double* d, const double* f) {
 
 constexpr int kEB = 9;   // bond, angle, virial, w[6]
 
-// one thread per term (bonds, then angles, then FENE bonds): the term's force on each member atom into that member's
-// slot (bond t: members 2t, 2t + 1; angle t: 2nb + 3t + 0…2; FENE t: 2nb + 3na + 2t + 0…1), stored at the member's place
-// spos[member] in the per-atom order of slots, energies and virial into per-block sums. Every term is computed once;
-// k_sum_forces adds each atom's slots, which lie together, in a fixed order.
-template <bool kE>
-__device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf, CellArgs C, const double* __restrict__ x,
-                                                const int32_t* __restrict__ bij, const double* __restrict__ bkr,
-                                                const int32_t* __restrict__ aijk, const double* __restrict__ akt,
-                                                const int32_t* __restrict__ fij, const double* __restrict__ fp, const int32_t* __restrict__ spos,
-                                                double* __restrict__ slot, double* __restrict__ epart, uint32_t* __restrict__ flags) {
-  // no contraction of a·b + c into FMAs here: the energy variant's extra arithmetic must not change how the forces are
-  // rounded (forces are the same whether or not energies are asked for)
+// The forces of term g (bonds [0, nb), then angles, then FENE bonds) on its members: put(place, fx, fy, fz) for each
+// member, place being the member's place in the per-atom order of slots (bond t: member slots 2t, 2t + 1; angle t:
+// 2nb + 3t + 0…2; FENE t: 2nb + 3na + 2t + 0…1); with kE and own, the term's energy and virial into A. No contraction
+// of a·b + c into FMAs here: the energy variant's extra arithmetic must not change how the forces are rounded (forces
+// are the same whether or not energies are asked for, and wherever the term is evaluated).
+template <bool kE, int kOrtho, class Put>
+__device__ __forceinline__ void term_forces(int g, bool own, int nb, int na, int nf, const CellArgs& C, const double* __restrict__ x,
+                                            const int32_t* __restrict__ bij, const double* __restrict__ bkr,
+                                            const int32_t* __restrict__ aijk, const double* __restrict__ akt,
+                                            const int32_t* __restrict__ fij, const double* __restrict__ fp,
+                                            const int32_t* __restrict__ spos, Put&& put, double* A, uint32_t* __restrict__ flags) {
 #pragma clang fp contract(off)
-  const int g = bx * blockDim.x + threadIdx.x;
-  double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (g < nb || (g >= nb + na && g < nb + na + nf)) {
     const bool fene = g >= nb;
     const int t = fene ? g - nb - na : g;
@@ -81,7 +81,7 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
     const size_t s0 = fene ? 2 * size_t(nb) + 3 * size_t(na) + 2 * size_t(t) : 2 * size_t(t);
     const int32_t p0 = spos[s0], p1 = spos[s0 + 1];
     double d[3] = {x[3 * b] - x[3 * a], x[3 * b + 1] - x[3 * a + 1], x[3 * b + 2] - x[3 * a + 2]};
-    min_image(C, d);
+    min_image<kOrtho>(C, d);
     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     double en = 0, dEdr = 0;
     bool ok = true;
@@ -107,22 +107,23 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
     double fj[3] = {0, 0, 0};
     if (ok && r > 1e-12)
       for (int m = 0; m < 3; ++m) fj[m] = d[m] * (-dEdr / r);
-    double* o0 = slot + 3 * size_t(p0);
-    double* o1 = slot + 3 * size_t(p1);
-    for (int m = 0; m < 3; ++m) o0[m] = -fj[m], o1[m] = fj[m];
-    if (kE && ok) {
+    put(p0, -fj[0], -fj[1], -fj[2]);
+    put(p1, fj[0], fj[1], fj[2]);
+    if (kE && own && ok) {
       A[0] += en;
       vir_add(A, d, fj);
     }
-  } else if (g >= nb && g < nb + na) {
+    return;
+  }
+  if (g >= nb && g < nb + na) {
     const int t = g - nb;
     const int32_t ia = aijk[3 * t], ja = aijk[3 * t + 1], ka = aijk[3 * t + 2];
     const size_t s0 = 2 * size_t(nb) + 3 * size_t(t);
     const int32_t p0 = spos[s0], p1 = spos[s0 + 1], p2 = spos[s0 + 2];
     double u[3] = {x[3 * ia] - x[3 * ja], x[3 * ia + 1] - x[3 * ja + 1], x[3 * ia + 2] - x[3 * ja + 2]};
     double v[3] = {x[3 * ka] - x[3 * ja], x[3 * ka + 1] - x[3 * ja + 1], x[3 * ka + 2] - x[3 * ja + 2]};
-    min_image(C, u);
-    min_image(C, v);
+    min_image<kOrtho>(C, u);
+    min_image<kOrtho>(C, v);
     const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (lu * lv);
     cs = cs < -1 ? -1 : cs > 1 ? 1 : cs;
@@ -134,26 +135,50 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
       fi[m] = (v[m] * (1 / (lu * lv)) - u[m] * (cs / (lu * lu))) * gg;
       fk[m] = (u[m] * (1 / (lu * lv)) - v[m] * (cs / (lv * lv))) * gg;
     }
-    double* o0 = slot + 3 * size_t(p0);
-    double* o1 = slot + 3 * size_t(p1);
-    double* o2 = slot + 3 * size_t(p2);
-    for (int m = 0; m < 3; ++m) o0[m] = fi[m], o1[m] = -(fi[m] + fk[m]), o2[m] = fk[m];
-    if (kE) {
+    put(p0, fi[0], fi[1], fi[2]);
+    put(p1, -(fi[0] + fk[0]), -(fi[1] + fk[1]), -(fi[2] + fk[2]));
+    put(p2, fk[0], fk[1], fk[2]);
+    if (kE && own) {
       A[1] += kt * dth * dth;
       vir_add(A, u, fi);
       vir_add(A, v, fk);
     }
   }
-  if (kE) {
-    __shared__ double red[kEB][256];
-    for (int q = 0; q < kEB; ++q) red[q][threadIdx.x] = A[q];
+}
+
+// the sums of kEB partials over a 256-thread workgroup (red holds kEB·256 doubles), written to out by the first threads
+__device__ __forceinline__ void block_sums(const double* A, double* red, double* __restrict__ out) {
+  for (int q = 0; q < kEB; ++q) red[q * 256 + threadIdx.x] = A[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < kEB; ++q) red[q * 256 + threadIdx.x] += red[q * 256 + threadIdx.x + s];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s)
-        for (int q = 0; q < kEB; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x < kEB) epart[kEB * size_t(bx) + threadIdx.x] = red[threadIdx.x][0];
+  }
+  if (threadIdx.x < kEB) out[threadIdx.x] = red[threadIdx.x * 256];
+}
+
+// one thread per term: the term's force on each member atom into that member's slot, at the member's place spos[member]
+// in the per-atom order of slots, energies and virial into per-block sums. Every term is computed once; k_sum_forces
+// adds each atom's slots, which lie together, in a fixed order. (Used when the atoms cannot be cut into ranges whose
+// slots fit a workgroup's LDS; k_bonded_sum otherwise.)
+template <bool kE>
+__device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf, CellArgs C, const double* __restrict__ x,
+                                                const int32_t* __restrict__ bij, const double* __restrict__ bkr,
+                                                const int32_t* __restrict__ aijk, const double* __restrict__ akt,
+                                                const int32_t* __restrict__ fij, const double* __restrict__ fp, const int32_t* __restrict__ spos,
+                                                double* __restrict__ slot, double* __restrict__ epart, uint32_t* __restrict__ flags) {
+  const int g = bx * blockDim.x + threadIdx.x;
+  double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  term_forces<kE, -1>(g, true, nb, na, nf, C, x, bij, bkr, aijk, akt, fij, fp, spos,
+                  [&](int32_t p, double f0, double f1, double f2) {
+                    double* o = slot + 3 * size_t(p);
+                    o[0] = f0, o[1] = f1, o[2] = f2;
+                  },
+                  A, flags);
+  if (kE) {
+    __shared__ double red[kEB * 256];
+    block_sums(A, red, epart + kEB * size_t(bx));
   }
 }
 template <bool kE>
@@ -204,6 +229,84 @@ __global__ void __launch_bounds__(256) k_sum_forces(int n, const uint32_t* __res
                                                     const float* __restrict__ frec, const int32_t* __restrict__ aoff,
                                                     const double* __restrict__ slot, double* __restrict__ f) {
   b_sum_forces(blockIdx.x, gridDim.x, n, iperm, fnb, frec, aoff, slot, f);
+}
+
+// The bonded terms and the force sum in one launch, by ranges of consecutive atoms (cut on the host so that a range's
+// slots fit in LDS): a workgroup evaluates every term with a member in its range (a term reaching two ranges is
+// evaluated in both, with the same arithmetic and so the same values; its energy is counted by the range of its first
+// member), keeps the forces on its own atoms in LDS at their places in the per-atom order of slots, then adds each
+// atom's slots in that order and writes the atom's total force: the sums of k_bonded + k_sum_forces, bit for bit,
+// without the slots' round trip through device memory.
+constexpr int kRangeAtoms = 256, kRangeSlots = 1024;   // 24 KB of slots per workgroup
+struct BondSumArgs {
+  int nb, na, nf;
+  CellArgs C;
+  const double* x;
+  const int32_t *bij, *aijk, *fij, *spos, *aoff;
+  const double *bkr, *akt, *fp;
+  const int32_t *rstart, *roff;   // range r: atoms rstart[r] … rstart[r+1] − 1, terms rterm[roff[r]] … (top bit: counted here)
+  const uint32_t* rterm;
+  const uint32_t* iperm;          // the pair forces by slot of the cluster list (null: none)
+  const float4* fnb;
+  const float* frec;              // the reciprocal forces (null: none)
+  double* f;
+  double* epart;                  // kEB per range
+  uint32_t* flags;
+  int bonded;                     // 0: no bonded part in this evaluation (the force sum alone)
+};
+
+template <bool kE, int kOrtho>
+__device__ __forceinline__ void b_bonded_sum(int r, const BondSumArgs& a) {
+  __shared__ double sl[3 * kRangeSlots];   // the range's slots, by place − its first place; then the energy sums
+  const int r0 = a.rstart[r], r1 = a.rstart[r + 1];
+  const int s0 = a.bonded ? a.aoff[r0] : 0, ns = a.bonded ? a.aoff[r1] - s0 : 0;
+  double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (a.bonded) {
+#pragma unroll 1
+    for (int k = a.roff[r] + int(threadIdx.x); k < a.roff[r + 1]; k += blockDim.x) {
+      const uint32_t code = a.rterm[k];
+      term_forces<kE, kOrtho>(int(code & 0x7fffffffu), (code >> 31) != 0, a.nb, a.na, a.nf, a.C, a.x, a.bij, a.bkr, a.aijk, a.akt, a.fij, a.fp,
+                      a.spos,
+                      [&](int32_t p, double f0, double f1, double f2) {   // the range's own members only
+                        const int q = p - s0;
+                        if (q >= 0 && q < ns) sl[3 * q] = f0, sl[3 * q + 1] = f1, sl[3 * q + 2] = f2;
+                      },
+                      A, a.flags);
+    }
+  }
+  __syncthreads();
+  // per atom, as k_sum_forces: the pair and reciprocal forces, the bonded slots in order
+  for (int i = r0 + int(threadIdx.x); i < r1; i += blockDim.x) {
+    const float4 pq = a.fnb ? a.fnb[a.iperm[i]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float pr[3] = {0.0f, 0.0f, 0.0f};
+    if (a.frec)
+      for (int m = 0; m < 3; ++m) pr[m] = a.frec[3 * i + m];
+    double Fb[3] = {0, 0, 0};
+    if (a.bonded)
+      for (int e = a.aoff[i] - s0, e1 = a.aoff[i + 1] - s0; e < e1; ++e)
+        for (int m = 0; m < 3; ++m) Fb[m] += sl[3 * e + m];
+    double F[3] = {0, 0, 0};
+    if (a.fnb) F[0] += double(pq.x), F[1] += double(pq.y), F[2] += double(pq.z);
+    if (a.frec)
+      for (int m = 0; m < 3; ++m) F[m] += double(pr[m]);
+    if (a.bonded)
+      for (int m = 0; m < 3; ++m) F[m] += Fb[m];
+    for (int m = 0; m < 3; ++m) a.f[3 * size_t(i) + m] = F[m];
+  }
+  if (kE) {
+    static_assert(3 * kRangeSlots >= kEB * 256, "the energy sums reuse the slots' space");
+    __syncthreads();
+    block_sums(A, sl, a.epart + kEB * size_t(r));
+  }
+}
+template <bool kE, int kOrtho>
+__global__ void __launch_bounds__(256) k_bonded_sum(BondSumArgs a) {
+  b_bonded_sum<kE, kOrtho>(int(blockIdx.x), a);
+}
+// the force-only variant for orthorhombic cells (every step of most runs): held to 80 registers, so six workgroups
+// share a CU as their LDS allows, without spills
+__global__ void __launch_bounds__(256, 6) k_bonded_sum_ortho(BondSumArgs a) {
+  b_bonded_sum<false, 1>(int(blockIdx.x), a);
 }
 
 // Σ_slots x ⊗ F of the pair forces, x the listed positions X + origin (core/src/nbnxm.cpp:585-592)
@@ -510,6 +613,63 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
   B.aoff.upload(off, s);
   B.aslot.upload(idx, s);
   B.spos.upload(spos, s);
+  // ranges of consecutive atoms: at most kRangeAtoms atoms and kRangeSlots slots each (none when one atom alone has more
+  // slots: then the two-launch path). Each range lists the terms with a member in it, in term order.
+  const int64_t nterms = int64_t(B.nb) + B.na + B.nf;
+  std::vector<int32_t> rstart(1, 0), rof(size_t(n), 0);
+  bool ranged = n > 0 && nterms < (int64_t(1) << 31);
+  for (int i = 0, cnt = 0, sl = 0; i < n && ranged; ++i) {
+    const int d = off[size_t(i) + 1] - off[size_t(i)];
+    if (d > kRangeSlots) ranged = false;
+    if (cnt == kRangeAtoms || sl + d > kRangeSlots) rstart.push_back(i), cnt = 0, sl = 0;
+    ++cnt, sl += d;
+    rof[size_t(i)] = int32_t(rstart.size()) - 1;
+  }
+  if (ranged) {
+    rstart.push_back(n);
+    const int nr = int(rstart.size()) - 1;
+    std::vector<int32_t> roff(size_t(nr) + 1, 0);
+    auto members = [&](int64_t g, int32_t* m) {
+      if (g < B.nb) return m[0] = ff.bond_atoms[2 * g], m[1] = ff.bond_atoms[2 * g + 1], 2;
+      if (g < B.nb + B.na) {
+        const int64_t t = g - B.nb;
+        return m[0] = ff.angle_atoms[3 * t], m[1] = ff.angle_atoms[3 * t + 1], m[2] = ff.angle_atoms[3 * t + 2], 3;
+      }
+      const int64_t t = g - B.nb - B.na;
+      return m[0] = ff.fene_atoms[2 * t], m[1] = ff.fene_atoms[2 * t + 1], 2;
+    };
+    // each distinct range of a term's members, the first member's range first
+    auto ranges_of = [&](int64_t g, int32_t* rr) {
+      int32_t m[3];
+      const int k = members(g, m);
+      int nd = 0;
+      for (int j = 0; j < k; ++j) {
+        const int32_t q = rof[size_t(m[j])];
+        bool seen = false;
+        for (int u = 0; u < nd; ++u) seen = seen || rr[u] == q;
+        if (!seen) rr[nd++] = q;
+      }
+      return nd;
+    };
+    for (int64_t g = 0; g < nterms; ++g) {
+      int32_t rr[3];
+      const int nd = ranges_of(g, rr);
+      for (int u = 0; u < nd; ++u) ++roff[size_t(rr[u]) + 1];
+    }
+    for (int q = 0; q < nr; ++q) roff[size_t(q) + 1] += roff[size_t(q)];
+    std::vector<uint32_t> rterm(size_t(std::max(roff[size_t(nr)], 1)), 0u);
+    std::vector<int32_t> fill(roff.begin(), roff.end() - 1);
+    for (int64_t g = 0; g < nterms; ++g) {
+      int32_t rr[3];
+      const int nd = ranges_of(g, rr);
+      for (int u = 0; u < nd; ++u) rterm[size_t(fill[size_t(rr[u])]++)] = uint32_t(g) | (u == 0 ? 0x80000000u : 0u);
+    }
+    B.rstart.upload(rstart, s);
+    B.roff.upload(roff, s);
+    B.rterm.upload(rterm, s);
+    B.nranges = nr;
+  }
+  B.ranged = ranged;
   B.slot.resize(3 * (2 * size_t(B.nb) + 3 * size_t(B.na) + 2 * size_t(B.nf)) + 3);
   auto up_i = [&](DevBuf<int32_t>& d, const int32_t* p, size_t m) {
     if (m) d.upload(p, m, s);
@@ -541,6 +701,45 @@ void bonded_forces(capsgpu_ctx& c, bool want) {
   } else {
     hipLaunchKernelGGL(k_bonded<false>, dim3(tb), dim3(256), 0, c.stream, B.nb, B.na, B.nf, C, c.x.p, B.bij.p, B.bkr.p, B.aijk.p, B.akt.p,
                        B.fij.p, B.fp.p, B.spos.p, B.slot.p, nullptr, c.flags.p);
+  }
+  LAUNCHCHECK();
+  kt_end(c);
+}
+
+void bonded_and_sum(capsgpu_ctx& c, bool want, bool nb, bool rec, bool bond, double* fdst) {
+  Bonded& B = c.bonded;
+  if (!B.ranged) {
+    if (bond) bonded_forces(c, want);
+    sum_forces(c, nb, rec, bond, fdst);
+    return;
+  }
+  BondSumArgs a{};
+  a.nb = B.nb, a.na = B.na, a.nf = B.nf;
+  a.C = cell_args(c);
+  a.x = c.x.p;
+  a.bij = B.bij.p, a.aijk = B.aijk.p, a.fij = B.fij.p, a.spos = B.spos.p, a.aoff = B.aoff.p;
+  a.bkr = B.bkr.p, a.akt = B.akt.p, a.fp = B.fp.p;
+  a.rstart = B.rstart.p, a.roff = B.roff.p, a.rterm = B.rterm.p;
+  a.iperm = nb ? c.L.iperm.p : nullptr;
+  a.fnb = nb ? c.L.fnb.p : nullptr;
+  a.frec = rec ? c.pme.fpme.p : nullptr;
+  a.f = fdst;
+  a.flags = c.flags.p;
+  a.bonded = bond ? 1 : 0;
+  const bool ke = want && bond;
+  if (ke) {
+    B.eblocks = B.nranges;
+    B.epart.resize(kEB * size_t(B.nranges));
+    a.epart = B.epart.p;
+  }
+  kt_begin(c, "k_bonded_sum");
+  const dim3 grid(unsigned(B.nranges)), block(256);
+  if (a.C.ortho) {
+    if (ke) hipLaunchKernelGGL((k_bonded_sum<true, 1>), grid, block, 0, c.stream, a);
+    else hipLaunchKernelGGL(k_bonded_sum_ortho, grid, block, 0, c.stream, a);
+  } else {
+    if (ke) hipLaunchKernelGGL((k_bonded_sum<true, 0>), grid, block, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_bonded_sum<false, 0>), grid, block, 0, c.stream, a);
   }
   LAUNCHCHECK();
   kt_end(c);
@@ -663,6 +862,9 @@ struct StepRep {
   const double *bkr, *akt, *fp;
   double* slot;
   const float* frec;
+  const int32_t *rstart, *roff;
+  const uint32_t* rterm;
+  int bonded;
 };
 
 __device__ inline int find_rep(const int* __restrict__ off, int nr, int b) {
@@ -695,6 +897,18 @@ __global__ void __launch_bounds__(256) kg_sum_forces(const StepRep* __restrict__
   b_sum_forces(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, nullptr, nullptr, a.frec, a.aoff, a.slot, a.f);
 }
 
+__global__ void __launch_bounds__(256) kg_bonded_sum(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int r = find_rep(off, nr, blockIdx.x);
+  const StepRep& a = R[r];
+  BondSumArgs b{};
+  b.nb = a.nb, b.na = a.na, b.nf = a.nf, b.C = a.C, b.x = a.x;
+  b.bij = a.bij, b.aijk = a.aijk, b.fij = a.fij, b.spos = a.spos, b.aoff = a.aoff;
+  b.bkr = a.bkr, b.akt = a.akt, b.fp = a.fp;
+  b.rstart = a.rstart, b.roff = a.roff, b.rterm = a.rterm;
+  b.frec = a.frec, b.f = a.f, b.flags = a.flags, b.bonded = a.bonded;
+  b_bonded_sum<false, -1>(int(blockIdx.x) - off[r], b);
+}
+
 __global__ void __launch_bounds__(256) kg_kick(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
   const int r = find_rep(off, nr, blockIdx.x);
   const StepRep& a = R[r];
@@ -710,8 +924,9 @@ __global__ void kg_step_advance(const StepRep* __restrict__ R, int nr, uint64_t 
 struct StepGroup {
   int nr = 0;
   DevBuf<unsigned char> reps;             // StepRep[nr]
-  DevBuf<int32_t> off_atoms, off_terms;   // workgroup prefixes
-  int h_atoms = 0, h_terms = 0;
+  DevBuf<int32_t> off_atoms, off_terms, off_ranges;   // workgroup prefixes
+  int h_atoms = 0, h_terms = 0, h_ranges = 0;
+  bool ranged = true;                                 // every replica's atoms in ranges: one bonded-and-sum launch
 };
 
 StepGroup* step_group_create(const std::vector<StepGroupInput>& in, hipStream_t s) {
@@ -719,7 +934,7 @@ StepGroup* step_group_create(const std::vector<StepGroupInput>& in, hipStream_t 
   g->nr = int(in.size());
   if (g->nr < 1) throw Error(-1, "a step group needs a replica");
   std::vector<StepRep> R(in.size());
-  std::vector<int32_t> oa(1, 0), ot(1, 0);
+  std::vector<int32_t> oa(1, 0), ot(1, 0), orr(1, 0);
   for (size_t i = 0; i < in.size(); ++i) {
     capsgpu_ctx& c = *in[i].c;
     Bonded& B = c.bonded;
@@ -733,12 +948,15 @@ StepGroup* step_group_create(const std::vector<StepGroupInput>& in, hipStream_t 
     a.bij = B.bij.p, a.aijk = B.aijk.p, a.fij = B.fij.p, a.spos = B.spos.p, a.aoff = in[i].bonded ? B.aoff.p : nullptr;
     a.bkr = B.bkr.p, a.akt = B.akt.p, a.fp = B.fp.p, a.slot = B.slot.p;
     a.frec = in[i].recip ? c.pme.fpme.p : nullptr;
+    a.rstart = B.rstart.p, a.roff = B.roff.p, a.rterm = B.rterm.p, a.bonded = in[i].bonded ? 1 : 0;
+    g->ranged = g->ranged && B.ranged;
+    orr.push_back(orr.back() + B.nranges);
     oa.push_back(oa.back() + int(blocks_for(size_t(c.n), 256)));
     ot.push_back(ot.back() + int(blocks_for(size_t(std::max(a.nb + a.na + a.nf, 1)), 256)));
   }
   g->reps.upload(reinterpret_cast<const unsigned char*>(R.data()), R.size() * sizeof(StepRep), s);
-  g->off_atoms.upload(oa, s), g->off_terms.upload(ot, s);
-  g->h_atoms = oa.back(), g->h_terms = ot.back();
+  g->off_atoms.upload(oa, s), g->off_terms.upload(ot, s), g->off_ranges.upload(orr, s);
+  g->h_atoms = oa.back(), g->h_terms = ot.back(), g->h_ranges = orr.back();
   return g.release();
 }
 
@@ -752,10 +970,15 @@ void step_group_open(StepGroup& g, hipStream_t s, uint64_t step_add) {
 
 void step_group_close(StepGroup& g, hipStream_t s) {
   const StepRep* R = reinterpret_cast<const StepRep*>(g.reps.p);
-  hipLaunchKernelGGL(kg_bonded, dim3(unsigned(g.h_terms)), dim3(256), 0, s, R, g.off_terms.p, g.nr);
-  LAUNCHCHECK();
-  hipLaunchKernelGGL(kg_sum_forces, dim3(unsigned(g.h_atoms)), dim3(256), 0, s, R, g.off_atoms.p, g.nr);
-  LAUNCHCHECK();
+  if (g.ranged) {
+    hipLaunchKernelGGL(kg_bonded_sum, dim3(unsigned(g.h_ranges)), dim3(256), 0, s, R, g.off_ranges.p, g.nr);
+    LAUNCHCHECK();
+  } else {
+    hipLaunchKernelGGL(kg_bonded, dim3(unsigned(g.h_terms)), dim3(256), 0, s, R, g.off_terms.p, g.nr);
+    LAUNCHCHECK();
+    hipLaunchKernelGGL(kg_sum_forces, dim3(unsigned(g.h_atoms)), dim3(256), 0, s, R, g.off_atoms.p, g.nr);
+    LAUNCHCHECK();
+  }
   hipLaunchKernelGGL(kg_kick, dim3(unsigned(g.h_atoms)), dim3(256), 0, s, R, g.off_atoms.p, g.nr);
   LAUNCHCHECK();
 }
