@@ -859,6 +859,12 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     };
     const auto t0 = std::chrono::steady_clock::now();
     bool alpha_pending = false;
+    // a step's closing half kicks are put off to the next step's first launch when nothing reads the velocities in
+    // between (no thermo row, barostat, checkpoint or Bussi scaling, not the last step): one read and write of v less
+    Kicks pend;
+    auto quiet = [&](int64_t step, bool thermo, bool bstep) {
+      return o->thermostat != 1 && !thermo && !bstep && !checkpoint_due(step) && step < o->steps;
+    };
     for (int64_t step = 1; step <= o->steps; ++step) {
       const bool thermo = step % every == 0 || step == o->steps;
       if (ramp) {
@@ -870,9 +876,10 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       if (listed) c->flags.zero(s);
       if (R > 1) {
         const double h = dt / R, hkh = 0.5 * h * kAcc;
-        kick(*c, hk, false, kpart.p, fslow.p);
+        pend.put(hk, fslow.p);
         for (int j = 0; j < R; ++j) {
-          vv_open(*c, hkh, h, nullptr, lim_b, lim_p, ffast.p);
+          vv_open(*c, hkh, h, nullptr, lim_b, lim_p, ffast.p, pend);
+          pend = Kicks{};
           if (j + 1 < R) {
             evaluate(*c, false, t, true, 1, ffast.p);
           } else {
@@ -882,9 +889,14 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
             }
             full_eval(thermo || bstep, t, true);
           }
-          kick(*c, hkh, false, kpart.p, ffast.p);
+          pend.put(hkh, ffast.p);
         }
-        kick(*c, hk, thermo || o->thermostat == 1, kpart.p, fslow.p);
+        if (quiet(step, thermo, bstep)) {
+          pend.put(hk, fslow.p);
+        } else {
+          kick(*c, hk, thermo || o->thermostat == 1, kpart.p, fslow.p, pend);
+          pend = Kicks{};
+        }
         if (o->thermostat == 1) {
           const double r1 = rng.normal();
           const double sum = ndof > 1 ? rng.gamma((ndof - 1) / 2, 2.0) : 0.0;
@@ -901,11 +913,12 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
         continue;
       }
       if (o->thermostat == 2) {
-        baoab_open(*c, hk, 0.5 * dt, c1, c2_now, sdv.p, o->seed, nullptr, uint64_t(step + o->step_offset), lim_b, lim_p, bpart.p);
+        baoab_open(*c, hk, 0.5 * dt, c1, c2_now, sdv.p, o->seed, nullptr, uint64_t(step + o->step_offset), lim_b, lim_p, bpart.p, pend);
       } else {
-        vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p);
+        vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p, nullptr, pend);
         alpha_pending = false;
       }
+      pend = Kicks{};
       if (listed) {
         read_flags(*c);
         if (c->h_flags[3]) throw Error(CAPSGPU_EFIELD, "the simulation became unstable at step " + std::to_string(step));
@@ -919,7 +932,8 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
       } else {
         evaluate(*c, thermo || bstep, t, true);
       }
-      kick(*c, hk, thermo || o->thermostat == 1, kpart.p);
+      if (!LR && quiet(step, thermo, bstep)) pend.put(hk, c->f.p);
+      else kick(*c, hk, thermo || o->thermostat == 1, kpart.p);
       if (o->thermostat == 1) {
         const double r1 = rng.normal();
         const double sum = ndof > 1 ? rng.gamma((ndof - 1) / 2, 2.0) : 0.0;

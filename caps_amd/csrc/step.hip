@@ -423,45 +423,65 @@ __device__ inline void block_sum_store(double (&v)[kParts], double* part, int ro
   }
 }
 
-// v ← α v (Bussi, the previous step) then v += h·kAcc/m F; x += dt v
+// atom i's velocity with the put-off half kicks applied, in order: the sums of as many k_kick launches
+// The integrator's arithmetic is written out (fma where a product is added, contraction off otherwise), so a kick
+// gives the same bits in whichever launch it runs: alone, put off into the next launch, or in a grouped launch.
+__device__ __forceinline__ void kicked_v(const Kicks& K, const double* __restrict__ minv, const double* __restrict__ v, int i, double* w) {
+#pragma clang fp contract(off)
+  for (int m = 0; m < 3; ++m) w[m] = v[3 * i + m];
+#pragma unroll
+  for (int j = 0; j < Kicks::kMax; ++j)
+    if (j < K.n) {
+      const double a = K.hk[j] * minv[i];
+      for (int m = 0; m < 3; ++m) w[m] = fma(a, K.f[j][3 * i + m], w[m]);
+    }
+}
+
+// the put-off half kicks, then v ← α v (Bussi, the previous step) and v += h·kAcc/m F; x += dt v
 __global__ void __launch_bounds__(256) k_vv_open(int n, double hk, double dt, const double* __restrict__ minv, const double* __restrict__ alpha,
                                                  const double* __restrict__ f, double* __restrict__ x, double* __restrict__ v,
                                                  const double* __restrict__ x0, const double* __restrict__ x0p, double lim_b, double lim_p,
-                                                 uint32_t* __restrict__ flags) {
+                                                 uint32_t* __restrict__ flags, Kicks pre) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const double al = alpha ? *alpha : 1.0, a = hk * minv[i];
+    double w[3];
+    kicked_v(pre, minv, v, i, w);
     for (int m = 0; m < 3; ++m) {
-      const double vv = al * v[3 * i + m] + a * f[3 * i + m];
+      const double vv = fma(a, f[3 * i + m], al * w[m]);
       v[3 * i + m] = vv;
-      x[3 * i + m] += dt * vv;
+      x[3 * i + m] = fma(dt, vv, x[3 * i + m]);
     }
   }
   move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
 }
 
-// B (half kick), A (half drift), O, A; partials: kinetic energy before and after O (the bath's share)
+// the put-off half kicks, then B (half kick), A (half drift), O, A; partials: kinetic energy before and after O (the
+// bath's share)
 __device__ __forceinline__ void b_baoab_open(int bx, int gx, int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
                                                     const double* __restrict__ sdv, uint64_t seed, const uint64_t* __restrict__ stepc,
                                                     uint64_t step_add, const double* __restrict__ f,
                                                     double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
                                                     const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part, const Kicks& pre) {
+#pragma clang fp contract(off)
   const int i = bx * blockDim.x + threadIdx.x;
   const uint64_t step = (stepc ? *stepc : 0) + step_add;
   double ke[1] = {0};
   if (i < n) {
-    const double a = hk * minv[i], mi = 1.0 / minv[i];
-    double g[3];
+    const double a = hk * minv[i], mi = 1.0 / minv[i], cs = c2 * sdv[i];
+    double g[3], w[3];
     normals3(seed, step, uint32_t(i), g);
+    kicked_v(pre, minv, v, i, w);
     double k0 = 0, k1 = 0;
     for (int m = 0; m < 3; ++m) {
-      double vv = v[3 * i + m] + a * f[3 * i + m];
-      double xx = x[3 * i + m] + hdt * vv;
-      k0 += vv * vv;
-      vv = c1 * vv + c2 * sdv[i] * g[m];
-      k1 += vv * vv;
-      xx += hdt * vv;
+      double vv = fma(a, f[3 * i + m], w[m]);
+      double xx = fma(hdt, vv, x[3 * i + m]);
+      k0 = fma(vv, vv, k0);
+      vv = fma(c1, vv, cs * g[m]);
+      k1 = fma(vv, vv, k1);
+      xx = fma(hdt, vv, xx);
       v[3 * i + m] = vv;
       x[3 * i + m] = xx;
     }
@@ -475,20 +495,23 @@ __global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt
                                                     uint64_t step_add, const double* __restrict__ f,
                                                     double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
                                                     const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
-                                                    double* __restrict__ part) {
-  b_baoab_open(blockIdx.x, gridDim.x, n, hk, hdt, c1, c2, minv, sdv, seed, stepc, step_add, f, x, v, x0, x0p, lim_b, lim_p, flags, part);
+                                                    double* __restrict__ part, Kicks pre) {
+  b_baoab_open(blockIdx.x, gridDim.x, n, hk, hdt, c1, c2, minv, sdv, seed, stepc, step_add, f, x, v, x0, x0p, lim_b, lim_p, flags, part, pre);
 }
 
-// the closing half kick; kinetic energy partials (Σ ½ m v², and the tensor m v_a v_b) when asked
+// the put-off half kicks and the closing half kick; kinetic energy partials (Σ ½ m v², and the tensor m v_a v_b) when
+// asked
 template <bool kKE>
 __device__ __forceinline__ void b_kick(int bx, int gx, int n, double hk, const double* __restrict__ minv, const double* __restrict__ f,
-                                              double* __restrict__ v, double* __restrict__ part) {
+                                              double* __restrict__ v, double* __restrict__ part, const Kicks& pre) {
+#pragma clang fp contract(off)
   const int i = bx * blockDim.x + threadIdx.x;
   double k[7] = {0, 0, 0, 0, 0, 0, 0};
   if (i < n) {
     const double a = hk * minv[i];
-    double vv[3];
-    for (int m = 0; m < 3; ++m) vv[m] = v[3 * i + m] + a * f[3 * i + m], v[3 * i + m] = vv[m];
+    double vv[3], w[3];
+    kicked_v(pre, minv, v, i, w);
+    for (int m = 0; m < 3; ++m) vv[m] = fma(a, f[3 * i + m], w[m]), v[3 * i + m] = vv[m];
     if (kKE) {
       const double mi = 1.0 / minv[i];
       k[0] = 0.5 * mi * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]) / kAcc;
@@ -500,8 +523,8 @@ __device__ __forceinline__ void b_kick(int bx, int gx, int n, double hk, const d
 }
 template <bool kKE>
 __global__ void __launch_bounds__(256) k_kick(int n, double hk, const double* __restrict__ minv, const double* __restrict__ f,
-                                              double* __restrict__ v, double* __restrict__ part) {
-  b_kick<kKE>(blockIdx.x, gridDim.x, n, hk, minv, f, v, part);
+                                              double* __restrict__ v, double* __restrict__ part, Kicks pre) {
+  b_kick<kKE>(blockIdx.x, gridDim.x, n, hk, minv, f, v, part, pre);
 }
 
 // Bussi's α from the reduced kinetic energy (dynamics.cpp:466-477); r1 and Σ R² drawn on the host in the reference's
@@ -776,30 +799,30 @@ void check_moves(capsgpu_ctx& c, double lim_build, double lim_prune) {
   kt_end(c);
 }
 
-void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p, const double* f) {
+void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p, const double* f, const Kicks& pre) {
   kt_begin(c, "k_vv_open");
   hipLaunchKernelGGL(k_vv_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, dt, c.d_minv.p, alpha, f ? f : c.f.p, c.x.p,
-                     c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p);
+                     c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, pre);
   LAUNCHCHECK();
   kt_end(c);
 }
 
 void baoab_open(capsgpu_ctx& c, double hk, double hdt, double c1, double c2, const double* sdv, uint64_t seed, const uint64_t* stepc,
-                uint64_t step_add, double lim_b, double lim_p, double* part) {
+                uint64_t step_add, double lim_b, double lim_p, double* part, const Kicks& pre) {
   kt_begin(c, "k_baoab_open");
   hipLaunchKernelGGL(k_baoab_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, hdt, c1, c2, c.d_minv.p, sdv, seed, stepc,
-                     step_add, c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part);
+                     step_add, c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part, pre);
   LAUNCHCHECK();
   kt_end(c);
 }
 
-void kick(capsgpu_ctx& c, double hk, bool ke, double* part, const double* f) {
+void kick(capsgpu_ctx& c, double hk, bool ke, double* part, const double* f, const Kicks& pre) {
   if (!f) f = c.f.p;
   kt_begin(c, "k_kick");
   if (ke)
-    hipLaunchKernelGGL(k_kick<true>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, f, c.v.p, part);
+    hipLaunchKernelGGL(k_kick<true>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, f, c.v.p, part, pre);
   else
-    hipLaunchKernelGGL(k_kick<false>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, f, c.v.p, part);
+    hipLaunchKernelGGL(k_kick<false>, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, c.d_minv.p, f, c.v.p, part, pre);
   LAUNCHCHECK();
   kt_end(c);
 }
@@ -881,7 +904,7 @@ __global__ void __launch_bounds__(256) kg_baoab_open(const StepRep* __restrict__
   const int r = find_rep(off, nr, blockIdx.x);
   const StepRep& a = R[r];
   b_baoab_open(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.hk, a.hdt, a.c1, a.c2, a.minv, a.sdv, a.seed, a.stepc, step_add, a.f,
-               a.x, a.v, a.x0, a.x0p, -1.0, -1.0, a.flags, a.bpart);
+               a.x, a.v, a.x0, a.x0p, -1.0, -1.0, a.flags, a.bpart, Kicks{});
 }
 
 __global__ void __launch_bounds__(256) kg_bonded(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
@@ -912,7 +935,7 @@ __global__ void __launch_bounds__(256) kg_bonded_sum(const StepRep* __restrict__
 __global__ void __launch_bounds__(256) kg_kick(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
   const int r = find_rep(off, nr, blockIdx.x);
   const StepRep& a = R[r];
-  b_kick<false>(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.hk, a.minv, a.f, a.v, a.kpart);
+  b_kick<false>(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.hk, a.minv, a.f, a.v, a.kpart, Kicks{});
 }
 
 __global__ void kg_step_advance(const StepRep* __restrict__ R, int nr, uint64_t by) {
