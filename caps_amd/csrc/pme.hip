@@ -923,7 +923,9 @@ __global__ void __launch_bounds__(256) k_fft_y(GridArgs G, FftPlan FP, const flo
 // spectrum stays in LDS between the passes (one read of the plane and one write instead of two each). The inverse runs
 // the same steps backwards (k_fft_y(−1), then k_fft_x_c2r). Used when K2 is even (the pairs stay within a plane) and the
 // plane fits: LDS holds two regions of plane_region() elements (the Stockham stages' source and destination), then the
-// twiddles of a and b.
+// twiddles of a and b. With fewer planes than CUs, each plane goes to two workgroups (part 0 and 1 of parts = 2): the
+// forward transform's both run every line along a and each its half of the lines along b, the inverse's both every line
+// along b and each its half of the row pairs along a — three quarters of a plane's work each, on twice the CUs.
 constexpr int kPlaneThreads = 1024, kPlaneBatch = 8;
 __host__ __device__ inline int plane_region(int K1, int K2) {
   const int h1 = K1 / 2 + 1, a = ((K2 + 1) / 2) * line_stride(K1), b = h1 * line_stride(K2);
@@ -937,8 +939,8 @@ __device__ inline void plane_twiddles(const float2* __restrict__ tw, const FftPl
 
 // KA, KB: the plane's sizes as compile-time constants (0: the run-time G.k), so the common grids index with shifts
 template <int KA, int KB>
-__device__ __forceinline__ void b_fft_xy_fwd(int kz, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
-                                             float2* __restrict__ half) {
+__device__ __forceinline__ void b_fft_xy_fwd(int kz, int part, int parts, GridArgs G, FftPlan FP, const float2* __restrict__ tw,
+                                             const float* __restrict__ grid, float2* __restrict__ half) {
   extern __shared__ float2 lds[];
   const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, ls1 = line_stride(K1), ls2 = line_stride(K2), npair = K2 / 2;
   const int S = plane_region(K1, K2);
@@ -970,9 +972,11 @@ __device__ __forceinline__ void b_fft_xy_fwd(int kz, GridArgs G, FftPlan FP, con
   __syncthreads();
   float2* Z = fft_lines<+1>(A, B, npair, FP.ax[0], twa);
   float2* Y = Z == A ? B : A;
+  // this part's lines along b: a-frequencies m0 … m1 − 1
+  const int hp = (h1 + parts - 1) / parts, m0 = part * hp, m1 = min(h1, m0 + hp), nm = m1 - m0;
   // the pairs' half spectra as the b-lines' elements: line m (a-frequency), element j (row)
-  for (int t = threadIdx.x; t < npair * h1; t += blockDim.x) {
-    const int pl = t / h1, m = t - pl * h1;
+  for (int t = threadIdx.x; t < npair * nm; t += blockDim.x) {
+    const int pl = t / nm, m = m0 + (t - pl * nm);
     const float2 zm = Z[pl * ls1 + lds_at(m)], zc0 = Z[pl * ls1 + lds_at((K1 - m) % K1)];
     const float2 zc = make_float2(zc0.x, -zc0.y);
     const float2 d = make_float2(zm.x - zc.x, zm.y - zc.y);
@@ -980,17 +984,17 @@ __device__ __forceinline__ void b_fft_xy_fwd(int kz, GridArgs G, FftPlan FP, con
     Y[m * ls2 + lds_at(2 * pl + 1)] = make_float2(0.5f * d.y, -0.5f * d.x);
   }
   __syncthreads();
-  const float2* W = fft_lines<+1>(Y, Z, h1, FP.ax[1], twb);
+  const float2* W = fft_lines<+1>(Y + m0 * ls2, Z + m0 * ls2, nm, FP.ax[1], twb);   // line m at W + (m − m0) ls2
   float2* dst = half + size_t(kz) * K2 * h1;
-  for (int t = threadIdx.x; t < K2 * h1; t += blockDim.x) {
-    const int j = t / h1, m = t - j * h1;
-    dst[t] = W[m * ls2 + lds_at(j)];
+  for (int t = threadIdx.x; t < K2 * nm; t += blockDim.x) {
+    const int j = t / nm, m = t - j * nm;
+    dst[size_t(j) * h1 + m0 + m] = W[m * ls2 + lds_at(j)];
   }
 }
 
 template <int KA, int KB>
-__device__ __forceinline__ void b_fft_xy_inv(int kz, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
-                                             float* __restrict__ grid) {
+__device__ __forceinline__ void b_fft_xy_inv(int kz, int part, int parts, GridArgs G, FftPlan FP, const float2* __restrict__ tw,
+                                             const float2* __restrict__ half, float* __restrict__ grid) {
   extern __shared__ float2 lds[];
   const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, ls1 = line_stride(K1), ls2 = line_stride(K2), npair = K2 / 2;
   const int S = plane_region(K1, K2);
@@ -1020,44 +1024,50 @@ __device__ __forceinline__ void b_fft_xy_inv(int kz, GridArgs G, FftPlan FP, con
   __syncthreads();
   float2* Y = fft_lines<-1>(A, B, h1, FP.ax[1], twb);
   float2* X = Y == A ? B : A;
+  // this part's row pairs: p0 … p1 − 1 (rows 2 p0 … 2 p1 − 1)
+  const int pp = (npair + parts - 1) / parts, p0 = part * pp, p1 = min(npair, p0 + pp), np = p1 - p0;
   // two rows' half spectra rebuilt whole into one complex line, Z = A + iB
-  for (int t = threadIdx.x; t < npair * K1; t += blockDim.x) {
-    const int pl = t / K1, x = t - pl * K1;
+  for (int t = threadIdx.x; t < np * K1; t += blockDim.x) {
+    const int pl = p0 + t / K1, x = t - (pl - p0) * K1;
     const int m = x > K1 / 2 ? K1 - x : x;
     float2 a = Y[m * ls2 + lds_at(2 * pl)], b = Y[m * ls2 + lds_at(2 * pl + 1)];
     if (x > K1 / 2) a.y = -a.y, b.y = -b.y;
     X[pl * ls1 + lds_at(x)] = make_float2(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
-  const float2* Z = fft_lines<-1>(X, Y, npair, FP.ax[0], twa);
+  const float2* Z = fft_lines<-1>(X + p0 * ls1, Y + p0 * ls1, np, FP.ax[0], twa);   // pair pl at Z + (pl − p0) ls1
   const float* Zf = reinterpret_cast<const float*>(Z);
-  float* dst = grid + size_t(kz) * K2 * K1;
-  for (int t = threadIdx.x; t < K2 * K1; t += blockDim.x) {
+  float* dst = grid + size_t(kz) * K2 * K1 + size_t(2 * p0) * K1;
+  for (int t = threadIdx.x; t < 2 * np * K1; t += blockDim.x) {
     const int row = t / K1, x = t - row * K1;
     dst[t] = Zf[2 * ((row >> 1) * ls1 + lds_at(x)) + (row & 1)];
   }
 }
 
-__device__ __forceinline__ void b_fft_xy(int kz, const GridArgs& G, const FftPlan& FP, const float2* __restrict__ tw, float* __restrict__ grid,
-                                         float2* __restrict__ half, int forward) {
+__device__ __forceinline__ void b_fft_xy(int kz, int part, int parts, const GridArgs& G, const FftPlan& FP, const float2* __restrict__ tw,
+                                         float* __restrict__ grid, float2* __restrict__ half, int forward) {
   const int K1 = G.k[0], K2 = G.k[1];
-#define CAPSGPU_PLANE(A, B)                                      \
-  if (K1 == A && K2 == B) {                                      \
-    if (forward) b_fft_xy_fwd<A, B>(kz, G, FP, tw, grid, half);  \
-    else b_fft_xy_inv<A, B>(kz, G, FP, tw, half, grid);          \
-    return;                                                      \
+#define CAPSGPU_PLANE(A, B)                                                   \
+  if (K1 == A && K2 == B) {                                                   \
+    if (forward) b_fft_xy_fwd<A, B>(kz, part, parts, G, FP, tw, grid, half);  \
+    else b_fft_xy_inv<A, B>(kz, part, parts, G, FP, tw, half, grid);          \
+    return;                                                                   \
   }
   CAPSGPU_PLANE(64, 64)
   CAPSGPU_PLANE(128, 128)
   CAPSGPU_PLANE(256, 256)
 #undef CAPSGPU_PLANE
-  if (forward) b_fft_xy_fwd<0, 0>(kz, G, FP, tw, grid, half);
-  else b_fft_xy_inv<0, 0>(kz, G, FP, tw, half, grid);
+  if (forward) b_fft_xy_fwd<0, 0>(kz, part, parts, G, FP, tw, grid, half);
+  else b_fft_xy_inv<0, 0>(kz, part, parts, G, FP, tw, half, grid);
 }
 
+// parts workgroups per plane (1 or 2); with two and a multiple of eight planes, a plane's two workgroups are b and b + 8,
+// which run on one XCD (workgroups go round-robin over the eight) and share its L2
 __global__ void __launch_bounds__(kPlaneThreads) k_fft_xy(GridArgs G, FftPlan FP, const float2* __restrict__ tw, float* __restrict__ grid,
-                                                         float2* __restrict__ half, int forward) {
-  b_fft_xy(blockIdx.x, G, FP, tw, grid, half, forward);
+                                                         float2* __restrict__ half, int forward, int parts) {
+  const int b = blockIdx.x;
+  if (parts == 2 && G.k[2] % 8 == 0) b_fft_xy((b >> 4) * 8 + (b & 7), (b >> 3) & 1, 2, G, FP, tw, grid, half, forward);
+  else b_fft_xy(b / parts, b % parts, parts, G, FP, tw, grid, half, forward);
 }
 
 // the influence function and its virial weights at half-spectrum index (a, b, c), in double (f90:518-564)
@@ -1417,6 +1427,18 @@ bool plane_fits(int device, int K1, int K2) {
   return plane_lds(K1, K2) <= size_t(limit[device]);
 }
 
+// the current device's compute units (cached per device)
+int device_cus() {
+  static std::mutex mu;
+  static int cus[64];
+  int device = 0;
+  HIPCHECK(hipGetDevice(&device));
+  if (device < 0 || device >= 64) return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!cus[device] && hipDeviceGetAttribute(&cus[device], hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus[device] = 0;
+  return cus[device];
+}
+
 void pme_grid_dims(const capsgpu_ctx& c, const double cell[12], int k[3]) {
   for (int d = 0; d < 3; ++d) {
     const double* e = cell + 3 + 3 * d;
@@ -1553,8 +1575,10 @@ void launch_fft_z_conv(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool
 void launch_fft_xy(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool forward) {
   Pme& P = c.pme;
   kt_begin(c, forward ? "k_fft_xy_fwd" : "k_fft_xy_inv");
-  hipLaunchKernelGGL(k_fft_xy, dim3(unsigned(P.k[2])), dim3(kPlaneThreads), plane_lds(P.k[0], P.k[1]), c.stream, G, F, P.tw.p, P.grid.p,
-                     P.half.p, forward ? 1 : 0);
+  // two workgroups per plane when the planes alone would leave half the CUs idle
+  const int parts = 2 * P.k[2] <= device_cus() ? 2 : 1;
+  hipLaunchKernelGGL(k_fft_xy, dim3(unsigned(parts * P.k[2])), dim3(kPlaneThreads), plane_lds(P.k[0], P.k[1]), c.stream, G, F, P.tw.p,
+                     P.grid.p, P.half.p, forward ? 1 : 0, parts);
   LAUNCHCHECK();
   kt_end(c);
 }
@@ -1822,7 +1846,7 @@ __global__ void __launch_bounds__(256) kg_fft_x(const PmeRep* __restrict__ R, co
 __global__ void __launch_bounds__(kPlaneThreads) kg_fft_xy(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr, int forward) {
   const int r = find_rep(off, nr, blockIdx.x);
   const PmeRep& a = R[r];
-  b_fft_xy(int(blockIdx.x) - off[r], a.G, a.F, a.tw, a.grid, a.half, forward);
+  b_fft_xy(int(blockIdx.x) - off[r], 0, 1, a.G, a.F, a.tw, a.grid, a.half, forward);
 }
 
 template <int Sign>
