@@ -336,7 +336,7 @@ class ScratchBinding {
     Pme& P = c.pme;
     bind(P.grid, a), bind(P.half, a), bind(P.gtab, a), bind(P.bmod, a), bind(P.theta, a), bind(P.base, a), bind(P.home, a);
     bind(P.tile, a), bind(P.tcount, a), bind(P.tfill, a), bind(P.tatoms, a), bind(P.wsort, a), bind(P.bsort, a);
-    bind(P.tstart, a), bind(P.fpme, a), bind(P.tw, a), bind(P.epart, a);
+    bind(P.tstart, a), bind(P.tlocal, a), bind(P.fpme, a), bind(P.tw, a), bind(P.epart, a);
     bind(c.bonded.slot, a), bind(c.bonded.epart, a), bind(c.partials, a);
     for (int d = 0; d < 4; ++d) bind(c.scan.tsum[d], a), bind(c.scan.toff[d], a);
     P.green_ok = false, P.counts_clear = false;

@@ -117,11 +117,20 @@ __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const
   b_splines(blockIdx.x, gridDim.x, G, n, x, q, wq, base, tcount);
 }
 
+// tstart: the tiles' first places, or with sums (the scan's block totals, nsums of them) the offsets within each scan
+// block of kScanTile tiles: then the block offsets are added here, and the tiles' first places written to tout
+// (0 … ntiles) for the kernels after this one
 __device__ __forceinline__ void b_tile_fill(int bx, int gx, int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
-                            uint32_t* __restrict__ tmp) {
+                            uint32_t* __restrict__ tmp, const int64_t* __restrict__ sums = nullptr, int nsums = 0, int ntiles = 0,
+                            int64_t* __restrict__ tout = nullptr) {
   const int i = bx * blockDim.x + threadIdx.x;
   const bool valid = i < n;
   const int t = valid ? base[i].w : 0;
+  __shared__ int64_t boff[64];
+  if (sums) {
+    block_offsets(sums, nsums, boff);
+    for (int u = i; u <= ntiles; u += gx * blockDim.x) tout[u] = tstart[u] + boff[u / kScanTile];
+  }
   // one atomic per run of lanes in the same tile; the run's lanes take consecutive places from its result (the order
   // within a tile does not matter here: k_tile_rank orders each tile by atom index)
   int head, len;
@@ -129,11 +138,11 @@ __device__ __forceinline__ void b_tile_fill(int bx, int gx, int n, const int4* _
   uint32_t at = 0;
   if (valid && pos == 0) at = atomicAdd(&tfill[t], uint32_t(len));
   at = __shfl(at, head, 64);
-  if (valid) tmp[tstart[t] + at + uint32_t(pos)] = uint32_t(i);
+  if (valid) tmp[tstart[t] + (sums ? boff[t / kScanTile] : 0) + at + uint32_t(pos)] = uint32_t(i);
 }
 __global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
-                            uint32_t* __restrict__ tmp) {
-  b_tile_fill(blockIdx.x, gridDim.x, n, base, tstart, tfill, tmp);
+                            uint32_t* __restrict__ tmp, const int64_t* __restrict__ sums, int nsums, int ntiles, int64_t* __restrict__ tout) {
+  b_tile_fill(blockIdx.x, gridDim.x, n, base, tstart, tfill, tmp, sums, nsums, ntiles, tout);
 }
 
 // the atoms of each tile in index order: an atom's place is its rank among its tile's atoms (filled unordered in tmp);
@@ -1150,6 +1159,13 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
       if (t < LY * K3 && li < nl) A[li * ls + lds_at(k)] = r[u];
     }
   }
+  // the influence function of the first batch of elements, requested now so its latency passes under the transform
+  float gpre[kFftBatch];
+#pragma unroll
+  for (int u = 0; u < kFftBatch; ++u) {
+    const int t = u * kFftThreads + threadIdx.x, li = t / K3, k = t - li * K3;
+    gpre[u] = t < nl * K3 ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+  }
   store_twiddles(twl, twf, FP.ax[2]);
   __syncthreads();
   float2* Z = fft_lines<+1>(A, B, nl, FP.ax[2], twl);
@@ -1159,7 +1175,7 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, li = t / K3, k = t - li * K3;
-      gb[u] = t < nl * K3 ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+      gb[u] = t0 == 0 ? gpre[u] : t < nl * K3 ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
@@ -1683,11 +1699,15 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   LAUNCHCHECK();
   kt_end(c);
   P.tstart.resize(size_t(ntiles) + 1);
-  c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
+  // the count scan: its blocks' offsets added by k_tile_fill when there are few blocks (one launch less), else whole
+  P.tlocal.resize(size_t(ntiles) + 1);
+  const int nsums = c.scan.blocks_only(P.tcount.p, P.tlocal.p, size_t(ntiles), s);
+  if (!nsums) c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
   P.tile.resize(size_t(n));
   P.tatoms.resize(size_t(n));
   kt_begin(c, "k_tile_fill");
-  hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, P.tstart.p, P.tfill.p, P.tile.p);
+  hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, nsums ? P.tlocal.p : P.tstart.p, P.tfill.p, P.tile.p,
+                     nsums ? c.scan.tsum[0].p : nullptr, nsums, ntiles, P.tstart.p);
   LAUNCHCHECK();
   kt_end(c);
   P.wsort.resize(4 * size_t(n));

@@ -61,6 +61,22 @@ static __global__ void __launch_bounds__(kScanThreads) k_scan_add(int64_t* __res
 }
 
 // the offsets of at most 64 blocks summed by each block's first wave from the block totals, then added
+// the exclusive offsets of at most 64 block totals, into boff[0 … 63] (LDS), by the workgroup's first wave; a barrier
+// follows, so every thread of the workgroup must call it
+__device__ inline void block_offsets(const int64_t* __restrict__ sums, int nblocks, int64_t* boff) {
+  if (threadIdx.x < 64) {
+    const int64_t v = int(threadIdx.x) < nblocks ? sums[threadIdx.x] : 0;
+    int64_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int64_t o = __shfl_up(inc, d, 64);
+      if (int(threadIdx.x) >= d) inc += o;
+    }
+    boff[threadIdx.x] = inc - v;
+  }
+  __syncthreads();
+}
+
 static __global__ void __launch_bounds__(kScanThreads) k_scan_add_small(int64_t* __restrict__ out, size_t n, const int64_t* __restrict__ sums) {
   __shared__ int64_t off;
   if (threadIdx.x < 64) {
@@ -84,6 +100,18 @@ struct Scanner {
   template <class T>
   void exclusive(const T* in, int64_t* out, size_t n, hipStream_t s) {
     run(in, out, n, 0, s);
+  }
+  // The blocks' scans alone, for a consumer that adds the block offsets itself: when n + 1 items take at most 64 blocks
+  // of kScanTile, out[i] = Σ of in over i's block before i, tsum[0][b] = block b's total, and the block count is
+  // returned; otherwise 0, with nothing launched.
+  template <class T>
+  int blocks_only(const T* in, int64_t* out, size_t n, hipStream_t s) {
+    const size_t tiles = (n + 1 + kScanTile - 1) / kScanTile;
+    if (tiles > 64) return 0;
+    tsum[0].resize(tiles + 1);
+    hipLaunchKernelGGL(k_scan_tiles<T>, dim3(unsigned(tiles)), dim3(kScanThreads), 0, s, in, out, n, tsum[0].p);
+    LAUNCHCHECK();
+    return int(tiles);
   }
 
  private:
