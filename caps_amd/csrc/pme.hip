@@ -1159,29 +1159,30 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
       if (t < LY * K3 && li < nl) A[li * ls + lds_at(k)] = r[u];
     }
   }
-  // the influence function of the first batch of elements, requested now so its latency passes under the transform
+  // the influence function, element by element as the lines were read (consecutive lanes, consecutive a-frequencies:
+  // whole cache lines), the first batch requested now so its latency passes under the transform
   float gpre[kFftBatch];
 #pragma unroll
   for (int u = 0; u < kFftBatch; ++u) {
-    const int t = u * kFftThreads + threadIdx.x, li = t / K3, k = t - li * K3;
-    gpre[u] = t < nl * K3 ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+    const int t = u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
+    gpre[u] = t < LY * K3 && li < nl ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
   }
   store_twiddles(twl, twf, FP.ax[2]);
   __syncthreads();
   float2* Z = fft_lines<+1>(A, B, nl, FP.ax[2], twl);
   double e = 0, w[6] = {0, 0, 0, 0, 0, 0};
-  for (int t0 = 0; t0 < nl * K3; t0 += kFftBatch * kFftThreads) {
+  for (int t0 = 0; t0 < LY * K3; t0 += kFftBatch * kFftThreads) {
     float gb[kFftBatch];
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
-      const int t = t0 + u * kFftThreads + threadIdx.x, li = t / K3, k = t - li * K3;
-      gb[u] = t0 == 0 ? gpre[u] : t < nl * K3 ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+      const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
+      gb[u] = t0 == 0 ? gpre[u] : t < LY * K3 && li < nl ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
-      const int t = t0 + u * kFftThreads + threadIdx.x;
-      if (t >= nl * K3) continue;
-      const int li = t / K3, k = t - li * K3, a = m0 + li;
+      const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
+      if (t >= LY * K3 || li >= nl) continue;
+      const int a = m0 + li;
       const float g = gb[u];
       float2 v = Z[li * ls + lds_at(k)];
       if (kE && (a || jy || k)) {
