@@ -24,6 +24,7 @@ namespace {
 
 struct CellArgs {
   double a[3], b[3], c[3], hi[3][3];
+  double il[3];   // orthorhombic: 1 / the edge lengths
   int ortho;
 };
 
@@ -32,7 +33,7 @@ template <int kOrtho = -1>
 __device__ inline void min_image(const CellArgs& C, double* d) {
   if (kOrtho == 1 || (kOrtho == -1 && C.ortho)) {
     const double L[3] = {C.a[0], C.b[1], C.c[2]};
-    for (int k = 0; k < 3; ++k) d[k] -= L[k] * round(d[k] / L[k]);
+    for (int k = 0; k < 3; ++k) d[k] -= L[k] * round(d[k] * C.il[k]);
     return;
   }
   // Cell::minimum_image (core/src/system.cpp:27-45)
@@ -125,15 +126,16 @@ __device__ __forceinline__ void term_forces(int g, bool own, int nb, int na, int
     min_image<kOrtho>(C, u);
     min_image<kOrtho>(C, v);
     const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (lu * lv);
+    const double ilu = 1 / lu, ilv = 1 / lv, iuv = ilu * ilv;
+    double cs = (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) * iuv;
     cs = cs < -1 ? -1 : cs > 1 ? 1 : cs;
     const double th = acos(cs), sn = fmax(sqrt(1 - cs * cs), 1e-8);
     const double dth = th - akt[2 * t + 1], kt = akt[2 * t];
-    const double gg = 2 * kt * dth / sn;
+    const double gg = 2 * kt * dth / sn, cu = cs * ilu * ilu, cv = cs * ilv * ilv;
     double fi[3], fk[3];
     for (int m = 0; m < 3; ++m) {
-      fi[m] = (v[m] * (1 / (lu * lv)) - u[m] * (cs / (lu * lu))) * gg;
-      fk[m] = (u[m] * (1 / (lu * lv)) - v[m] * (cs / (lv * lv))) * gg;
+      fi[m] = (v[m] * iuv - u[m] * cu) * gg;
+      fk[m] = (u[m] * iuv - v[m] * cv) * gg;
     }
     put(p0, fi[0], fi[1], fi[2]);
     put(p1, -(fi[0] + fk[0]), -(fi[1] + fk[1]), -(fi[2] + fk[2]));
@@ -237,7 +239,10 @@ __global__ void __launch_bounds__(256) k_sum_forces(int n, const uint32_t* __res
 // member), keeps the forces on its own atoms in LDS at their places in the per-atom order of slots, then adds each
 // atom's slots in that order and writes the atom's total force: the sums of k_bonded + k_sum_forces, bit for bit,
 // without the slots' round trip through device memory.
-constexpr int kRangeAtoms = 256, kRangeSlots = 1024;   // 24 KB of slots per workgroup
+constexpr int kRangeAtoms = 256, kRangeSlots = 1024;   // 25 KB of slots per workgroup
+// slot q's three doubles in LDS: 3q, with one double skipped after every eighth slot, so neighbouring atoms' runs of
+// slots (a few slots apart) start in different banks
+__device__ __forceinline__ int slot_at(int q) { return 3 * q + (q >> 3); }
 struct BondSumArgs {
   int nb, na, nf;
   CellArgs C;
@@ -257,7 +262,7 @@ struct BondSumArgs {
 
 template <bool kE, int kOrtho>
 __device__ __forceinline__ void b_bonded_sum(int r, const BondSumArgs& a) {
-  __shared__ double sl[3 * kRangeSlots];   // the range's slots, by place − its first place; then the energy sums
+  __shared__ double sl[3 * kRangeSlots + kRangeSlots / 8];   // the range's slots, by place − its first place; then the energy sums
   const int r0 = a.rstart[r], r1 = a.rstart[r + 1];
   const int s0 = a.bonded ? a.aoff[r0] : 0, ns = a.bonded ? a.aoff[r1] - s0 : 0;
   double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -269,7 +274,7 @@ __device__ __forceinline__ void b_bonded_sum(int r, const BondSumArgs& a) {
                       a.spos,
                       [&](int32_t p, double f0, double f1, double f2) {   // the range's own members only
                         const int q = p - s0;
-                        if (q >= 0 && q < ns) sl[3 * q] = f0, sl[3 * q + 1] = f1, sl[3 * q + 2] = f2;
+                        if (q >= 0 && q < ns) sl[slot_at(q)] = f0, sl[slot_at(q) + 1] = f1, sl[slot_at(q) + 2] = f2;
                       },
                       A, a.flags);
     }
@@ -284,7 +289,7 @@ __device__ __forceinline__ void b_bonded_sum(int r, const BondSumArgs& a) {
     double Fb[3] = {0, 0, 0};
     if (a.bonded)
       for (int e = a.aoff[i] - s0, e1 = a.aoff[i + 1] - s0; e < e1; ++e)
-        for (int m = 0; m < 3; ++m) Fb[m] += sl[3 * e + m];
+        for (int m = 0; m < 3; ++m) Fb[m] += sl[slot_at(e) + m];
     double F[3] = {0, 0, 0};
     if (a.fnb) F[0] += double(pq.x), F[1] += double(pq.y), F[2] += double(pq.z);
     if (a.frec)
@@ -594,6 +599,7 @@ CellArgs cell_args(const capsgpu_ctx& c) {
   CellArgs C{};
   for (int k = 0; k < 3; ++k) C.a[k] = c.cell[3 + k], C.b[k] = c.cell[6 + k], C.c[k] = c.cell[9 + k];
   C.ortho = C.a[1] == 0 && C.a[2] == 0 && C.b[0] == 0 && C.b[2] == 0 && C.c[0] == 0 && C.c[1] == 0;
+  C.il[0] = 1 / C.a[0], C.il[1] = 1 / C.b[1], C.il[2] = 1 / C.c[2];
   const double* a = C.a;
   const double* b = C.b;
   const double* cc = C.c;
