@@ -174,6 +174,75 @@ def force_accuracy(ca, system, opts):
                     "benchmark's positions after warm-up; bonded terms + PME reciprocal part"}
 
 
+def ewald_beta(rc, rtol):
+    """erfc(beta rc) = rtol by bisection on [0, 20/rc] (core/src/kspace.cpp:35-43)"""
+    from scipy.special import erfc
+    lo, hi = 0.0, 20.0 / rc
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if erfc(mid * rc) > rtol:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def local_floor(ca, system, opts, k=4, radius=8.0, sample=32768, seed=SEED):
+    """The error floor of any predictor that sees only the atoms within `radius` of each atom: over one k-step interval
+    of the benchmark's Langevin dynamics, the increment of PME's reciprocal force (the GPU's, both ends) against the
+    exact increment of the reciprocal-space pair field of the atoms within the radius (Ewald's erf(beta r)/r over the
+    minimum image, summed in FP64 with numpy), on a random sample of atoms. Both relative to the whole force at the
+    interval's end (bonded + PME), as the learned run judges its own predictions; the stale force's error (no increment)
+    beside it."""
+    import copy
+    import numpy as np
+    from scipy.spatial import cKDTree
+    from scipy.special import erf
+    rec = copy.copy(opts)
+    rec.parts = 4
+    ev_rec, ev_all = ca.Evaluator(system.ff, rec), ca.Evaluator(system.ff, opts)
+    s = ca.System(x=system.x.copy(), cell=system.cell.copy(), ff=system.ff, v=None if system.v is None else system.v.copy())
+    x0 = s.x.copy()
+    fr0 = ev_rec.compute(x0, s.cell)[1]
+    ca.run_dynamics(s, ca.DynamicsOptions(dt=DT, steps=k, temperature=TEMP, thermostat="langevin", tau_t=TAU, seed=seed + 7,
+                                          new_velocities=s.v is None, thermo_every=10 ** 9, energy=opts), evaluator=ev_all)
+    x1 = s.x.copy()
+    fr1 = ev_rec.compute(x1, s.cell)[1]
+    F1 = ev_all.compute(x1, s.cell)[1]
+    ev_rec.close(), ev_all.close()
+    h = s.cell[3:].reshape(3, 3)
+    if np.count_nonzero(h - np.diag(np.diag(h))):
+        return None                      # the minimum image below is for a rectangular cell
+    L = np.diag(h)
+    beta, q = ewald_beta(RC, RTOL), s.ff.charge
+    n = s.ff.n
+    idx = np.sort(np.random.default_rng(seed).choice(n, size=min(sample, n), replace=False))
+    tree = cKDTree(np.mod(x0 - s.cell[:3], L), boxsize=L)
+    nb = tree.query_ball_point(np.mod(x0[idx] - s.cell[:3], L), radius)
+    cnt = np.array([len(v) for v in nb])
+    i = np.repeat(idx, cnt)
+    j = np.concatenate([np.asarray(v, np.int64) for v in nb])
+    keep = i != j
+    i, j, row = i[keep], j[keep], np.repeat(np.arange(idx.size), cnt)[keep]
+
+    def field(x):
+        d = x[i] - x[j]
+        d -= L * np.round(d / L)
+        r = np.sqrt((d * d).sum(1))
+        g = (erf(beta * r) / r ** 2 - 2.0 * beta / np.sqrt(np.pi) * np.exp(-beta * beta * r * r) / r) / r * q[j]
+        return np.stack([np.bincount(row, weights=g * d[:, m], minlength=idx.size) for m in range(3)], 1)
+
+    dloc = 332.06371 * q[idx, None] * (field(x1) - field(x0))
+    dtrue = (fr1 - fr0)[idx]
+    den = (F1[idx] ** 2).sum()
+    return {"k": k, "radius_A": radius, "atoms_sampled": int(idx.size), "pairs": int(i.size),
+            "force_rmse_stale": float(np.sqrt((dtrue ** 2).sum() / den)),
+            "force_rmse_local_floor": float(np.sqrt(((dtrue - dloc) ** 2).sum() / den)),
+            "what": "over one k-step interval: the increment of the reciprocal force left unexplained by the exact field "
+                    "change of every atom within the radius (what a predictor seeing only those atoms can reach at best), "
+                    "relative to the whole force; the stale force's error beside it"}
+
+
 def nve_drift(ca, system, opts, steps):
     """energy drift of NVE velocity Verlet from the benchmark state, kT/ns/atom (the reference's T5 pass criterion is
     |drift| < 2 kT/ns/atom at 300 K, core/src/bench.cpp:386-390)"""
@@ -372,6 +441,8 @@ def main():
                    "force_rmse_stale": float(np.mean([r["learned_rmse_stale"] for r in judged])) if judged else None,
                    "speedup_vs_plain": secs / rep.seconds}
             out["learned_respa"]["at_tolerance_%g" % tol] = leg
+        log("the error floor of an 8 A local predictor")
+        out["learned_respa"]["floor_8A"] = local_floor(ca, system, dyn.energy)
     # r-RESPA (dynamics.cpp:113-123, 643-656) on the same system and terms: Bussi 300 K, the slow forces (here PME's
     # reciprocal part) kicking over a 2 fs outer step, the bonded terms over two 1 fs inner steps
     ro = ca.DynamicsOptions(dt=2.0, temperature=TEMP, thermostat="bussi", tau_t=TAU, seed=SEED, new_velocities=False,

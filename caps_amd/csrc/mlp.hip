@@ -2,12 +2,13 @@
 // the last PME force plus an MLP's predicted increment: ΔF_i = q_i·[N(ΔD_i) − N(0)], where ΔD_i is the change of the
 // atom's 96 radial/charge descriptors within 8 Å since the last full step and N the network (96-128-128-3, SiLU) on
 // BF16 MFMA tiles with FP32 accumulation. It is trained online (Adam, FP32 master weights) on the PME forces
-// themselves: pairs of full steps k apart give (ΔD, ΔF/q). At every full step the prediction is judged against the
-// fresh PME force: above the tolerance (relative force RMSE against the full force) the next interval runs plain PME.
+// themselves over a run's first training frames (every step a full step): pairs of full steps k apart give (ΔD, ΔF/q);
+// then it is frozen. At every full step the prediction is judged against the fresh PME force: above the tolerance
+// (relative force RMSE against the full force) the next interval runs plain PME.
 //
 //  k_cell_*      atoms binned into cells of at least 4 Å (fractional coordinates), sorted by atom in each cell
 //  k_desc        per atom: 24 hat-function radial bins × {Σ q_j h, Σ q_j h d̂_x, d̂_y, d̂_z} over neighbours within
-//                8 Å (smooth cosine cut-off), accumulated in LDS, FP32
+//                8 Å (smooth cosine cut-off), summed in registers, FP32
 //  k_delta_in    the network input: (D − D0)·s per feature, BF16
 //  k_gemm        C = A·B with 32x32x16 BF16 MFMA tiles, any strides, split K with ordered partial sums
 //  k_bias_silu   bias, SiLU and its derivative, the BF16 copy of the activation
@@ -972,7 +973,7 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
   const size_t n3 = 3 * size_t(c.n), nd = size_t(c.n) * kIn;
   // after a failed judgment the run waits with plain PME steps (no descriptors, no prediction): `wait` steps, doubling
   // with each failure in a row up to kMaxWait; then a reference full step, and k steps later a full step that judges
-  // the k-step prediction made from the reference, trains on the pair and decides whether the next interval is learned
+  // the k-step prediction made from the reference and decides whether the next interval is learned
   // — the network is always judged on the horizon it predicts, and a network that misses the tolerance costs a few
   // full steps per kMaxWait + k steps instead of one per k
   const bool waiting = L.ref_at >= 0 && step < L.ref_at;
@@ -1010,15 +1011,14 @@ int learned_step(capsgpu_ctx& c, Learned& L, int64_t step, bool want, double* t)
       ++L.judgments;
       if (!(L.rmse <= L.tol)) ++L.failures;
     }
-    // train on the pair (k steps ago, now): from the ring while every step is full, else from the last full step
+    // train on the pair (k steps ago, now) from the ring while every step is full; once the run's training frames are
+    // spent the network is frozen and the full steps only judge it
     if (training) {
       const size_t slot = size_t(L.frames % L.k);
       if (L.ring_n >= L.k) train_on_pair(c, L, L.Dring[slot].p, L.Fring[slot].p);
       HIPCHECK(hipMemcpyAsync(L.Dring[slot].p, L.Dcur.p, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
       HIPCHECK(hipMemcpyAsync(L.Fring[slot].p, L.fslow.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
       L.ring_n = std::min(L.ring_n + 1, L.k);
-    } else if (!ref_only && L.last_full >= 0 && step - L.last_full == L.k) {
-      train_on_pair(c, L, L.D0.p, L.f0.p);
     }
     ++L.frames;
     // the reference point of the next interval (the current descriptors become D0; Dcur is rewritten every step)

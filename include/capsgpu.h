@@ -161,7 +161,8 @@ typedef struct capsgpu_dyn_opts {
   int32_t learned_k;
   double learned_tol;
   int32_t learned_train;        /* learned r-RESPA: PME frames (plain steps) the network trains on before it predicts;
-                                   < 0: 2000. The network persists in the context across runs. */
+                                   < 0: 2000. Afterwards it is frozen: later full steps judge it without training it.
+                                   The network persists in the context across runs. */
   /* pressure coupling (DynamicsOptions::barostat ... barostat_every, dynamics.hpp:80-87; dynamics.cpp:701-723):
    * isotropic, every barostat_every steps after the step: 0 none, 1 Berendsen, 2 stochastic cell rescaling
    * (Bernetti & Bussi 2020), the cell and positions scaled about the cell origin, at most 1 % in volume per update */
