@@ -356,8 +356,9 @@ def main():
     dyn.steps = a.warmup
     ca.run_dynamics(system, dyn, evaluator=ev)
     dyn.new_velocities = False
-    # the mesh transforms' tile shape for this grid, picked by timing (lines per workgroup along a, and along b and c)
+    # the mesh transforms' tile shape and the gather's workgroups for this grid, picked by timing
     cfg["transform_tiles"] = list(ev.tune())
+    cfg["gather_shape"] = "runs of 64 sorted atoms" if ev.gather_shape() == 1 else "tiles"
     # an untimed profiled pass: every kernel's launch time (two HIP events per launch on the library's stream)
     log("profiled pass")
     ev.profile(True)

@@ -186,6 +186,8 @@ def lib():
         L.capsgpu_learned_descriptors.argtypes = [C.c_void_p, C.c_void_p]
         L.capsgpu_tune.argtypes = [C.c_void_p, C.c_void_p]
         L.capsgpu_set_tiles.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        L.capsgpu_set_gather.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.capsgpu_get_gather.argtypes = [C.c_void_p, C.c_void_p]
         L.capsgpu_batch_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
         L.capsgpu_batch_run.argtypes = [C.c_void_p, C.POINTER(BOpts), C.c_void_p, C.c_void_p, C.POINTER(BReport)]
         L.capsgpu_batch_destroy.argtypes = [C.c_void_p]
@@ -396,6 +398,19 @@ class Evaluator:
         t = np.zeros(2, np.int32)
         _check(lib().capsgpu_tune(self._h, _p(t)))
         return int(t[0]), int(t[1])
+
+    def gather_shape(self):
+        """the force gather's workgroups in use: 0 one per 4x4x4 tile, 1 one per run of 64 sorted atoms (picked by tune())"""
+        g = np.zeros(1, np.int32)
+        _check(lib().capsgpu_get_gather(self._h, _p(g)))
+        return int(g[0])
+
+    def set_gather(self, shape):
+        """sets the force gather's workgroups (0 per tile, 1 per run of 64 sorted atoms, one atom per lane, where K1 is a
+        multiple of 4); the forces are the same either way"""
+        g = np.zeros(1, np.int32)
+        _check(lib().capsgpu_set_gather(self._h, int(shape), _p(g)))
+        return int(g[0])
 
     def set_tiles(self, lines_x, lines_yz):
         """sets the mesh-transform tile shape (powers of two, 1 to 16 lines)"""

@@ -979,7 +979,7 @@ int32_t capsgpu_tune(capsgpu_ctx* c, int32_t tiles[2]) {
     if (!c) throw Error(CAPSGPU_EINVAL, "null argument");
     if (!pme_on(*c)) throw Error(CAPSGPU_EINVAL, "the tuned tile shapes are the PME transforms': the context has no PME");
     if (!c->state_set) throw Error(CAPSGPU_EINVAL, "set the positions first (capsgpu_set_state)");
-    if (!c->pme.green_ok) pme_forces(*c, false);
+    pme_forces(*c, false);   // the arrays the gather's candidates are timed on, for the current cell and grid
     int t[2];
     pme_tune(*c, t);
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -995,6 +995,22 @@ int32_t capsgpu_set_tiles(capsgpu_ctx* c, int32_t lines_x, int32_t lines_yz, int
       throw Error(CAPSGPU_EINVAL, "tile shapes are powers of two from 1 to 16 lines (lines_x 0: the plane transform)");
     c->pme.lines_x = lines_x, c->pme.lines_yz = lines_yz;
     if (tiles) tiles[0] = lines_x, tiles[1] = lines_yz;
+  });
+}
+
+int32_t capsgpu_set_gather(capsgpu_ctx* c, int32_t shape, int32_t* in_use) {
+  return guard([&] {
+    if (!c) throw Error(CAPSGPU_EINVAL, "null argument");
+    if (shape != 0 && shape != 1) throw Error(CAPSGPU_EINVAL, "the gather's shape is 0 (a workgroup per tile) or 1 (per run of 64 sorted atoms)");
+    c->pme.gather_shape = shape;
+    if (in_use) *in_use = shape;
+  });
+}
+
+int32_t capsgpu_get_gather(capsgpu_ctx* c, int32_t* shape) {
+  return guard([&] {
+    if (!c || !shape) throw Error(CAPSGPU_EINVAL, "null argument");
+    *shape = c->pme.gather_shape;
   });
 }
 

@@ -1333,6 +1333,114 @@ __device__ __forceinline__ void b_gather(int tile, GridArgs G, const int64_t* __
     __syncthreads();
   }
 }
+// The gather by runs of the tile order: one 64-lane workgroup per 64 consecutive atoms of the tile-sorted arrays, one
+// atom per lane (every lane busy, where the gather by tiles keeps p lanes per atom and a tile's ~8 atoms fill 40 of 64).
+// A pass takes the lanes whose home tiles share one (b, c) row of tiles and span at most kSeqTiles tiles along a; it
+// loads the box of mesh points those atoms reach — 4 (tiles + 2) points along a by p + 3 along b and c, as float4s
+// (rows of the mesh are 16-byte aligned: K1 a multiple of 4) — into LDS, then each lane computes its atom's force with
+// the per-tile gather's arithmetic in its order: the same splines, for each row dy the sums over dz and a as that
+// kernel's lane dy computes them, the rows' sums added in order dy = 0 … p − 1 as its first lane adds them. The forces
+// are bit-identical to the per-tile gather's. Dynamic LDS: the largest box, 4 (kSeqTiles + 2) (p + 3)² floats.
+constexpr int kSeqTiles = 12;
+template <int P>
+__device__ __forceinline__ void b_gather_seq(int wg, int n, GridArgs G, const uint32_t* __restrict__ tatoms,
+                                             const float4* __restrict__ wsort, const int4* __restrict__ bsort,
+                                             const float* __restrict__ grid, float* __restrict__ f) {
+  extern __shared__ float2 lds[];
+  constexpr int NY = P + 3;                               // box points along b and c
+  constexpr int NX4 = kSeqTiles + 2, RX = 4 * NX4;        // a box row: float4s, floats
+  constexpr int PS = NY * RX + 4;                         // a box plane along c, floats
+  float* box = reinterpret_cast<float*>(lds);             // [c][b][a]: planes of PS floats, rows of RX
+  float4* box4 = reinterpret_cast<float4*>(lds);
+  const int lane = threadIdx.x, c4 = lane & 15, r4 = lane >> 4;
+  const int64_t s = int64_t(wg) * 64 + lane;
+  const bool valid = s < n;
+  const int64_t sl = valid ? s : int64_t(n) - 1;
+  const uint32_t i = tatoms[sl];
+  const float4 w = wsort[sl];
+  const int4 b = bsort[sl];
+  const int tx = b.x >> 2, ty = b.y >> 2, tz = b.z >> 2;
+  const int key = tz * G.nt[1] + ty;                     // the atom's (b, c) row of tiles
+  bool done = !valid;
+  float th[3][P], dth[3][P];
+  bspline<P>(w.x, th[0], dth[0]);
+  bspline<P>(w.y, th[1], dth[1]);
+  bspline<P>(w.z, th[2], dth[2]);
+  for (;;) {
+    const uint64_t left = __ballot(!done);
+    if (!left) break;
+    // this pass: the lanes of the first remaining lane's row, up to kSeqTiles tiles along a (the sorted order keeps a
+    // row's atoms in consecutive lanes, their tiles along a non-decreasing)
+    const int first = __builtin_ffsll((long long)left) - 1;
+    const int key0 = __builtin_amdgcn_readlane(key, first), tx0 = __builtin_amdgcn_readlane(tx, first);
+    const int ty0 = __builtin_amdgcn_readlane(ty, first), tz0 = __builtin_amdgcn_readlane(tz, first);
+    const bool active = !done && key == key0 && tx < tx0 + kSeqTiles;
+    const uint64_t am = __ballot(active);
+    const int last = 63 - __builtin_clzll(am);
+    const int ntx = __builtin_amdgcn_readlane(tx, last) - tx0 + 1;
+    // the box: rows of NX4 float4s (the widest box's), ntx + 2 of them loaded (4 ntx + 8 >= 4 ntx + p - 1 points); lane
+    // = (float4 c4 of a row, row of four), so each row's column and mesh index along a are the lane's own. Planes along c
+    // are PS floats apart (one float4 more than their rows), so atoms at different heights in a tile read other banks.
+    if (c4 < ntx + 2) {
+#pragma unroll 4
+      for (int r = r4; r < NY * NY; r += 4) {
+        const int yy = r % NY, zz = r / NY;
+        int gy = 4 * ty0 + yy, gz = 4 * tz0 + zz;
+        gy -= gy >= G.k[1] ? G.k[1] : 0;
+        gy -= gy >= G.k[1] ? G.k[1] : 0;
+        gz -= gz >= G.k[2] ? G.k[2] : 0;
+        gz -= gz >= G.k[2] ? G.k[2] : 0;
+        int gx = 4 * (tx0 + c4);
+        gx -= gx >= G.k[0] ? G.k[0] : 0;
+        gx -= gx >= G.k[0] ? G.k[0] : 0;
+        box4[zz * (PS / 4) + yy * NX4 + c4] = *reinterpret_cast<const float4*>(grid + (size_t(gz) * G.k[1] + gy) * G.k[0] + gx);
+      }
+    }
+    __syncthreads();
+    if (active) {
+      // the atom's first point relative to the box's corner: 0 … 4 ntx − 1 along a, 0 … 3 along b and c
+      const int ox = b.x - 4 * tx0, oy = b.y - 4 * ty0, oz = b.z - 4 * tz0;
+      float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+      for (int sub = 0; sub < P; ++sub) {
+        const float tyv = th[1][sub], dyv = dth[1][sub];
+        const float* row = box + oz * PS + (oy + sub) * RX + ox;
+        float g0s = 0.0f, g1s = 0.0f, g2s = 0.0f;
+#pragma unroll
+        for (int dz = 0; dz < P; ++dz) {
+          float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+          for (int a = 0; a < P; ++a) {
+            const float v = row[dz * PS + a];
+            s0 += dth[0][a] * v;
+            s1 += th[0][a] * v;
+          }
+          const float tzv = th[2][dz], dzv = dth[2][dz];
+          g0s += s0 * tyv * tzv;
+          g1s += s1 * dyv * tzv;
+          g2s += s1 * tyv * dzv;
+        }
+        if (sub == 0) t0 = g0s, t1 = g1s, t2 = g2s;
+        else t0 += g0s, t1 += g1s, t2 += g2s;
+      }
+      for (int m = 0; m < 3; ++m) {
+        const double grad = G.k[0] * double(t0) * G.hi[0][m] + G.k[1] * double(t1) * G.hi[1][m] + G.k[2] * double(t2) * G.hi[2][m];
+        f[3 * size_t(i) + m] = float(-double(w.w) * grad);
+      }
+    }
+    done = done || active;
+    __syncthreads();
+  }
+}
+template <int P>
+__global__ void __launch_bounds__(64) k_gather_seq(GridArgs G, int n, const uint32_t* __restrict__ tatoms, const float4* __restrict__ wsort,
+                                                   const int4* __restrict__ bsort, const float* __restrict__ grid, float* __restrict__ f) {
+  const int nwg = (n + 63) / 64;
+  b_gather_seq<P>(int(xcd_block(blockIdx.x, nwg)), n, G, tatoms, wsort, bsort, grid, f);
+}
+// its dynamic LDS: the largest box
+size_t gather_seq_lds(int p) { return size_t(p + 3) * (size_t(4) * (kSeqTiles + 2) * (p + 3) + 4) * sizeof(float); }
+
 template <int P>
 __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tatoms,
                                                const float4* __restrict__ wsort, const int4* __restrict__ bsort,
@@ -1619,9 +1727,39 @@ void fft_forward_conv_inverse(capsgpu_ctx& c, const GridArgs& G, const FftPlan& 
   launch_fft_z_conv(c, G, F, want);
   fft_ab(c, G, F, false);
 }
+
+// the gather by runs of 64 sorted atoms where the context's shape asks for it (0 tiles, 1 runs) and the mesh rows are
+// 16-byte aligned (K1 a multiple of 4)
+bool gather_by_runs(const Pme& P, const GridArgs& G) { return P.gather_shape == 1 && G.k[0] % 4 == 0; }
+
+// the gather of the last spread's atoms from P.grid into P.fpme (the arrays of the last evaluation)
+void launch_gather(capsgpu_ctx& c, const GridArgs& G, bool seq) {
+  Pme& P = c.pme;
+  const float4* wsort = reinterpret_cast<const float4*>(P.wsort.p);
+  const int4* bsort = reinterpret_cast<const int4*>(P.bsort.p);
+  const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+  const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 4) + 64 * 6) * sizeof(float);   // box + ⌊64/p⌋ slots of 6p
+  kt_begin(c, "k_gather");
+  switch (G.p) {
+#define CAPSGPU_GATHER(Q)                                                                                                      \
+  case Q:                                                                                                                      \
+    if (seq)                                                                                                                   \
+      hipLaunchKernelGGL(k_gather_seq<Q>, dim3(unsigned((c.n + 63) / 64)), dim3(64), gather_seq_lds(Q), c.stream, G, c.n,       \
+                         P.tatoms.p, wsort, bsort, P.grid.p, P.fpme.p);                                                        \
+    else                                                                                                                       \
+      hipLaunchKernelGGL(k_gather<Q>, dim3(unsigned(ntiles)), dim3(64), lds_gather, c.stream, G, P.tstart.p, P.tatoms.p, wsort, \
+                         bsort, P.grid.p, P.fpme.p);                                                                           \
+    break;
+    CAPSGPU_GATHER(3) CAPSGPU_GATHER(4) CAPSGPU_GATHER(5) CAPSGPU_GATHER(6) CAPSGPU_GATHER(7) CAPSGPU_GATHER(8)
+#undef CAPSGPU_GATHER
+  }
+  LAUNCHCHECK();
+  kt_end(c);
+}
 }  // namespace
 
-// the transforms' tile shape of this context, picked by timing on its own mesh: every pair of the passes along a and b
+// the gather's workgroup shape and the transforms' tile shape of this context, picked by timing on its own mesh: the
+// gather by tiles or by runs of sorted atoms, and every pair of the passes along a and b
 // (the plane kernel when the planes fit, lines_x 0, or the a-lines with 2, 4, 8 or 16 line pairs per workgroup) and the
 // lines along b and c (4, 8 or 16 per workgroup), the whole transform with the convolution, each the best of three runs
 // timed with HIP events on the context's stream. The passes run in place on the
@@ -1650,6 +1788,15 @@ void pme_tune(capsgpu_ctx& c, int out[2]) {
     return best;
   };
   try {
+    // the gather's workgroups (one per tile, or one per run of 64 sorted atoms where the mesh rows allow it), each timed
+    // alone on the last evaluation's sorted atoms and mesh (before the transforms below overwrite the mesh); the same
+    // forces either way
+    const GridArgs G0 = grid_args(P);
+    if (G0.k[0] % 4 == 0 && P.fpme.n >= 3 * size_t(c.n)) {
+      float tg[2];
+      for (int shape = 0; shape < 2; ++shape) tg[shape] = best_of([&] { launch_gather(c, G0, shape == 1); });
+      P.gather_shape = tg[1] < tg[0] ? 1 : 0;
+    }
     // every pair of shapes over the whole transform (the b pass belongs to the plane kernel or to the line tiles, so the
     // two choices are not independent)
     float tbest = 1e30f;
@@ -1737,16 +1884,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   kt_end(c);
   fft_forward_conv_inverse(c, G, F, want);
   P.fpme.resize(3 * size_t(n));
-  kt_begin(c, "k_gather");
-  const size_t lds_gather = (size_t(G.p + 3) * (G.p + 3) * (G.p + 4) + 64 * 6) * sizeof(float);   // box + ⌊64/p⌋ slots of 6p
-  switch (G.p) {
-#define CAPSGPU_GATHER(Q) \
-  case Q: hipLaunchKernelGGL(k_gather<Q>, dim3(unsigned(ntiles)), dim3(64), lds_gather, s, G, P.tstart.p, P.tatoms.p, wsort, bsort, P.grid.p, P.fpme.p); break;
-    CAPSGPU_GATHER(3) CAPSGPU_GATHER(4) CAPSGPU_GATHER(5) CAPSGPU_GATHER(6) CAPSGPU_GATHER(7) CAPSGPU_GATHER(8)
-#undef CAPSGPU_GATHER
-  }
-  LAUNCHCHECK();
-  kt_end(c);
+  launch_gather(c, G, gather_by_runs(P, G));
 }
 
 

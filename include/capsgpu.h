@@ -215,12 +215,16 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* ctx, const capsgpu_dyn_opts* opts, cap
 /* ---- kernel tile shapes (no counterpart in the reference: its kernels have fixed loop shapes) --------------------------
  * The mesh transforms run a tile of lines per workgroup: line pairs along a (tiles[0]) and lines along b and c
  * (tiles[1]), powers of two; tiles[0] = 0 (the default) runs the passes along a and b together, a whole c-plane per
- * workgroup in LDS, when the grid's planes fit (K2 even, the device's LDS), else 8 line pairs. capsgpu_tune picks them
- * for this context by timing each candidate on its own mesh (after one reciprocal-space evaluation, run here if there
- * was none; needs PME and a set state) and keeps the fastest; capsgpu_set_tiles sets them. Every shape gives
- * bit-identical forces. tiles: the shape now in use (may be NULL). */
+ * workgroup in LDS, when the grid's planes fit (K2 even, the device's LDS), else 8 line pairs. The force gather runs a
+ * workgroup per 4x4x4 tile (shape 0, the default) or per run of 64 atoms of the tile order, one atom per lane (shape 1,
+ * where K1 is a multiple of 4; per tile otherwise). capsgpu_tune picks both for this context by timing each candidate on its own mesh (after a
+ * reciprocal-space evaluation run here; needs PME and a set state) and keeps the fastest; capsgpu_set_tiles and
+ * capsgpu_set_gather set them, capsgpu_get_gather reads the gather's. Every shape gives bit-identical forces. tiles,
+ * in_use: the shape now in use (may be NULL). */
 int32_t capsgpu_tune(capsgpu_ctx* ctx, int32_t tiles[2]);
 int32_t capsgpu_set_tiles(capsgpu_ctx* ctx, int32_t lines_x, int32_t lines_yz, int32_t tiles[2]);
+int32_t capsgpu_set_gather(capsgpu_ctx* ctx, int32_t shape, int32_t* in_use);
+int32_t capsgpu_get_gather(capsgpu_ctx* ctx, int32_t* shape);
 
 /* ---- replicas stepped together ------------------------------------------------------------------------------------
  * The reference runs the replicas of a cooling scan one after another, each through run_dynamics

@@ -94,3 +94,11 @@ def test_tile_entry_points_reject_bad_arguments_without_a_device():
     assert L.capsgpu_tune(None, None) == -1
     assert b"null" in L.capsgpu_last_error()
     assert L.capsgpu_set_tiles(None, 8, 16, None) == -1
+
+
+def test_gather_shape_entry_points_reject_bad_arguments_without_a_device():
+    L = ca.lib()
+    assert L.capsgpu_set_gather(None, 1, None) == -1
+    assert b"null" in L.capsgpu_last_error()
+    assert L.capsgpu_get_gather(None, None) == -1
+    assert b"null" in L.capsgpu_last_error()

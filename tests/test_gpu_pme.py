@@ -100,3 +100,29 @@ def test_reciprocal_part_at_full_size(chains, order):
     assert t.coulomb == pytest.approx(e, rel=1e-5)
     assert err < 1e-5
     assert np.abs(np.array(t.w) - vo).max() < 1e-5 * np.abs(vo).max()
+
+
+@pytest.mark.parametrize("case,order", [("tri", 5), ("tri", 3), ("32k", 4), ("32k", 6), ("32k", 8), ("262k", 5)],
+                         ids=["triclinic order 5 on 24^3", "triclinic order 3", "32k order 4 on 64^3", "32k order 6", "32k order 8",
+                              "262k order 5 on 128^3"])
+def test_gather_by_runs_gives_the_same_forces(case, order):
+    # the gather by runs of 64 sorted atoms (one lane each) runs the per-tile gather's arithmetic: forces bit-identical;
+    # the tuner keeps one of the two and the forces stay the same
+    s = H.triclinic_case(n=1100) if case == "tri" else ca.build_melt(256 if case == "32k" else 2048, seed=1)
+    print("case gather shapes", case, s.ff.n, "order", order, flush=True)
+    op = ca.EnergyOptions(cutoff=9.0 if case == "tri" else 12.0, coulomb=True, electrostatics="pme", ewald_rtol=1e-5, pme_order=order,
+                          pme_spacing=1.0, tail=False, parts=4)
+    ev = ca.Evaluator(s.ff, op)
+    assert ev.set_gather(0) == 0
+    t0, f0 = ev.compute(s.x, s.cell)
+    assert ev.set_gather(1) == 1 and ev.gather_shape() == 1
+    t1, f1 = ev.compute(s.x, s.cell)
+    assert np.array_equal(f1, f0)
+    assert t1.coulomb == t0.coulomb
+    ev.tune()
+    print("tuned gather shape", ev.gather_shape(), flush=True)
+    assert ev.gather_shape() in (0, 1)
+    t2, f2 = ev.compute(s.x, s.cell)
+    assert np.array_equal(f2, f0)
+    with pytest.raises(ValueError, match="shape"):
+        ev.set_gather(2)
