@@ -335,7 +335,7 @@ class ScratchBinding {
   ScratchBinding(capsgpu_ctx& c, Arena& a) : c_(c) {
     Pme& P = c.pme;
     bind(P.grid, a), bind(P.half, a), bind(P.gtab, a), bind(P.bmod, a), bind(P.theta, a), bind(P.base, a), bind(P.home, a);
-    bind(P.tile, a), bind(P.tcount, a), bind(P.tfill, a), bind(P.tatoms, a), bind(P.wsort, a), bind(P.bsort, a);
+    bind(P.tile, a), bind(P.tcount, a), bind(P.tfill, a), bind(P.ovf, a), bind(P.tatoms, a), bind(P.wsort, a), bind(P.bsort, a);
     bind(P.tstart, a), bind(P.tlocal, a), bind(P.fpme, a), bind(P.tw, a), bind(P.epart, a);
     bind(c.bonded.slot, a), bind(c.bonded.epart, a), bind(c.partials, a);
     for (int d = 0; d < 4; ++d) bind(c.scan.tsum[d], a), bind(c.scan.toff[d], a);
@@ -540,7 +540,8 @@ int32_t capsgpu_compute_noalloc(capsgpu_ctx* c, const double* x, const double ce
       pme_grid_dims(*c, cell, k);
       const size_t k3 = size_t(k[0]) * k[1] * k[2], nh = size_t(k[0] / 2 + 1) * k[1] * k[2];
       const size_t tiles = size_t((k[0] + 3) / 4) * ((k[1] + 3) / 4) * ((k[2] + 3) / 4);
-      need += 4 * k3 + 12 * nh + 16 * size_t(k[0] + k[1] + k[2]) + 12 * tiles + 60 * std::max({k[0], k[1], k[2]}) + 84 * n;
+      need += 4 * k3 + 12 * nh + 16 * size_t(k[0] + k[1] + k[2]) + (20 + 4 * size_t(tile_capacity(int64_t(n), int64_t(tiles)))) * tiles +
+              60 * std::max({k[0], k[1], k[2]}) + 88 * n;
     }
     std::lock_guard<std::mutex> lock(g_scratch_mu);
     unsigned char* base = scratch_on(c->device);

@@ -1,7 +1,9 @@
 // Internal: the device state of one capsgpu context and the host launchers of each kernel family.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <memory>
 #include <string>
 #include <utility>
@@ -93,7 +95,10 @@ struct Pme {
   DevBuf<float> theta;                  // per atom: the B-spline offsets w of the three axes and the charge (float4)
   DevBuf<int32_t> base;                 // per atom: the first grid index per axis and the home tile (int4)
   DevBuf<int32_t> home;                 // per axis and tile: the home tiles that reach it (count + 4)
-  DevBuf<uint32_t> tile, tcount, tfill, tatoms;   // tile: the atoms filled unordered per tile; tatoms: in index order
+  DevBuf<uint32_t> tile, tcount, tatoms;   // tile: each tile's bucket of cap slots (atoms filled unordered); tatoms: the
+                                        // atoms in tile order, each tile's in index order
+  DevBuf<uint32_t> ovf, tfill;          // the atoms past their bucket's cap; tfill: their count and a workgroup counter
+  int cap = 32;                         // slots per bucket
   bool counts_clear = false;            // tcount and tfill are zero on the device (k_tile_rank clears them)
   DevBuf<float> wsort;                  // theta in tatoms's order (float4 per atom)
   DevBuf<int32_t> bsort;                // base in tatoms's order (int4 per atom)
@@ -109,6 +114,14 @@ struct Pme {
   int gather_shape = 0;                 // the gather's workgroups: 0 one per tile, 1 one per run of 64 sorted atoms (K1 a
                                         // multiple of 4; else per tile); the same forces either way
 };
+
+// slots per tile of the PME buckets: the mean filling with a margin of eight standard deviations of a random filling
+// and 16 more, at least 32, a multiple of 8
+inline int tile_capacity(int64_t n, int64_t ntiles) {
+  const double mean = double(n) / double(ntiles > 0 ? ntiles : 1);
+  const int c = int(std::ceil(2.0 * mean + 8.0 * std::sqrt(mean) + 16.0));
+  return std::max(32, (c + 7) / 8 * 8);
+}
 
 struct Bonded {
   int nb = 0, na = 0, nf = 0;

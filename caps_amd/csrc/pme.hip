@@ -86,8 +86,14 @@ __device__ inline int wave_run(bool valid, uint32_t key, int* head, int* len) {
   return lane - *head;
 }
 
-__device__ __forceinline__ void b_splines(int bx, int gx, GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
-                          int4* __restrict__ base, uint32_t* __restrict__ tcount) {
+// Per atom: its splines' offsets w, charge, first mesh point per axis and home tile (the 4x4x4 tile of that point), and
+// a place in the home tile's bucket of cap slots: one atomic per run of lanes in the same tile (consecutive atoms, chain
+// neighbours, mostly share one), the run's lanes taking consecutive slots from its result. An atom past its bucket's cap
+// goes to the overflow list (ovf, its length in cnt[0]): correct for any filling, slow only where a tile holds more
+// than cap atoms, which the cap's margin over the mean makes a rarity in a liquid or solid.
+__device__ __forceinline__ void b_splines(int bx, int gx, GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q,
+                                          float4* __restrict__ wq, int4* __restrict__ base, uint32_t* __restrict__ tcount,
+                                          uint32_t* __restrict__ bucket, int cap, uint32_t* __restrict__ ovf, uint32_t* __restrict__ cnt) {
   const int i = bx * blockDim.x + threadIdx.x;
   const bool valid = i < n;
   int tile = 0;
@@ -107,76 +113,77 @@ __device__ __forceinline__ void b_splines(int bx, int gx, GridArgs G, int n, con
     wq[i] = make_float4(w[0], w[1], w[2], float(q[i]));
     base[i] = make_int4(b[0], b[1], b[2], tile);
   }
-  // the tile's count: one atomic per run of lanes in the same tile
   int head, len;
   const int pos = wave_run(valid, uint32_t(tile), &head, &len);
-  if (valid && pos == 0) atomicAdd(&tcount[tile], uint32_t(len));
+  uint32_t at = 0;
+  if (valid && pos == 0) at = atomicAdd(&tcount[tile], uint32_t(len));
+  at = __shfl(at, head, 64);
+  if (valid) {
+    const uint32_t slot = at + uint32_t(pos);
+    if (slot < uint32_t(cap)) bucket[size_t(tile) * cap + slot] = uint32_t(i);
+    else ovf[atomicAdd(&cnt[0], 1u)] = uint32_t(i);
+  }
 }
 __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
-                          int4* __restrict__ base, uint32_t* __restrict__ tcount) {
-  b_splines(blockIdx.x, gridDim.x, G, n, x, q, wq, base, tcount);
+                          int4* __restrict__ base, uint32_t* __restrict__ tcount, uint32_t* __restrict__ bucket, int cap,
+                          uint32_t* __restrict__ ovf, uint32_t* __restrict__ cnt) {
+  b_splines(blockIdx.x, gridDim.x, G, n, x, q, wq, base, tcount, bucket, cap, ovf, cnt);
 }
 
-// tstart: the tiles' first places, or with sums (the scan's block totals, nsums of them) the offsets within each scan
-// block of kScanTile tiles: then the block offsets are added here, and the tiles' first places written to tout
-// (0 … ntiles) for the kernels after this one
-__device__ __forceinline__ void b_tile_fill(int bx, int gx, int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
-                            uint32_t* __restrict__ tmp, const int64_t* __restrict__ sums = nullptr, int nsums = 0, int ntiles = 0,
-                            int64_t* __restrict__ tout = nullptr) {
+// The atoms of each tile in index order: an atom's place is the tile's first place (the scan of the counts) plus its
+// rank among the tile's atoms — those in its bucket (filled unordered) and, for a tile past its cap, those in the
+// overflow list; its spline offsets, charge and base point are copied to that place too, so the spread and the gather
+// read a tile's atoms contiguously. tstart: the tiles' first places, or with sums (the scan's block totals, nsums of
+// them) the offsets within each scan block of kScanTile tiles: then the block offsets are added here, and the tiles'
+// first places written to tout (0 … ntiles) for the kernels after this one. The tile counts (read by the scan, done)
+// are cleared here for the next evaluation; the overflow count, read here, by the spread that follows.
+__device__ __forceinline__ void b_tile_rank(int bx, int gx, int n, int ntiles, const float4* __restrict__ wq, const int4* __restrict__ base,
+                                            const int64_t* __restrict__ tstart, const uint32_t* __restrict__ bucket, int cap,
+                                            const uint32_t* __restrict__ ovf, uint32_t* __restrict__ cnt, uint32_t* __restrict__ tatoms,
+                                            float4* __restrict__ wsort, int4* __restrict__ bsort, uint32_t* __restrict__ tcount,
+                                            const int64_t* __restrict__ sums = nullptr, int nsums = 0, int64_t* __restrict__ tout = nullptr) {
   const int i = bx * blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  const int t = valid ? base[i].w : 0;
   __shared__ int64_t boff[64];
   if (sums) {
     block_offsets(sums, nsums, boff);
     for (int u = i; u <= ntiles; u += gx * blockDim.x) tout[u] = tstart[u] + boff[u / kScanTile];
   }
-  // one atomic per run of lanes in the same tile; the run's lanes take consecutive places from its result (the order
-  // within a tile does not matter here: k_tile_rank orders each tile by atom index)
-  int head, len;
-  const int pos = wave_run(valid, uint32_t(t), &head, &len);
-  uint32_t at = 0;
-  if (valid && pos == 0) at = atomicAdd(&tfill[t], uint32_t(len));
-  at = __shfl(at, head, 64);
-  if (valid) tmp[tstart[t] + (sums ? boff[t / kScanTile] : 0) + at + uint32_t(pos)] = uint32_t(i);
-}
-__global__ void k_tile_fill(int n, const int4* __restrict__ base, const int64_t* __restrict__ tstart, uint32_t* __restrict__ tfill,
-                            uint32_t* __restrict__ tmp, const int64_t* __restrict__ sums, int nsums, int ntiles, int64_t* __restrict__ tout) {
-  b_tile_fill(blockIdx.x, gridDim.x, n, base, tstart, tfill, tmp, sums, nsums, ntiles, tout);
-}
-
-// the atoms of each tile in index order: an atom's place is its rank among its tile's atoms (filled unordered in tmp);
-// its spline offsets, charge and base point are copied to that place too, so the spread and the gather read a tile's
-// atoms contiguously. The tile counters (read by the scan and by k_tile_fill, both done) are cleared here for the next
-// evaluation.
-__device__ __forceinline__ void b_tile_rank(int bx, int gx, int n, int ntiles, const float4* __restrict__ wq, const int4* __restrict__ base,
-                            const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tmp, uint32_t* __restrict__ tatoms,
-                            float4* __restrict__ wsort, int4* __restrict__ bsort, uint32_t* __restrict__ tcount,
-                            uint32_t* __restrict__ tfill) {
-  const int i = bx * blockDim.x + threadIdx.x;
-  for (int t = i; t < ntiles; t += gx * blockDim.x) tcount[t] = 0u, tfill[t] = 0u;
-  if (i >= n) return;
-  const int4 bi = base[i];
-  const float4 wi = wq[i];
-  const int64_t b = tstart[bi.w], e = tstart[bi.w + 1];
-  int64_t r = 0, k = b;
-  for (; k + 8 <= e; k += 8) {   // eight loads in flight
-    uint32_t t8[8];
+  for (int t = i; t < ntiles; t += gx * blockDim.x) tcount[t] = 0u;
+  const uint32_t novf = cnt[0];
+  if (i < n) {
+    const int4 bi = base[i];
+    const float4 wi = wq[i];
+    const int t = bi.w;
+    int64_t b = tstart[t], e = tstart[t + 1];
+    if (sums) b += boff[t / kScanTile], e += boff[(t + 1) / kScanTile];
+    const int64_t m = e - b;
+    const int nb = int(m < cap ? m : cap);
+    const uint32_t* bk = bucket + size_t(t) * cap;
+    int64_t r = 0;
+    int k = 0;
+    for (; k + 8 <= nb; k += 8) {   // eight loads in flight
+      uint32_t t8[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t8[u] = tmp[k + u];
+      for (int u = 0; u < 8; ++u) t8[u] = bk[k + u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) r += t8[u] < uint32_t(i) ? 1 : 0;
+      for (int u = 0; u < 8; ++u) r += t8[u] < uint32_t(i) ? 1 : 0;
+    }
+    for (; k < nb; ++k) r += bk[k] < uint32_t(i) ? 1 : 0;
+    if (m > cap)
+      for (uint32_t o = 0; o < novf; ++o) {
+        const uint32_t j = ovf[o];
+        r += (base[j].w == t && j < uint32_t(i)) ? 1 : 0;
+      }
+    tatoms[b + r] = uint32_t(i);
+    wsort[b + r] = wi;
+    bsort[b + r] = bi;
   }
-  for (; k < e; ++k) r += tmp[k] < uint32_t(i) ? 1 : 0;
-  tatoms[b + r] = uint32_t(i);
-  wsort[b + r] = wi;
-  bsort[b + r] = bi;
 }
 __global__ void k_tile_rank(int n, int ntiles, const float4* __restrict__ wq, const int4* __restrict__ base,
-                            const int64_t* __restrict__ tstart, const uint32_t* __restrict__ tmp, uint32_t* __restrict__ tatoms,
-                            float4* __restrict__ wsort, int4* __restrict__ bsort, uint32_t* __restrict__ tcount,
-                            uint32_t* __restrict__ tfill) {
-  b_tile_rank(blockIdx.x, gridDim.x, n, ntiles, wq, base, tstart, tmp, tatoms, wsort, bsort, tcount, tfill);
+                            const int64_t* __restrict__ tstart, const uint32_t* __restrict__ bucket, int cap, const uint32_t* __restrict__ ovf,
+                            uint32_t* __restrict__ cnt, uint32_t* __restrict__ tatoms, float4* __restrict__ wsort, int4* __restrict__ bsort,
+                            uint32_t* __restrict__ tcount, const int64_t* __restrict__ sums, int nsums, int64_t* __restrict__ tout) {
+  b_tile_rank(blockIdx.x, gridDim.x, n, ntiles, wq, base, tstart, bucket, cap, ovf, cnt, tatoms, wsort, bsort, tcount, sums, nsums, tout);
 }
 
 // one 64-lane workgroup per 4x4x4 tile. The atoms whose first point lies in a tile that can reach this one (the home
@@ -307,8 +314,12 @@ __device__ __forceinline__ void b_spread(int tile, GridArgs G, const int* __rest
   if (gx < G.k[0] && gy_ < G.k[1] && gz_ < G.k[2]) grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
 }
 template <int P>
+// (the spread launches also clear the bucket overflow count, ovfc, for the next evaluation: k_tile_rank, its last
+// reader, is done)
 __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict__ home, const int64_t* __restrict__ tstart,
-                                               const float4* __restrict__ wsort, const int4* __restrict__ bsort, float* __restrict__ grid) {
+                                               const float4* __restrict__ wsort, const int4* __restrict__ bsort, float* __restrict__ grid,
+                                               uint32_t* __restrict__ ovfc) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ovfc = 0u;
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
   const int tile = int(xcd_block(blockIdx.x, ntiles));   // neighbouring tiles on one XCD (shared atoms and mesh lines)
   if (tile >= ntiles) return;
@@ -535,7 +546,8 @@ __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const i
 }
 template <int P>
 __global__ void __launch_bounds__(kBlkThreads) k_spread_blk(GridArgs G, const int64_t* __restrict__ tstart, const float4* __restrict__ wsort,
-                                                            const int4* __restrict__ bsort, float* __restrict__ grid) {
+                                                            const int4* __restrict__ bsort, float* __restrict__ grid, uint32_t* __restrict__ ovfc) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ovfc = 0u;
   const int nblk = (G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2);
   b_spread_blk<P>(int(xcd_block(blockIdx.x, nblk)), G, tstart, wsort, bsort, grid);
 }
@@ -1832,9 +1844,12 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   P.theta.resize(4 * size_t(n));
   P.base.resize(4 * size_t(n));
   P.tcount.resize(size_t(ntiles));
-  P.tfill.resize(size_t(ntiles));
-  // the tile counters start at zero: cleared here after a new grid (or an evaluation that did not finish), else left
-  // clear by the previous evaluation's k_tile_rank
+  P.tfill.resize(2);
+  P.cap = tile_capacity(n, ntiles);
+  P.tile.resize(size_t(ntiles) * size_t(P.cap));
+  P.ovf.resize(size_t(n));
+  // the tile counts and the overflow count start at zero: cleared here after a new grid (or an evaluation that did not
+  // finish), else left clear by the previous evaluation's k_tile_rank
   if (!P.counts_clear) {
     P.tcount.zero(s);
     P.tfill.zero(s);
@@ -1843,28 +1858,24 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   float4* wq = reinterpret_cast<float4*>(P.theta.p);
   int4* base = reinterpret_cast<int4*>(P.base.p);
   kt_begin(c, "k_splines");
-  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p);
+  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p, P.tile.p,
+                     P.cap, P.ovf.p, P.tfill.p);
   LAUNCHCHECK();
   kt_end(c);
   P.tstart.resize(size_t(ntiles) + 1);
-  // the count scan: its blocks' offsets added by k_tile_fill when there are few blocks (one launch less), else whole
+  // the count scan: its blocks' offsets added by k_tile_rank when there are few blocks (one launch less), else whole
   P.tlocal.resize(size_t(ntiles) + 1);
   const int nsums = c.scan.blocks_only(P.tcount.p, P.tlocal.p, size_t(ntiles), s);
   if (!nsums) c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
-  P.tile.resize(size_t(n));
   P.tatoms.resize(size_t(n));
-  kt_begin(c, "k_tile_fill");
-  hipLaunchKernelGGL(k_tile_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, base, nsums ? P.tlocal.p : P.tstart.p, P.tfill.p, P.tile.p,
-                     nsums ? c.scan.tsum[0].p : nullptr, nsums, ntiles, P.tstart.p);
-  LAUNCHCHECK();
-  kt_end(c);
   P.wsort.resize(4 * size_t(n));
   P.bsort.resize(4 * size_t(n));
   float4* wsort = reinterpret_cast<float4*>(P.wsort.p);
   int4* bsort = reinterpret_cast<int4*>(P.bsort.p);
   kt_begin(c, "k_tile_rank");
-  hipLaunchKernelGGL(k_tile_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ntiles, wq, base, P.tstart.p, P.tile.p, P.tatoms.p, wsort,
-                     bsort, P.tcount.p, P.tfill.p);
+  hipLaunchKernelGGL(k_tile_rank, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ntiles, wq, base, nsums ? P.tlocal.p : P.tstart.p, P.tile.p,
+                     P.cap, P.ovf.p, P.tfill.p, P.tatoms.p, wsort, bsort, P.tcount.p, nsums ? c.scan.tsum[0].p : nullptr, nsums,
+                     nsums ? P.tstart.p : nullptr);
   LAUNCHCHECK();
   P.counts_clear = true;
   kt_end(c);
@@ -1874,8 +1885,11 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   switch (G.p) {
 #define CAPSGPU_SPREAD(Q)                                                                                                          \
   case Q:                                                                                                                          \
-    if (blocks) hipLaunchKernelGGL(k_spread_blk<Q>, dim3(nblk), dim3(kBlkThreads), 0, s, G, P.tstart.p, wsort, bsort, P.grid.p);  \
-    else hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), 0, s, G, P.home.p, P.tstart.p, wsort, bsort, P.grid.p); \
+    if (blocks)                                                                                                                    \
+      hipLaunchKernelGGL(k_spread_blk<Q>, dim3(nblk), dim3(kBlkThreads), 0, s, G, P.tstart.p, wsort, bsort, P.grid.p, P.tfill.p);   \
+    else                                                                                                                           \
+      hipLaunchKernelGGL(k_spread<Q>, dim3(unsigned(ntiles)), dim3(64), 0, s, G, P.home.p, P.tstart.p, wsort, bsort, P.grid.p,      \
+                         P.tfill.p);                                                                                               \
     break;
     CAPSGPU_SPREAD(3) CAPSGPU_SPREAD(4) CAPSGPU_SPREAD(5) CAPSGPU_SPREAD(6) CAPSGPU_SPREAD(7) CAPSGPU_SPREAD(8)
 #undef CAPSGPU_SPREAD
@@ -1902,7 +1916,8 @@ struct PmeRep {
   const double* q;
   float4* wq;
   int4* base;
-  uint32_t *tcount, *tfill, *tile, *tatoms;
+  uint32_t *tcount, *tfill, *tile, *tatoms, *ovf;   // tile: the buckets (cap slots per tile); tfill: the overflow counts
+  int cap;
   int64_t* tstart;
   float4* wsort;
   int4* bsort;
@@ -1930,7 +1945,7 @@ __device__ inline int find_rep(const int* __restrict__ off, int nr, int b) {
 __global__ void __launch_bounds__(256) kg_splines(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
   const int r = find_rep(off, nr, blockIdx.x);
   const PmeRep& a = R[r];
-  b_splines(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.n, a.x, a.q, a.wq, a.base, a.tcount);
+  b_splines(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.G, a.n, a.x, a.q, a.wq, a.base, a.tcount, a.tile, a.cap, a.ovf, a.tfill);
 }
 
 // one workgroup of 1024 threads per replica: its tile counts' exclusive sums (tstart[ntiles] = the total); thread t
@@ -1958,17 +1973,11 @@ __global__ void __launch_bounds__(1024) kg_tile_scan(const PmeRep* __restrict__ 
   }
 }
 
-__global__ void __launch_bounds__(256) kg_tile_fill(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
-  const int r = find_rep(off, nr, blockIdx.x);
-  const PmeRep& a = R[r];
-  b_tile_fill(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.base, a.tstart, a.tfill, a.tile);
-}
-
 __global__ void __launch_bounds__(256) kg_tile_rank(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
   const int r = find_rep(off, nr, blockIdx.x);
   const PmeRep& a = R[r];
-  b_tile_rank(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.ntiles, a.wq, a.base, a.tstart, a.tile, a.tatoms, a.wsort, a.bsort,
-              a.tcount, a.tfill);
+  b_tile_rank(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.ntiles, a.wq, a.base, a.tstart, a.tile, a.cap, a.ovf, a.tfill, a.tatoms,
+              a.wsort, a.bsort, a.tcount);
 }
 
 template <int P>
@@ -1976,6 +1985,7 @@ __global__ void __launch_bounds__(64) kg_spread(const PmeRep* __restrict__ R, co
   const int g = int(xcd_block(blockIdx.x, off[nr]));
   const int r = find_rep(off, nr, g);
   const PmeRep& a = R[r];
+  if (g == off[r] && threadIdx.x == 0) *a.tfill = 0u;   // the replica's overflow count, for its next evaluation
   b_spread<P>(g - off[r], a.G, a.home, a.tstart, a.wsort, a.bsort, a.grid);
 }
 
@@ -1984,6 +1994,7 @@ __global__ void __launch_bounds__(kBlkThreads) kg_spread_blk(const PmeRep* __res
   const int g = int(xcd_block(blockIdx.x, off[nr]));
   const int r = find_rep(off, nr, g);
   const PmeRep& a = R[r];
+  if (g == off[r] && threadIdx.x == 0) *a.tfill = 0u;   // the replica's overflow count, for its next evaluation
   b_spread_blk<P>(g - off[r], a.G, a.tstart, a.wsort, a.bsort, a.grid);
 }
 
@@ -2052,6 +2063,7 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
     a.x = c.x.p, a.q = c.d_charge.p;
     a.wq = reinterpret_cast<float4*>(P.theta.p), a.base = reinterpret_cast<int4*>(P.base.p);
     a.tcount = P.tcount.p, a.tfill = P.tfill.p, a.tile = P.tile.p, a.tatoms = P.tatoms.p, a.tstart = P.tstart.p;
+    a.ovf = P.ovf.p, a.cap = P.cap;
     a.wsort = reinterpret_cast<float4*>(P.wsort.p), a.bsort = reinterpret_cast<int4*>(P.bsort.p);
     a.home = P.home.p, a.grid = P.grid.p, a.half = P.half.p, a.tw = P.tw.p, a.gtab = P.gtab.p, a.bmod = P.bmod.p;
     a.fac = kPi * kPi / (P.beta * P.beta), a.vol = P.vol, a.fpme = P.fpme.p;
@@ -2086,8 +2098,8 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
 
 void pme_group_destroy(PmeGroup* g) { delete g; }
 
-// the reciprocal forces of every replica into its pme.fpme: nine launches for the whole group when every replica is on
-// the plane transform, eleven when none is
+// the reciprocal forces of every replica into its pme.fpme: eight launches for the whole group when every replica is on
+// the plane transform, ten when none is
 void pme_group_forces(PmeGroup& g, hipStream_t s) {
   const PmeRep* R = reinterpret_cast<const PmeRep*>(g.reps.p);
   const int nr = g.nr;
@@ -2095,8 +2107,6 @@ void pme_group_forces(PmeGroup& g, hipStream_t s) {
   hipLaunchKernelGGL(kg_splines, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
   LAUNCHCHECK();
   hipLaunchKernelGGL(kg_tile_scan, dim3(unsigned(nr)), dim3(1024), 0, s, R);
-  LAUNCHCHECK();
-  hipLaunchKernelGGL(kg_tile_fill, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
   LAUNCHCHECK();
   hipLaunchKernelGGL(kg_tile_rank, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
   LAUNCHCHECK();

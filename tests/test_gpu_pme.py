@@ -126,3 +126,32 @@ def test_gather_by_runs_gives_the_same_forces(case, order):
     assert np.array_equal(f2, f0)
     with pytest.raises(ValueError, match="shape"):
         ev.set_gather(2)
+
+
+@pytest.mark.parametrize("shape", [0, 1], ids=["gather by tiles", "gather by runs"])
+def test_a_tile_past_its_bucket_capacity(shape):
+    # 80 of 300 atoms inside one 4x4x4 tile of a 20^3 mesh (the tile buckets hold 32 atoms here): the atoms past the cap
+    # go through the overflow list; forces and energy as the oracle's, and the same on a second evaluation (the overflow
+    # count is cleared for it)
+    rng = np.random.default_rng(5)
+    n, L = 300, 20.0
+    x = rng.random((n, 3)) * L
+    x[:80] = 6.3 + rng.random((80, 3)) * 0.8
+    q = rng.normal(0.0, 0.3, n)
+    q -= q.mean()
+    ff = ca.ForceField(type=np.zeros(n, np.int32), charge=q, mass=np.full(n, 12.0), eps=np.full((1, 1), 0.1),
+                       sigma=np.full((1, 1), 3.0), excl_offset=np.zeros(n + 1, np.int32), excl_index=np.zeros(1, np.int32))
+    s = ca.System(x=x, cell=np.array([0, 0, 0, L, 0, 0, 0, L, 0, 0, 0, L], float), ff=ff)
+    print("case overflowing tile, gather shape", shape, flush=True)
+    op = ca.EnergyOptions(cutoff=9.0, coulomb=True, electrostatics="pme", ewald_rtol=1e-5, pme_order=5, pme_spacing=1.0, tail=False,
+                          parts=4)
+    ev = ca.Evaluator(s.ff, op)
+    ev.set_gather(shape)
+    t, f = ev.compute(s.x, s.cell)
+    e, fo, vo, grid = _oracle_recip(s, 9.0, 1e-5, 1.0, 5)
+    err = np.sqrt(((f - fo) ** 2).sum() / (fo ** 2).sum())
+    print("grid", grid, "E", t.coulomb, e, "force rel RMS", err, flush=True)
+    assert t.coulomb == pytest.approx(e, rel=1e-5)
+    assert err < 1e-5
+    t2, f2 = ev.compute(s.x, s.cell)
+    assert np.array_equal(f2, f) and t2.coulomb == t.coulomb
