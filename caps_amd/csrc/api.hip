@@ -866,6 +866,15 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
     auto quiet = [&](int64_t step, bool thermo, bool bstep) {
       return o->thermostat != 1 && !thermo && !bstep && !checkpoint_due(step) && step < o->steps;
     };
+    // the plain evaluation of a step runs PME's reciprocal part (evaluate's rule for the options' parts)
+    const bool recip_in_step = pme_on(*c) && (c->opt.parts & 6) && !(c->opt.parts & 8);
+    // a failure inside the loop leaves no half-done PME bookkeeping behind for the context's next evaluation
+    struct DropPresplines {
+      capsgpu_ctx* c;
+      ~DropPresplines() {
+        if (c->pme.presplined) pme_drop_presplines(*c);
+      }
+    } drop_presplines{c};
     for (int64_t step = 1; step <= o->steps; ++step) {
       const bool thermo = step % every == 0 || step == o->steps;
       if (ramp) {
@@ -913,10 +922,14 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* c, const capsgpu_dyn_opts* o, capsgpu_
         if (checkpoint_due(step)) checkpoint(step);
         continue;
       }
+      // the opening launch also does the per-atom half of the evaluation's PME bookkeeping on the positions it writes
+      // (the evaluation below is the plain one with PME's reciprocal part)
+      AtomSplines sp{};
+      const AtomSplines* spp = !LR && recip_in_step && pme_presplines(*c, sp) ? &sp : nullptr;
       if (o->thermostat == 2) {
-        baoab_open(*c, hk, 0.5 * dt, c1, c2_now, sdv.p, o->seed, nullptr, uint64_t(step + o->step_offset), lim_b, lim_p, bpart.p, pend);
+        baoab_open(*c, hk, 0.5 * dt, c1, c2_now, sdv.p, o->seed, nullptr, uint64_t(step + o->step_offset), lim_b, lim_p, bpart.p, pend, spp);
       } else {
-        vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p, nullptr, pend);
+        vv_open(*c, hk, dt, alpha_pending ? scal.p : nullptr, lim_b, lim_p, nullptr, pend, spp);
         alpha_pending = false;
       }
       pend = Kicks{};

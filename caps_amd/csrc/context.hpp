@@ -100,6 +100,7 @@ struct Pme {
   DevBuf<uint32_t> ovf, tfill;          // the atoms past their bucket's cap; tfill: their count and a workgroup counter
   int cap = 32;                         // slots per bucket
   bool counts_clear = false;            // tcount and tfill are zero on the device (k_tile_rank clears them)
+  bool presplined = false;              // the next evaluation's per-atom bookkeeping is done (by the integrator's launch)
   DevBuf<float> wsort;                  // theta in tatoms's order (float4 per atom)
   DevBuf<int32_t> bsort;                // base in tatoms's order (int4 per atom)
   DevBuf<int64_t> tstart;
@@ -214,6 +215,13 @@ void pme_grid_dims(const capsgpu_ctx& c, const double cell[12], int k[3]);
 void pme_forces(capsgpu_ctx& c, bool want);
 // picks the transforms' tile shape by timing (out: line pairs along a, lines along b and c per workgroup)
 void pme_tune(capsgpu_ctx& c, int out[2]);
+struct AtomSplines;
+// The per-atom half of the next evaluation's PME bookkeeping for a launch that moves the atoms to do on the positions it
+// writes (the mesh set up for the current cell, the arrays sized): false when the tile counters are not clear (a new
+// grid, an unfinished evaluation); else the next pme_forces leaves its splines launch out.
+bool pme_presplines(capsgpu_ctx& c, AtomSplines& out);
+// after a failure between such a launch and its evaluation: the next evaluation starts its bookkeeping afresh
+void pme_drop_presplines(capsgpu_ctx& c);
 // many contexts' reciprocal forces, one launch per kernel (each context evaluated once before; one spline order)
 struct PmeGroup;
 PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s);

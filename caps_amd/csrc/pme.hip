@@ -24,6 +24,7 @@
 #include <mutex>
 
 #include "context.hpp"
+#include "pme_atoms.hpp"
 #include "prof.hpp"
 
 namespace capsgpu {
@@ -67,62 +68,28 @@ struct GridArgs {
   double hi[3][3];
 };
 
-// fractional coordinate s = h⁻¹ x, u = (s − ⌊s⌋) K, the first point ⌊u⌋ − p + 1 (mod K) and w = u − ⌊u⌋
-// Runs of a wave's lanes: a valid lane whose key equals the previous lane's continues that lane's run. Returns the lane's
-// place in its run and its run's first lane; the first lane also gets the run's length. Consecutive atoms (neighbours
-// along a chain) mostly share a tile, so one atomic per run replaces one per atom. Every lane of the wave must call it.
-__device__ inline int wave_run(bool valid, uint32_t key, int* head, int* len) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t prev = __shfl_up(key, 1, 64);
-  const uint64_t vmask = __ballot(valid);
-  const bool prev_valid = lane > 0 && ((vmask >> (lane - 1)) & 1ull);
-  const bool first = valid && !(prev_valid && prev == key);
-  const uint64_t heads = __ballot(first);
-  const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;   // lanes 0 … lane
-  const uint64_t below = heads & upto;
-  *head = below ? 63 - __builtin_clzll(below) : lane;
-  const uint64_t ends = (heads | ~vmask) & ~upto;                       // run boundaries past this lane
-  *len = (ends ? __builtin_ffsll((long long)ends) - 1 : 64) - lane;
-  return lane - *head;
+// the arrays of the per-atom half of the bookkeeping for the mesh of G
+__device__ __host__ inline AtomSplines atom_splines(const GridArgs& G, const double* q, float4* wq, int4* base, uint32_t* tcount,
+                                                    uint32_t* bucket, int cap, uint32_t* ovf, uint32_t* cnt) {
+  AtomSplines A;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) A.hi[r][c] = G.hi[r][c];
+    A.k[r] = G.k[r], A.nt[r] = G.nt[r];
+  }
+  A.p = G.p, A.cap = cap, A.q = q, A.wq = wq, A.base = base, A.tcount = tcount, A.bucket = bucket, A.ovf = ovf, A.cnt = cnt;
+  return A;
 }
-
 // Per atom: its splines' offsets w, charge, first mesh point per axis and home tile (the 4x4x4 tile of that point), and
-// a place in the home tile's bucket of cap slots: one atomic per run of lanes in the same tile (consecutive atoms, chain
-// neighbours, mostly share one), the run's lanes taking consecutive slots from its result. An atom past its bucket's cap
-// goes to the overflow list (ovf, its length in cnt[0]): correct for any filling, slow only where a tile holds more
-// than cap atoms, which the cap's margin over the mean makes a rarity in a liquid or solid.
+// a place in the home tile's bucket of cap slots (spline_atom). An atom past its bucket's cap goes to the overflow list
+// (ovf, its length in cnt[0]): correct for any filling, slow only where a tile holds more than cap atoms, which the
+// cap's margin over the mean makes a rarity in a liquid or solid.
 __device__ __forceinline__ void b_splines(int bx, int gx, GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q,
                                           float4* __restrict__ wq, int4* __restrict__ base, uint32_t* __restrict__ tcount,
                                           uint32_t* __restrict__ bucket, int cap, uint32_t* __restrict__ ovf, uint32_t* __restrict__ cnt) {
   const int i = bx * blockDim.x + threadIdx.x;
   const bool valid = i < n;
-  int tile = 0;
-  if (valid) {
-    float w[3];
-    int b[3];
-    for (int d = 0; d < 3; ++d) {
-      const double s = G.hi[d][0] * x[3 * i] + G.hi[d][1] * x[3 * i + 1] + G.hi[d][2] * x[3 * i + 2];
-      const double u = (s - floor(s)) * G.k[d];
-      const double fl = floor(u);
-      w[d] = float(u - fl);
-      int t = (int(fl) - G.p + 1) % G.k[d];
-      if (t < 0) t += G.k[d];
-      b[d] = t;
-    }
-    tile = ((b[2] >> 2) * G.nt[1] + (b[1] >> 2)) * G.nt[0] + (b[0] >> 2);
-    wq[i] = make_float4(w[0], w[1], w[2], float(q[i]));
-    base[i] = make_int4(b[0], b[1], b[2], tile);
-  }
-  int head, len;
-  const int pos = wave_run(valid, uint32_t(tile), &head, &len);
-  uint32_t at = 0;
-  if (valid && pos == 0) at = atomicAdd(&tcount[tile], uint32_t(len));
-  at = __shfl(at, head, 64);
-  if (valid) {
-    const uint32_t slot = at + uint32_t(pos);
-    if (slot < uint32_t(cap)) bucket[size_t(tile) * cap + slot] = uint32_t(i);
-    else ovf[atomicAdd(&cnt[0], 1u)] = uint32_t(i);
-  }
+  const double xi[3] = {valid ? x[3 * i] : 0.0, valid ? x[3 * i + 1] : 0.0, valid ? x[3 * i + 2] : 0.0};
+  spline_atom(atom_splines(G, q, wq, base, tcount, bucket, cap, ovf, cnt), valid, i, xi);
 }
 __global__ void k_splines(GridArgs G, int n, const double* __restrict__ x, const double* __restrict__ q, float4* __restrict__ wq,
                           int4* __restrict__ base, uint32_t* __restrict__ tcount, uint32_t* __restrict__ bucket, int cap,
@@ -1658,6 +1625,7 @@ void pme_setup(capsgpu_ctx& c) {
   for (int q = 0; q < 12; ++q) P.cell[q] = cell[q];
   P.green_ok = true;
   P.counts_clear = false;   // a new grid: its tile counters are cleared before the next spread
+  P.presplined = false;
 }
 
 namespace {
@@ -1833,6 +1801,28 @@ void pme_tune(capsgpu_ctx& c, int out[2]) {
   out[0] = P.lines_x, out[1] = P.lines_yz;
 }
 
+bool pme_presplines(capsgpu_ctx& c, AtomSplines& out) {
+  Pme& P = c.pme;
+  pme_setup(c);
+  if (!P.counts_clear) return false;
+  const GridArgs G = grid_args(P);
+  const int n = c.n, ntiles = G.nt[0] * G.nt[1] * G.nt[2];
+  P.theta.resize(4 * size_t(n));
+  P.base.resize(4 * size_t(n));
+  P.tcount.resize(size_t(ntiles));
+  P.tfill.resize(2);
+  P.cap = tile_capacity(n, ntiles);
+  P.tile.resize(size_t(ntiles) * size_t(P.cap));
+  P.ovf.resize(size_t(n));
+  out = atom_splines(G, c.d_charge.p, reinterpret_cast<float4*>(P.theta.p), reinterpret_cast<int4*>(P.base.p), P.tcount.p, P.tile.p, P.cap,
+                     P.ovf.p, P.tfill.p);
+  P.counts_clear = false;
+  P.presplined = true;
+  return true;
+}
+
+void pme_drop_presplines(capsgpu_ctx& c) { c.pme.presplined = false, c.pme.counts_clear = false; }
+
 void pme_forces(capsgpu_ctx& c, bool want) {
   Pme& P = c.pme;
   pme_setup(c);
@@ -1843,25 +1833,30 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   const int ntiles = G.nt[0] * G.nt[1] * G.nt[2];
   P.theta.resize(4 * size_t(n));
   P.base.resize(4 * size_t(n));
-  P.tcount.resize(size_t(ntiles));
-  P.tfill.resize(2);
-  P.cap = tile_capacity(n, ntiles);
-  P.tile.resize(size_t(ntiles) * size_t(P.cap));
-  P.ovf.resize(size_t(n));
-  // the tile counts and the overflow count start at zero: cleared here after a new grid (or an evaluation that did not
-  // finish), else left clear by the previous evaluation's k_tile_rank
-  if (!P.counts_clear) {
-    P.tcount.zero(s);
-    P.tfill.zero(s);
-  }
-  P.counts_clear = false;
   float4* wq = reinterpret_cast<float4*>(P.theta.p);
   int4* base = reinterpret_cast<int4*>(P.base.p);
-  kt_begin(c, "k_splines");
-  hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p, P.tile.p,
-                     P.cap, P.ovf.p, P.tfill.p);
-  LAUNCHCHECK();
-  kt_end(c);
+  if (P.presplined) {
+    // the integrator's launch did the per-atom part on these positions (pme_presplines sized the arrays)
+    P.presplined = false;
+  } else {
+    P.tcount.resize(size_t(ntiles));
+    P.tfill.resize(2);
+    P.cap = tile_capacity(n, ntiles);
+    P.tile.resize(size_t(ntiles) * size_t(P.cap));
+    P.ovf.resize(size_t(n));
+    // the tile counts and the overflow count start at zero: cleared here after a new grid (or an evaluation that did
+    // not finish), else left clear by the previous evaluation's k_tile_rank and spread
+    if (!P.counts_clear) {
+      P.tcount.zero(s);
+      P.tfill.zero(s);
+    }
+    P.counts_clear = false;
+    kt_begin(c, "k_splines");
+    hipLaunchKernelGGL(k_splines, dim3(blocks_for(n, 256)), dim3(256), 0, s, G, n, c.x.p, c.d_charge.p, wq, base, P.tcount.p, P.tile.p,
+                       P.cap, P.ovf.p, P.tfill.p);
+    LAUNCHCHECK();
+    kt_end(c);
+  }
   P.tstart.resize(size_t(ntiles) + 1);
   // the count scan: its blocks' offsets added by k_tile_rank when there are few blocks (one launch less), else whole
   P.tlocal.resize(size_t(ntiles) + 1);

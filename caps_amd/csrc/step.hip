@@ -16,6 +16,7 @@
 #include <cmath>
 
 #include "context.hpp"
+#include "pme_atoms.hpp"
 #include "prof.hpp"
 #include "step.hpp"
 
@@ -442,13 +443,15 @@ __device__ __forceinline__ void kicked_v(const Kicks& K, const double* __restric
     }
 }
 
-// the put-off half kicks, then v ← α v (Bussi, the previous step) and v += h·kAcc/m F; x += dt v
+// the put-off half kicks, then v ← α v (Bussi, the previous step) and v += h·kAcc/m F; x += dt v; with sp.wq set, the
+// per-atom half of the next PME evaluation's bookkeeping on the new positions (its splines launch is then left out)
 __global__ void __launch_bounds__(256) k_vv_open(int n, double hk, double dt, const double* __restrict__ minv, const double* __restrict__ alpha,
                                                  const double* __restrict__ f, double* __restrict__ x, double* __restrict__ v,
                                                  const double* __restrict__ x0, const double* __restrict__ x0p, double lim_b, double lim_p,
-                                                 uint32_t* __restrict__ flags, Kicks pre) {
+                                                 uint32_t* __restrict__ flags, Kicks pre, AtomSplines sp) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double xn[3] = {0, 0, 0};
   if (i < n) {
     const double al = alpha ? *alpha : 1.0, a = hk * minv[i];
     double w[3];
@@ -456,10 +459,12 @@ __global__ void __launch_bounds__(256) k_vv_open(int n, double hk, double dt, co
     for (int m = 0; m < 3; ++m) {
       const double vv = fma(a, f[3 * i + m], al * w[m]);
       v[3 * i + m] = vv;
-      x[3 * i + m] = fma(dt, vv, x[3 * i + m]);
+      xn[m] = fma(dt, vv, x[3 * i + m]);
+      x[3 * i + m] = xn[m];
     }
   }
   move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
+  if (sp.wq) spline_atom(sp, i < n, i, xn);
 }
 
 // the put-off half kicks, then B (half kick), A (half drift), O, A; partials: kinetic energy before and after O (the
@@ -469,11 +474,11 @@ __device__ __forceinline__ void b_baoab_open(int bx, int gx, int n, double hk, d
                                                     uint64_t step_add, const double* __restrict__ f,
                                                     double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
                                                     const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
-                                                    double* __restrict__ part, const Kicks& pre) {
+                                                    double* __restrict__ part, const Kicks& pre, const AtomSplines* sp = nullptr) {
 #pragma clang fp contract(off)
   const int i = bx * blockDim.x + threadIdx.x;
   const uint64_t step = (stepc ? *stepc : 0) + step_add;
-  double ke[1] = {0};
+  double ke[1] = {0}, xn[3] = {0, 0, 0};
   if (i < n) {
     const double a = hk * minv[i], mi = 1.0 / minv[i], cs = c2 * sdv[i];
     double g[3], w[3];
@@ -489,10 +494,12 @@ __device__ __forceinline__ void b_baoab_open(int bx, int gx, int n, double hk, d
       xx = fma(hdt, vv, xx);
       v[3 * i + m] = vv;
       x[3 * i + m] = xx;
+      xn[m] = xx;
     }
     ke[0] = 0.5 * mi * (k1 - k0) / kAcc;
   }
   move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
+  if (sp && sp->wq) spline_atom(*sp, i < n, i, xn);
   block_sum_store<1, true>(ke, part, bx);
 }
 __global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
@@ -500,8 +507,9 @@ __global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt
                                                     uint64_t step_add, const double* __restrict__ f,
                                                     double* __restrict__ x, double* __restrict__ v, const double* __restrict__ x0,
                                                     const double* __restrict__ x0p, double lim_b, double lim_p, uint32_t* __restrict__ flags,
-                                                    double* __restrict__ part, Kicks pre) {
-  b_baoab_open(blockIdx.x, gridDim.x, n, hk, hdt, c1, c2, minv, sdv, seed, stepc, step_add, f, x, v, x0, x0p, lim_b, lim_p, flags, part, pre);
+                                                    double* __restrict__ part, Kicks pre, AtomSplines sp) {
+  b_baoab_open(blockIdx.x, gridDim.x, n, hk, hdt, c1, c2, minv, sdv, seed, stepc, step_add, f, x, v, x0, x0p, lim_b, lim_p, flags, part, pre,
+               &sp);
 }
 
 // the put-off half kicks and the closing half kick; kinetic energy partials (Σ ½ m v², and the tensor m v_a v_b) when
@@ -805,19 +813,22 @@ void check_moves(capsgpu_ctx& c, double lim_build, double lim_prune) {
   kt_end(c);
 }
 
-void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p, const double* f, const Kicks& pre) {
+void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p, const double* f, const Kicks& pre,
+             const AtomSplines* sp) {
+  AtomSplines none{};
   kt_begin(c, "k_vv_open");
   hipLaunchKernelGGL(k_vv_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, dt, c.d_minv.p, alpha, f ? f : c.f.p, c.x.p,
-                     c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, pre);
+                     c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, pre, sp ? *sp : none);
   LAUNCHCHECK();
   kt_end(c);
 }
 
 void baoab_open(capsgpu_ctx& c, double hk, double hdt, double c1, double c2, const double* sdv, uint64_t seed, const uint64_t* stepc,
-                uint64_t step_add, double lim_b, double lim_p, double* part, const Kicks& pre) {
+                uint64_t step_add, double lim_b, double lim_p, double* part, const Kicks& pre, const AtomSplines* sp) {
+  AtomSplines none{};
   kt_begin(c, "k_baoab_open");
   hipLaunchKernelGGL(k_baoab_open, dim3(blocks_for(c.n, 256)), dim3(256), 0, c.stream, c.n, hk, hdt, c1, c2, c.d_minv.p, sdv, seed, stepc,
-                     step_add, c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part, pre);
+                     step_add, c.f.p, c.x.p, c.v.p, c.x0.p, c.x0p.p, lim_b, lim_p, c.flags.p, part, pre, sp ? *sp : none);
   LAUNCHCHECK();
   kt_end(c);
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "context.hpp"
+#include "pme_atoms.hpp"
 
 namespace capsgpu {
 // Half kicks put off from the end of a step to the next launch that moves the atoms (the forces they use are unchanged
@@ -19,13 +20,14 @@ struct Kicks {
 };
 void slot_virial(capsgpu_ctx& c, double* part);
 void colsum(capsgpu_ctx& c, const double* part, int64_t rows, int cols, double* out);
-// f: the force to kick with (nullptr: the context's c.f)
+// f: the force to kick with (nullptr: the context's c.f); sp: the next PME evaluation's per-atom bookkeeping, done on the
+// new positions here (from pme_presplines), or nullptr
 void vv_open(capsgpu_ctx& c, double hk, double dt, const double* alpha, double lim_b, double lim_p, const double* f = nullptr,
-             const Kicks& pre = Kicks{});
+             const Kicks& pre = Kicks{}, const AtomSplines* sp = nullptr);
 // the Langevin step's noise is drawn for step (stepc ? *stepc : 0) + step_add; part: per-workgroup rows the bath's heat
 // is added to (bath_add takes them)
 void baoab_open(capsgpu_ctx& c, double hk, double hdt, double c1, double c2, const double* sdv, uint64_t seed, const uint64_t* stepc,
-                uint64_t step_add, double lim_b, double lim_p, double* part, const Kicks& pre = Kicks{});
+                uint64_t step_add, double lim_b, double lim_p, double* part, const Kicks& pre = Kicks{}, const AtomSplines* sp = nullptr);
 // *stepc += by on the stream
 void step_advance(capsgpu_ctx& c, uint64_t* stepc, uint64_t by);
 void kick(capsgpu_ctx& c, double hk, bool ke, double* part, const double* f = nullptr, const Kicks& pre = Kicks{});
