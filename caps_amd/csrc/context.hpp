@@ -112,7 +112,7 @@ struct Pme {
   int lines_x = 0, lines_yz = 16;       // the transforms' tile shape (line pairs along a, lines along b and c); lines_x 0:
                                         // a c-plane per workgroup for a and b together when it fits in LDS (else 8 pairs)
   bool plane_ok = false;                // the grid's c-planes fit the plane transform (even K2, LDS)
-  int gather_shape = 0;                 // the gather's workgroups: 0 one per tile, 1 one per run of 64 sorted atoms (K1 a
+  int gather_shape = 1;                 // the gather's workgroups: 0 one per tile, 1 one per run of 64 sorted atoms (K1 a
                                         // multiple of 4; else per tile); the same forces either way
 };
 

@@ -1771,11 +1771,15 @@ void pme_tune(capsgpu_ctx& c, int out[2]) {
     // the gather's workgroups (one per tile, or one per run of 64 sorted atoms where the mesh rows allow it), each timed
     // alone on the last evaluation's sorted atoms and mesh (before the transforms below overwrite the mesh); the same
     // forces either way
+    // (the runs are kept unless the tiles are clearly faster: three launches a sample, the best of three samples)
     const GridArgs G0 = grid_args(P);
     if (G0.k[0] % 4 == 0 && P.fpme.n >= 3 * size_t(c.n)) {
       float tg[2];
-      for (int shape = 0; shape < 2; ++shape) tg[shape] = best_of([&] { launch_gather(c, G0, shape == 1); });
-      P.gather_shape = tg[1] < tg[0] ? 1 : 0;
+      for (int shape = 0; shape < 2; ++shape)
+        tg[shape] = best_of([&] {
+          for (int rep = 0; rep < 3; ++rep) launch_gather(c, G0, shape == 1);
+        });
+      P.gather_shape = tg[0] < 0.95f * tg[1] ? 0 : 1;
     }
     // every pair of shapes over the whole transform (the b pass belongs to the plane kernel or to the line tiles, so the
     // two choices are not independent)

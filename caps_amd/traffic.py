@@ -1,7 +1,8 @@
 """HBM traffic of one kernel per launch, from rocprofv3's FETCH_SIZE and WRITE_SIZE counters (KiB per dispatch),
 calibrated against a plain device-to-device copy of known size in the same run.
 
-    # the program the counters are taken on: a 256 MiB copy, then plain PME evaluations of the 262,144-atom melt
+    # the program the counters are taken on: a 256 MiB copy, then plain PME evaluations of the 262,144-atom melt with the
+    # kernel shapes the tuner picks
     rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o t -- python3 -m caps_amd.traffic run
     rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o t -- python3 -m caps_amd.traffic run
     # the per-launch bytes of a kernel, written as JSON (and printed)
@@ -28,6 +29,8 @@ def run():
     op = ca.EnergyOptions(cutoff=12.0, skin=2.0, coulomb=True, electrostatics="pme", ewald_rtol=1e-5, pme_order=5, pme_spacing=1.0,
                           tail=False, parts=5)
     ev = ca.Evaluator(s.ff, op)
+    ev.compute(s.x, s.cell)
+    ev.tune()   # the kernel shapes a run takes: the gather's workgroups and the transforms' tiles, picked by timing
     for _ in range(5):
         ev.compute(s.x, s.cell)
     torch.cuda.synchronize()

@@ -216,7 +216,7 @@ int32_t capsgpu_run_dynamics(capsgpu_ctx* ctx, const capsgpu_dyn_opts* opts, cap
  * The mesh transforms run a tile of lines per workgroup: line pairs along a (tiles[0]) and lines along b and c
  * (tiles[1]), powers of two; tiles[0] = 0 (the default) runs the passes along a and b together, a whole c-plane per
  * workgroup in LDS, when the grid's planes fit (K2 even, the device's LDS), else 8 line pairs. The force gather runs a
- * workgroup per 4x4x4 tile (shape 0, the default) or per run of 64 atoms of the tile order, one atom per lane (shape 1,
+ * workgroup per 4x4x4 tile (shape 0) or per run of 64 atoms of the tile order, one atom per lane (shape 1, the default,
  * where K1 is a multiple of 4; per tile otherwise). capsgpu_tune picks both for this context by timing each candidate on its own mesh (after a
  * reciprocal-space evaluation run here; needs PME and a set state) and keeps the fastest; capsgpu_set_tiles and
  * capsgpu_set_gather set them, capsgpu_get_gather reads the gather's. Every shape gives bit-identical forces. tiles,
