@@ -84,8 +84,8 @@ void drop_groups(capsgpu_batch& b) {
 
 // one Langevin step of every replica as grouped launches on stream s (the noise for step *stepc + k)
 void grouped_step(capsgpu_batch& b, hipStream_t s, uint64_t k) {
-  step_group_open(*b.sg, s, k);
-  pme_group_forces(*b.pg, s);
+  step_group_open(*b.sg, s, k);   // with the per-atom half of the PME bookkeeping on the new positions
+  pme_group_forces(*b.pg, s, true);
   step_group_close(*b.sg, s);
 }
 

@@ -225,7 +225,10 @@ void pme_drop_presplines(capsgpu_ctx& c);
 // many contexts' reciprocal forces, one launch per kernel (each context evaluated once before; one spline order)
 struct PmeGroup;
 PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s);
-void pme_group_forces(PmeGroup& g, hipStream_t s);
+// splined: the step group's opening launch did the per-atom part of the bookkeeping (pme_atom_splines' arrays)
+void pme_group_forces(PmeGroup& g, hipStream_t s, bool splined = false);
+// the per-atom part's arguments for the context's current mesh and arrays (as the last evaluation sized them)
+AtomSplines pme_atom_splines(capsgpu_ctx& c);
 void pme_group_destroy(PmeGroup* g);
 // bonded.hip
 void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff);

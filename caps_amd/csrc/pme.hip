@@ -2099,12 +2099,20 @@ void pme_group_destroy(PmeGroup* g) { delete g; }
 
 // the reciprocal forces of every replica into its pme.fpme: eight launches for the whole group when every replica is on
 // the plane transform, ten when none is
-void pme_group_forces(PmeGroup& g, hipStream_t s) {
+AtomSplines pme_atom_splines(capsgpu_ctx& c) {
+  Pme& P = c.pme;
+  return atom_splines(grid_args(P), c.d_charge.p, reinterpret_cast<float4*>(P.theta.p), reinterpret_cast<int4*>(P.base.p), P.tcount.p,
+                      P.tile.p, P.cap, P.ovf.p, P.tfill.p);
+}
+
+void pme_group_forces(PmeGroup& g, hipStream_t s, bool splined) {
   const PmeRep* R = reinterpret_cast<const PmeRep*>(g.reps.p);
   const int nr = g.nr;
   const int na = g.h_atoms, nt = g.h_tiles, nx = g.h_x, ny = g.h_yz, nz = g.h_z;
-  hipLaunchKernelGGL(kg_splines, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
-  LAUNCHCHECK();
+  if (!splined) {
+    hipLaunchKernelGGL(kg_splines, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);
+    LAUNCHCHECK();
+  }
   hipLaunchKernelGGL(kg_tile_scan, dim3(unsigned(nr)), dim3(1024), 0, s, R);
   LAUNCHCHECK();
   hipLaunchKernelGGL(kg_tile_rank, dim3(unsigned(na)), dim3(256), 0, s, R, g.off_atoms.p, nr);

@@ -905,6 +905,7 @@ struct StepRep {
   const int32_t *rstart, *roff;
   const uint32_t* rterm;
   int bonded;
+  AtomSplines sp;   // the replica's per-atom PME bookkeeping, done by the opening launch (sp.wq null: none)
 };
 
 __device__ inline int find_rep(const int* __restrict__ off, int nr, int b) {
@@ -921,7 +922,7 @@ __global__ void __launch_bounds__(256) kg_baoab_open(const StepRep* __restrict__
   const int r = find_rep(off, nr, blockIdx.x);
   const StepRep& a = R[r];
   b_baoab_open(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, a.hk, a.hdt, a.c1, a.c2, a.minv, a.sdv, a.seed, a.stepc, step_add, a.f,
-               a.x, a.v, a.x0, a.x0p, -1.0, -1.0, a.flags, a.bpart, Kicks{});
+               a.x, a.v, a.x0, a.x0p, -1.0, -1.0, a.flags, a.bpart, Kicks{}, &a.sp);
 }
 
 __global__ void __launch_bounds__(256) kg_bonded(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
@@ -989,6 +990,8 @@ StepGroup* step_group_create(const std::vector<StepGroupInput>& in, hipStream_t 
     a.bkr = B.bkr.p, a.akt = B.akt.p, a.fp = B.fp.p, a.slot = B.slot.p;
     a.frec = in[i].recip ? c.pme.fpme.p : nullptr;
     a.rstart = B.rstart.p, a.roff = B.roff.p, a.rterm = B.rterm.p, a.bonded = in[i].bonded ? 1 : 0;
+    a.sp = AtomSplines{};
+    if (in[i].recip) a.sp = pme_atom_splines(c);
     g->ranged = g->ranged && B.ranged;
     orr.push_back(orr.back() + B.nranges);
     oa.push_back(oa.back() + int(blocks_for(size_t(c.n), 256)));
