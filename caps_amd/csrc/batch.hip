@@ -10,6 +10,7 @@
 // last node advances it by S), so every replay draws fresh noise. The launch shape (B, S), or direct launches without a
 // graph, is picked by timing each candidate for a few steps from a snapshot of the state that is restored afterwards.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <memory>
@@ -255,8 +256,23 @@ double seconds_since(std::chrono::steady_clock::time_point t0) {
 }
 
 // the fastest shape by timing: direct launches and graphs of B ∈ {1, 2, 4, …, 32} streams × S ∈ {4, 16, 64} steps (at
-// most 1,024 replica steps in one graph), each run for at least 16 steps after one untimed chunk (which also captures),
-// from the snapshot
+// most 1,024 replica steps in one graph), each timed twice over at least 32 steps after one untimed chunk (which also
+// captures), the better sample kept, from the snapshot
+// per step, the better of two samples over at least 32 steps after one untimed chunk (which also captures); the caller
+// saves the state before and restores it after
+double time_shape(capsgpu_batch& b, const Shape& sh) {
+  const int64_t chunk = sh.mode == 2 ? 1 : sh.steps, timed = std::max<int64_t>(32, chunk);
+  if (sh.mode == 1 || sh.mode == 3) capture(b, sh);
+  advance(b, sh, chunk);
+  double per = 1e300;
+  for (int rep = 0; rep < 2; ++rep) {   // one slow replay does not decide the shape
+    const auto t0 = std::chrono::steady_clock::now();
+    advance(b, sh, timed);
+    per = std::min(per, seconds_since(t0) / double(timed));
+  }
+  return per;
+}
+
 Shape tune(capsgpu_batch& b) {
   std::vector<Shape> cand;
   cand.push_back({2, 0, 1});
@@ -271,12 +287,7 @@ Shape tune(capsgpu_batch& b) {
   Shape best;
   double best_t = 1e300;
   for (const Shape& sh : cand) {
-    const int64_t chunk = sh.mode == 2 ? 1 : sh.steps, timed = std::max<int64_t>(16, chunk);
-    if (sh.mode == 1 || sh.mode == 3) capture(b, sh);
-    advance(b, sh, chunk);
-    const auto t0 = std::chrono::steady_clock::now();
-    advance(b, sh, timed);
-    const double per = seconds_since(t0) / double(timed);
+    const double per = time_shape(b, sh);
     snapshot(b, true);
     if (per < best_t) best_t = per, best = sh;
   }
@@ -367,8 +378,19 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
   // which the next evaluation overwrites), before the group tables take them and the launch shape is timed
   const bool retune = o.mode == 0 && !(b.have_tuned && b.tuned_key == key);
   double tile_s = 0;
+  // each replica's kernel shapes as they were set (the transforms' tiles, the gather's workgroups), kept to compare
+  std::vector<std::array<int, 3>> set_shapes;
+  auto swap_shapes = [&] {
+    for (size_t i = 0; i < b.r.size(); ++i) {
+      Pme& P = b.r[i].c->pme;
+      std::array<int, 3> cur{P.lines_x, P.lines_yz, P.gather_shape};
+      P.lines_x = set_shapes[i][0], P.lines_yz = set_shapes[i][1], P.gather_shape = set_shapes[i][2];
+      set_shapes[i] = cur;
+    }
+  };
   if (retune) {
     const auto t0 = std::chrono::steady_clock::now();
+    for (Rep& R : b.r) set_shapes.push_back({R.c->pme.lines_x, R.c->pme.lines_yz, R.c->pme.gather_shape});
     for (Rep& R : b.r)
       if (pme_on_ctx(*R.c)) {
         int t[2];
@@ -378,8 +400,9 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
     tile_s = seconds_since(t0);
   }
   // grouped launches when every replica has PME's reciprocal part with one spline order
-  drop_groups(b);
-  {
+  auto build_groups = [&] {
+    drop_graphs(b);
+    drop_groups(b);
     bool ok = true;
     for (Rep& R : b.r) ok = ok && pme_on_ctx(*R.c) && R.c->pme.counts_clear && R.c->pme.order == b.r[0].c->pme.order;
     if (ok) {
@@ -397,7 +420,8 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
     } else if (o.mode == 3) {
       throw Error(CAPSGPU_EINVAL, "grouped launches need PME's reciprocal part with one spline order in every replica");
     }
-  }
+  };
+  build_groups();
   Shape sh;
   double tune_s = 0;
   if (o.mode == 2) {
@@ -411,6 +435,21 @@ void batch_run(capsgpu_batch& b, const capsgpu_batch_opts& o, const double* temp
   } else {
     const auto t0 = std::chrono::steady_clock::now();
     sh = tune(b);
+    if (sh.mode == 3) {
+      // the replicas' shapes were picked each alone; grouped launches may run faster with the shapes they had: the
+      // grouped shape is timed with both and the faster kept
+      snapshot(b, false);
+      const double t_alone = time_shape(b, sh);
+      snapshot(b, true);
+      swap_shapes();
+      build_groups();
+      const double t_set = time_shape(b, sh);
+      snapshot(b, true);
+      if (t_alone < t_set) {
+        swap_shapes();
+        build_groups();
+      }
+    }
     tune_s = seconds_since(t0) + tile_s;
     b.have_tuned = true, b.tuned = sh, b.tuned_key = key;
   }

@@ -2004,6 +2004,13 @@ __global__ void __launch_bounds__(64) kg_gather(const PmeRep* __restrict__ R, co
   const PmeRep& a = R[r];
   b_gather<P>(g - off[r], a.G, a.tstart, a.tatoms, a.wsort, a.bsort, a.grid, a.fpme);
 }
+template <int P>
+__global__ void __launch_bounds__(64) kg_gather_seq(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr) {
+  const int g = int(xcd_block(blockIdx.x, off[nr]));
+  const int r = find_rep(off, nr, g);
+  const PmeRep& a = R[r];
+  b_gather_seq<P>(g - off[r], a.n, a.G, a.tatoms, a.wsort, a.bsort, a.grid, a.fpme);
+}
 
 __global__ void __launch_bounds__(256) kg_fft_x(const PmeRep* __restrict__ R, const int* __restrict__ off, int nr, int forward) {
   const int r = find_rep(off, nr, blockIdx.x);
@@ -2036,8 +2043,8 @@ __global__ void __launch_bounds__(256) kg_fft_z_conv(const PmeRep* __restrict__ 
 struct PmeGroup {
   int nr = 0, order = 0;
   DevBuf<unsigned char> reps;                         // PmeRep[nr]
-  DevBuf<int32_t> off_atoms, off_tiles, off_x, off_yz, off_z, off_xy, off_sp, off_blk;   // workgroup prefixes per launch
-  int h_atoms = 0, h_tiles = 0, h_x = 0, h_yz = 0, h_z = 0, h_xy = 0, h_sp = 0, h_blk = 0;  // the launches' workgroups
+  DevBuf<int32_t> off_atoms, off_tiles, off_x, off_yz, off_z, off_xy, off_sp, off_blk, off_gt, off_gs;   // workgroup prefixes per launch
+  int h_atoms = 0, h_tiles = 0, h_x = 0, h_yz = 0, h_z = 0, h_xy = 0, h_sp = 0, h_blk = 0, h_gt = 0, h_gs = 0;  // the launches' workgroups
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_xy = 0;
 };
 
@@ -2046,7 +2053,7 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
   g->nr = int(cs.size());
   if (g->nr < 1) throw Error(-1, "a PME group needs a replica");
   std::vector<PmeRep> R(cs.size());
-  std::vector<int32_t> oa(1, 0), ot(1, 0), ox(1, 0), oy(1, 0), oz(1, 0), oxy(1, 0), osp(1, 0), oblk(1, 0);
+  std::vector<int32_t> oa(1, 0), ot(1, 0), ox(1, 0), oy(1, 0), oz(1, 0), oxy(1, 0), osp(1, 0), oblk(1, 0), ogt(1, 0), ogs(1, 0);
   for (size_t i = 0; i < cs.size(); ++i) {
     capsgpu_ctx& c = *cs[i];
     Pme& P = c.pme;
@@ -2070,6 +2077,10 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
     const bool blocks = spread_blocks(G);
     osp.push_back(osp.back() + (blocks ? 0 : a.ntiles));
     oblk.push_back(oblk.back() + (blocks ? (G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2) : 0));
+    // the gather each replica takes alone: by runs of 64 sorted atoms or by tiles (the same forces)
+    const bool runs = gather_by_runs(P, G);
+    ogs.push_back(ogs.back() + (runs ? (c.n + 63) / 64 : 0));
+    ogt.push_back(ogt.back() + (runs ? 0 : a.ntiles));
     // a replica on the plane transform (G.lx = 0) has workgroups in the plane launches only, the others in the a- and
     // b-line launches only
     const int npairs = (P.k[1] * P.k[2] + 1) / 2, mblocks = (P.h1 + G.ly - 1) / G.ly;
@@ -2089,6 +2100,8 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
   g->off_atoms.upload(oa, s), g->off_tiles.upload(ot, s), g->off_x.upload(ox, s), g->off_yz.upload(oy, s), g->off_z.upload(oz, s);
   g->off_xy.upload(oxy, s), g->off_sp.upload(osp, s), g->off_blk.upload(oblk, s);
   g->h_sp = osp.back(), g->h_blk = oblk.back();
+  g->off_gt.upload(ogt, s), g->off_gs.upload(ogs, s);
+  g->h_gt = ogt.back(), g->h_gs = ogs.back();
   if (g->lds_xy > 65536)
     HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&kg_fft_xy), hipFuncAttributeMaxDynamicSharedMemorySize, int(g->lds_xy)));
   g->h_atoms = oa.back(), g->h_tiles = ot.back(), g->h_x = ox.back(), g->h_yz = oy.back(), g->h_z = oz.back(), g->h_xy = oxy.back();
@@ -2155,8 +2168,11 @@ void pme_group_forces(PmeGroup& g, hipStream_t s, bool splined) {
     LAUNCHCHECK();
   }
   switch (g.order) {
-#define CAPSGPU_GGATHER(Q) \
-  case Q: hipLaunchKernelGGL(kg_gather<Q>, dim3(unsigned(nt)), dim3(64), lds_gather, s, R, g.off_tiles.p, nr); break;
+#define CAPSGPU_GGATHER(Q)                                                                                                      \
+  case Q:                                                                                                                       \
+    if (g.h_gs) hipLaunchKernelGGL(kg_gather_seq<Q>, dim3(unsigned(g.h_gs)), dim3(64), gather_seq_lds(Q), s, R, g.off_gs.p, nr);  \
+    if (g.h_gt) hipLaunchKernelGGL(kg_gather<Q>, dim3(unsigned(g.h_gt)), dim3(64), lds_gather, s, R, g.off_gt.p, nr);              \
+    break;
     CAPSGPU_GGATHER(3) CAPSGPU_GGATHER(4) CAPSGPU_GGATHER(5) CAPSGPU_GGATHER(6) CAPSGPU_GGATHER(7) CAPSGPU_GGATHER(8)
 #undef CAPSGPU_GGATHER
   }
