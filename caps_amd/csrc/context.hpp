@@ -138,7 +138,8 @@ struct Bonded {
   bool ranged = false;
   int nranges = 0;
   DevBuf<int32_t> rstart, roff;         // range r: atoms rstart[r] … rstart[r+1] − 1; its terms rterm[roff[r]] …
-  DevBuf<uint32_t> rterm;               // term (bonds, angles, FENE bonds in one numbering); top bit: its energy counted here
+  DevBuf<int4> rterm;                   // term (bonds, angles, FENE bonds in one numbering; top bit: its energy counted
+                                        // here) and its member atoms
 };
 
 

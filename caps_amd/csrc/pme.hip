@@ -645,6 +645,17 @@ __device__ inline void dft(float2* v, int sign) {
 // fall in different banks (with plain n-apart lines of a power of two, sixteen lanes would hit one bank).
 __host__ __device__ inline int lds_at(int j) { return j + (j >> 4); }
 __host__ __device__ inline int line_stride(int n) { return n + (n >> 4) + 1; }
+// The c-line pass's blocks of LY lines along a (of h1): a tail of at most LY / 4 lines joins the last block instead of
+// taking a workgroup of its own (65 = 4 x 16 + 1 at K1 = 128) where the larger block's LDS stays within 64 KiB, so a
+// block holds at most zline_cap() lines.
+__host__ __device__ inline int zline_cap(int LY, int K3) {
+  const int m = LY + LY / 4;
+  return (2 * m * line_stride(K3) + K3) * int(sizeof(float2)) <= 65536 ? m : LY;
+}
+__host__ __device__ inline int zline_blocks(int h1, int LY, int K3) {
+  const int full = h1 / LY, rem = h1 - full * LY;
+  return full > 0 && rem > 0 && LY + rem <= zline_cap(LY, K3) ? full : full + (rem > 0);
+}
 
 // one Stockham stage of radix R over nl lines of length n (ls apart): butterfly i of a line takes
 // x_q = src[i + q n/R] × w^(q k), w = exp(sign 2πi / (pR)), k = i mod p, and writes its DFT to dst[(i − k) R + k + q p];
@@ -1117,12 +1128,14 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
   extern __shared__ float2 lds[];
   const int h1 = G.h1, K2 = G.k[1], K3 = G.k[2];
   const int LY = G.ly, lg = G.ly_log2;
-  const int mblocks = (h1 + LY - 1) >> lg;
-  const int jy = bx / mblocks, m0 = (bx % mblocks) << lg;
-  const int nl = min(LY, h1 - m0), ls = line_stride(K3);
+  const int mblocks = zline_blocks(h1, LY, K3);
+  const int jy = bx / mblocks, mb = bx % mblocks, m0 = mb << lg;
+  // the block's lines: LY, the last block's the rest (a short tail joins it); those past LY go through side loops
+  const int nl = mb == mblocks - 1 ? h1 - m0 : LY, nmain = min(nl, LY), nx = nl - nmain;
+  const int ls = line_stride(K3);
   float2* A = lds;
-  float2* B = lds + LY * ls;
-  float2* twl = B + LY * ls;
+  float2* B = lds + zline_cap(LY, K3) * ls;
+  float2* twl = B + zline_cap(LY, K3) * ls;
   const TwFetch twf = fetch_twiddles(tw, FP.ax[2]);
   const size_t zs = size_t(h1) * K2;
   for (int t0 = 0; t0 < LY * K3; t0 += kFftBatch * kFftThreads) {
@@ -1130,13 +1143,17 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-      r[u] = t < LY * K3 && li < nl ? half[size_t(m0 + li) + size_t(h1) * jy + zs * k] : make_float2(0.0f, 0.0f);
+      r[u] = t < LY * K3 && li < nmain ? half[size_t(m0 + li) + size_t(h1) * jy + zs * k] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-      if (t < LY * K3 && li < nl) A[li * ls + lds_at(k)] = r[u];
+      if (t < LY * K3 && li < nmain) A[li * ls + lds_at(k)] = r[u];
     }
+  }
+  for (int t = threadIdx.x; t < nx * K3; t += blockDim.x) {
+    const int k = t / nx, li = LY + (t - k * nx);
+    A[li * ls + lds_at(k)] = half[size_t(m0 + li) + size_t(h1) * jy + zs * k];
   }
   // the influence function, element by element as the lines were read (consecutive lanes, consecutive a-frequencies:
   // whole cache lines), the first batch requested now so its latency passes under the transform
@@ -1144,44 +1161,55 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
 #pragma unroll
   for (int u = 0; u < kFftBatch; ++u) {
     const int t = u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-    gpre[u] = t < LY * K3 && li < nl ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+    gpre[u] = t < LY * K3 && li < nmain ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
   }
   store_twiddles(twl, twf, FP.ax[2]);
   __syncthreads();
   float2* Z = fft_lines<+1>(A, B, nl, FP.ax[2], twl);
   double e = 0, w[6] = {0, 0, 0, 0, 0, 0};
+  // element (line li, point k) × G, with its energy and virial terms when kE
+  auto convolve = [&](int li, int k, float g) {
+    const int a = m0 + li;
+    float2 v = Z[li * ls + lds_at(k)];
+    if (kE && (a || jy || k)) {
+      double em = 0.5 * (double(v.x) * v.x + double(v.y) * v.y);
+      if (a > 0 && 2 * a != G.k[0]) em *= 2;
+      double gd, wv[6];
+      green_at(G, a, jy, k, fac, kcoul, vol, bm, &gd, wv);
+      e += em * double(g);
+      for (int q = 0; q < 6; ++q) w[q] += em * wv[q];
+    }
+    v.x *= g, v.y *= g;
+    Z[li * ls + lds_at(k)] = v;
+  };
   for (int t0 = 0; t0 < LY * K3; t0 += kFftBatch * kFftThreads) {
     float gb[kFftBatch];
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-      gb[u] = t0 == 0 ? gpre[u] : t < LY * K3 && li < nl ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+      gb[u] = t0 == 0 ? gpre[u] : t < LY * K3 && li < nmain ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-      if (t >= LY * K3 || li >= nl) continue;
-      const int a = m0 + li;
-      const float g = gb[u];
-      float2 v = Z[li * ls + lds_at(k)];
-      if (kE && (a || jy || k)) {
-        double em = 0.5 * (double(v.x) * v.x + double(v.y) * v.y);
-        if (a > 0 && 2 * a != G.k[0]) em *= 2;
-        double gd, wv[6];
-        green_at(G, a, jy, k, fac, kcoul, vol, bm, &gd, wv);
-        e += em * double(g);
-        for (int q = 0; q < 6; ++q) w[q] += em * wv[q];
-      }
-      v.x *= g, v.y *= g;
-      Z[li * ls + lds_at(k)] = v;
+      if (t >= LY * K3 || li >= nmain) continue;
+      convolve(li, k, gb[u]);
     }
+  }
+  for (int t = threadIdx.x; t < nx * K3; t += blockDim.x) {
+    const int k = t / nx, li = LY + (t - k * nx);
+    convolve(li, k, gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k]);
   }
   __syncthreads();
   float2* other = Z == A ? B : A;
   const float2* R = fft_lines<-1>(Z, other, nl, FP.ax[2], twl);
   for (int t = threadIdx.x; t < LY * K3; t += blockDim.x) {
     const int k = t >> lg, li = t & (LY - 1);
-    if (li < nl) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
+    if (li < nmain) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
+  }
+  for (int t = threadIdx.x; t < nx * K3; t += blockDim.x) {
+    const int k = t / nx, li = LY + (t - k * nx);
+    half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
   }
   if (kE) {
     // the workgroup's partial sums in a fixed order
@@ -1659,8 +1687,8 @@ void launch_fft_y(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, int sign)
 
 void launch_fft_z_conv(capsgpu_ctx& c, const GridArgs& G, const FftPlan& F, bool want) {
   Pme& P = c.pme;
-  const int mblocks = (P.h1 + G.ly - 1) / G.ly;
-  const size_t lds_z = (2 * size_t(G.ly) * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2);
+  const int mblocks = zline_blocks(P.h1, G.ly, P.k[2]);
+  const size_t lds_z = (2 * size_t(zline_cap(G.ly, P.k[2])) * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2);
   const unsigned zb = unsigned(mblocks * P.k[1]);
   const double fac = kPi * kPi / (P.beta * P.beta);
   kt_begin(c, "k_fft_z_conv");
@@ -1790,7 +1818,9 @@ void pme_tune(capsgpu_ctx& c, int out[2]) {
       for (int ly : {4, 8, 16}) {
         P.lines_x = lx, P.lines_yz = ly;
         const GridArgs G = grid_args(P);
-        const float t = best_of([&] { fft_forward_conv_inverse(c, G, F, false); });
+        const float t = best_of([&] {   // four transforms a sample: one alone is too short to rank the shapes reliably
+          for (int rep = 0; rep < 4; ++rep) fft_forward_conv_inverse(c, G, F, false);
+        });
         if (t < tbest) tbest = t, pick_x = lx, pick_yz = ly;
       }
     }
@@ -2089,12 +2119,12 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
     ot.push_back(ot.back() + a.ntiles);
     ox.push_back(ox.back() + (plane ? 0 : (npairs + G.lx - 1) / G.lx));
     oy.push_back(oy.back() + (plane ? 0 : mblocks * P.k[2]));
-    oz.push_back(oz.back() + mblocks * P.k[1]);
+    oz.push_back(oz.back() + zline_blocks(P.h1, G.ly, P.k[2]) * P.k[1]);
     oxy.push_back(oxy.back() + (plane ? P.k[2] : 0));
     if (plane) g->lds_xy = std::max(g->lds_xy, plane_lds(P.k[0], P.k[1]));
     else g->lds_x = std::max(g->lds_x, (2 * size_t(G.lx) * size_t(line_stride(P.k[0])) + P.k[0]) * sizeof(float2));
     g->lds_y = std::max(g->lds_y, (2 * size_t(G.ly) * size_t(line_stride(P.k[1])) + P.k[1]) * sizeof(float2));
-    g->lds_z = std::max(g->lds_z, (2 * size_t(G.ly) * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2));
+    g->lds_z = std::max(g->lds_z, (2 * size_t(zline_cap(G.ly, P.k[2])) * size_t(line_stride(P.k[2])) + P.k[2]) * sizeof(float2));
   }
   g->reps.upload(reinterpret_cast<const unsigned char*>(R.data()), R.size() * sizeof(PmeRep), s);
   g->off_atoms.upload(oa, s), g->off_tiles.upload(ot, s), g->off_x.upload(ox, s), g->off_yz.upload(oy, s), g->off_z.upload(oz, s);

@@ -63,22 +63,30 @@ __device__ inline void vir_add(double* A, const double* d, const double* f) {
 
 constexpr int kEB = 9;   // bond, angle, virial, w[6]
 
-// The forces of term g (bonds [0, nb), then angles, then FENE bonds) on its members: put(place, fx, fy, fz) for each
-// member, place being the member's place in the per-atom order of slots (bond t: member slots 2t, 2t + 1; angle t:
-// 2nb + 3t + 0…2; FENE t: 2nb + 3na + 2t + 0…1); with kE and own, the term's energy and virial into A. No contraction
-// of a·b + c into FMAs here: the energy variant's extra arithmetic must not change how the forces are rounded (forces
-// are the same whether or not energies are asked for, and wherever the term is evaluated).
+// term g's member atoms (two for a bond or FENE bond, three for an angle; the third −1 for a bond)
+__device__ __forceinline__ int3 term_members(int g, int nb, int na, int nf, const int32_t* __restrict__ bij, const int32_t* __restrict__ aijk,
+                                             const int32_t* __restrict__ fij) {
+  if (g < nb) return make_int3(bij[2 * g], bij[2 * g + 1], -1);
+  if (g < nb + na) return make_int3(aijk[3 * (g - nb)], aijk[3 * (g - nb) + 1], aijk[3 * (g - nb) + 2]);
+  if (g < nb + na + nf) return make_int3(fij[2 * (g - nb - na)], fij[2 * (g - nb - na) + 1], -1);
+  return make_int3(-1, -1, -1);
+}
+
+// The forces of term g (bonds [0, nb), then angles, then FENE bonds) with members m (term_members) on those members:
+// put(place, fx, fy, fz) for each member, place being the member's place in the per-atom order of slots (bond t: member
+// slots 2t, 2t + 1; angle t: 2nb + 3t + 0…2; FENE t: 2nb + 3na + 2t + 0…1); with kE and own, the term's energy and
+// virial into A. No contraction of a·b + c into FMAs here: the energy variant's extra arithmetic must not change how
+// the forces are rounded (forces are the same whether or not energies are asked for, and wherever the term is
+// evaluated).
 template <bool kE, int kOrtho, class Put>
-__device__ __forceinline__ void term_forces(int g, bool own, int nb, int na, int nf, const CellArgs& C, const double* __restrict__ x,
-                                            const int32_t* __restrict__ bij, const double* __restrict__ bkr,
-                                            const int32_t* __restrict__ aijk, const double* __restrict__ akt,
-                                            const int32_t* __restrict__ fij, const double* __restrict__ fp,
+__device__ __forceinline__ void term_forces(int g, int3 mem, bool own, int nb, int na, int nf, const CellArgs& C, const double* __restrict__ x,
+                                            const double* __restrict__ bkr, const double* __restrict__ akt, const double* __restrict__ fp,
                                             const int32_t* __restrict__ spos, Put&& put, double* A, uint32_t* __restrict__ flags) {
 #pragma clang fp contract(off)
   if (g < nb || (g >= nb + na && g < nb + na + nf)) {
     const bool fene = g >= nb;
     const int t = fene ? g - nb - na : g;
-    const int32_t a = fene ? fij[2 * t] : bij[2 * t], b = fene ? fij[2 * t + 1] : bij[2 * t + 1];
+    const int32_t a = mem.x, b = mem.y;
     // the members' places, requested with the atoms (they depend on the term only)
     const size_t s0 = fene ? 2 * size_t(nb) + 3 * size_t(na) + 2 * size_t(t) : 2 * size_t(t);
     const int32_t p0 = spos[s0], p1 = spos[s0 + 1];
@@ -119,7 +127,7 @@ __device__ __forceinline__ void term_forces(int g, bool own, int nb, int na, int
   }
   if (g >= nb && g < nb + na) {
     const int t = g - nb;
-    const int32_t ia = aijk[3 * t], ja = aijk[3 * t + 1], ka = aijk[3 * t + 2];
+    const int32_t ia = mem.x, ja = mem.y, ka = mem.z;
     const size_t s0 = 2 * size_t(nb) + 3 * size_t(t);
     const int32_t p0 = spos[s0], p1 = spos[s0 + 1], p2 = spos[s0 + 2];
     double u[3] = {x[3 * ia] - x[3 * ja], x[3 * ia + 1] - x[3 * ja + 1], x[3 * ia + 2] - x[3 * ja + 2]};
@@ -173,7 +181,7 @@ __device__ __forceinline__ void b_bonded(int bx, int gx, int nb, int na, int nf,
                                                 double* __restrict__ slot, double* __restrict__ epart, uint32_t* __restrict__ flags) {
   const int g = bx * blockDim.x + threadIdx.x;
   double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  term_forces<kE, -1>(g, true, nb, na, nf, C, x, bij, bkr, aijk, akt, fij, fp, spos,
+  term_forces<kE, -1>(g, term_members(g, nb, na, nf, bij, aijk, fij), true, nb, na, nf, C, x, bkr, akt, fp, spos,
                   [&](int32_t p, double f0, double f1, double f2) {
                     double* o = slot + 3 * size_t(p);
                     o[0] = f0, o[1] = f1, o[2] = f2;
@@ -240,7 +248,8 @@ __global__ void __launch_bounds__(256) k_sum_forces(int n, const uint32_t* __res
 // member), keeps the forces on its own atoms in LDS at their places in the per-atom order of slots, then adds each
 // atom's slots in that order and writes the atom's total force: the sums of k_bonded + k_sum_forces, bit for bit,
 // without the slots' round trip through device memory.
-constexpr int kRangeAtoms = 256, kRangeSlots = 1024;   // 25 KB of slots per workgroup
+// (one wave per range of 64 atoms measured slower: 18.0 -> 22.4 us on the 262,144-atom melt)
+constexpr int kRangeAtoms = 256, kRangeSlots = 1024, kRangeThreads = 256;   // 25 KB of slots per workgroup
 // slot q's three doubles in LDS: 3q, with one double skipped after every eighth slot, so neighbouring atoms' runs of
 // slots (a few slots apart) start in different banks
 __device__ __forceinline__ int slot_at(int q) { return 3 * q + (q >> 3); }
@@ -250,8 +259,8 @@ struct BondSumArgs {
   const double* x;
   const int32_t *bij, *aijk, *fij, *spos, *aoff;
   const double *bkr, *akt, *fp;
-  const int32_t *rstart, *roff;   // range r: atoms rstart[r] … rstart[r+1] − 1, terms rterm[roff[r]] … (top bit: counted here)
-  const uint32_t* rterm;
+  const int32_t *rstart, *roff;   // range r: atoms rstart[r] … rstart[r+1] − 1, terms rterm[roff[r]] …
+  const int4* rterm;              // (term | top bit when its energy is counted here, its members as term_members)
   const uint32_t* iperm;          // the pair forces by slot of the cluster list (null: none)
   const float4* fnb;
   const float* frec;              // the reciprocal forces (null: none)
@@ -268,11 +277,17 @@ __device__ __forceinline__ void b_bonded_sum(int r, const BondSumArgs& a) {
   const int s0 = a.bonded ? a.aoff[r0] : 0, ns = a.bonded ? a.aoff[r1] - s0 : 0;
   double A[kEB] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (a.bonded) {
+    // each term's record (term and members) requested one term ahead, so its load passes under the previous term's work
+    const int k1 = a.roff[r + 1];
+    int k = a.roff[r] + int(threadIdx.x);
+    int4 rec = k < k1 ? a.rterm[k] : make_int4(-1, -1, -1, -1);
 #pragma unroll 1
-    for (int k = a.roff[r] + int(threadIdx.x); k < a.roff[r + 1]; k += blockDim.x) {
-      const uint32_t code = a.rterm[k];
-      term_forces<kE, kOrtho>(int(code & 0x7fffffffu), (code >> 31) != 0, a.nb, a.na, a.nf, a.C, a.x, a.bij, a.bkr, a.aijk, a.akt, a.fij, a.fp,
-                      a.spos,
+    for (; k < k1; k += blockDim.x) {
+      const int4 cur = rec;
+      rec = k + int(blockDim.x) < k1 ? a.rterm[k + blockDim.x] : make_int4(-1, -1, -1, -1);
+      const uint32_t code = uint32_t(cur.x);
+      term_forces<kE, kOrtho>(int(code & 0x7fffffffu), make_int3(cur.y, cur.z, cur.w), (code >> 31) != 0, a.nb, a.na, a.nf, a.C, a.x,
+                      a.bkr, a.akt, a.fp, a.spos,
                       [&](int32_t p, double f0, double f1, double f2) {   // the range's own members only
                         const int q = p - s0;
                         if (q >= 0 && q < ns) sl[slot_at(q)] = f0, sl[slot_at(q) + 1] = f1, sl[slot_at(q) + 2] = f2;
@@ -300,18 +315,18 @@ __device__ __forceinline__ void b_bonded_sum(int r, const BondSumArgs& a) {
     for (int m = 0; m < 3; ++m) a.f[3 * size_t(i) + m] = F[m];
   }
   if (kE) {
-    static_assert(3 * kRangeSlots >= kEB * 256, "the energy sums reuse the slots' space");
+    static_assert(3 * kRangeSlots >= kEB * kRangeThreads && kRangeThreads == 256, "the energy sums reuse the slots' space");
     __syncthreads();
     block_sums(A, sl, a.epart + kEB * size_t(r));
   }
 }
 template <bool kE, int kOrtho>
-__global__ void __launch_bounds__(256) k_bonded_sum(BondSumArgs a) {
+__global__ void __launch_bounds__(kRangeThreads) k_bonded_sum(BondSumArgs a) {
   b_bonded_sum<kE, kOrtho>(int(blockIdx.x), a);
 }
 // the force-only variant for orthorhombic cells (every step of most runs): held to 80 registers, so six workgroups
 // share a CU as their LDS allows, without spills
-__global__ void __launch_bounds__(256, 6) k_bonded_sum_ortho(BondSumArgs a) {
+__global__ void __launch_bounds__(kRangeThreads, 6) k_bonded_sum_ortho(BondSumArgs a) {
   b_bonded_sum<false, 1>(int(blockIdx.x), a);
 }
 
@@ -412,20 +427,23 @@ __device__ inline void normals3(uint64_t seed, uint64_t step, uint32_t atom, dou
   g[2] = r1 * cos(2 * kPi * u[3]);
 }
 
-// the workgroup's sums of v into its row of part (kAdd: added to what the row holds)
+// the workgroup's sums of v into its row of part (kAdd: added to prior, what the row held, read by the caller when it
+// started so the load's latency passes under its work): each wave's sums by a butterfly, then the waves' in order
 template <int kParts, bool kAdd = false>
-__device__ inline void block_sum_store(double (&v)[kParts], double* part, int row) {
-  __shared__ double red[kParts][256];
-  for (int q = 0; q < kParts; ++q) red[q][threadIdx.x] = v[q];
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st)
-      for (int q = 0; q < kParts; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
-    __syncthreads();
+__device__ inline void block_sum_store(double (&v)[kParts], double* part, int row, double prior = 0.0) {
+  __shared__ double red[kParts][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = int(blockDim.x + 63) >> 6;
+  for (int q = 0; q < kParts; ++q) {
+    double s = v[q];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) red[q][wave] = s;
   }
+  __syncthreads();
   if (threadIdx.x < kParts) {
-    double& d = part[kParts * size_t(row) + threadIdx.x];
-    d = kAdd ? d + red[threadIdx.x][0] : red[threadIdx.x][0];
+    double t = red[threadIdx.x][0];
+    for (int w = 1; w < nw; ++w) t += red[threadIdx.x][w];
+    part[kParts * size_t(row) + threadIdx.x] = kAdd ? prior + t : t;
   }
 }
 
@@ -478,6 +496,7 @@ __device__ __forceinline__ void b_baoab_open(int bx, int gx, int n, double hk, d
 #pragma clang fp contract(off)
   const int i = bx * blockDim.x + threadIdx.x;
   const uint64_t step = (stepc ? *stepc : 0) + step_add;
+  const double prior = threadIdx.x == 0 ? part[bx] : 0.0;   // the row the bath's share is added to
   double ke[1] = {0}, xn[3] = {0, 0, 0};
   if (i < n) {
     const double a = hk * minv[i], mi = 1.0 / minv[i], cs = c2 * sdv[i];
@@ -500,7 +519,7 @@ __device__ __forceinline__ void b_baoab_open(int bx, int gx, int n, double hk, d
   }
   move_flags(i < n, x, x0, x0p, i, lim_b, lim_p, flags);
   if (sp && sp->wq) spline_atom(*sp, i < n, i, xn);
-  block_sum_store<1, true>(ke, part, bx);
+  block_sum_store<1, true>(ke, part, bx, prior);
 }
 __global__ void __launch_bounds__(256) k_baoab_open(int n, double hk, double hdt, double c1, double c2, const double* __restrict__ minv,
                                                     const double* __restrict__ sdv, uint64_t seed, const uint64_t* __restrict__ stepc,
@@ -694,12 +713,14 @@ void bonded_setup(capsgpu_ctx& c, const capsgpu_ff& ff) {
       for (int u = 0; u < nd; ++u) ++roff[size_t(rr[u]) + 1];
     }
     for (int q = 0; q < nr; ++q) roff[size_t(q) + 1] += roff[size_t(q)];
-    std::vector<uint32_t> rterm(size_t(std::max(roff[size_t(nr)], 1)), 0u);
+    std::vector<int4> rterm(size_t(std::max(roff[size_t(nr)], 1)), make_int4(-1, -1, -1, -1));
     std::vector<int32_t> fill(roff.begin(), roff.end() - 1);
     for (int64_t g = 0; g < nterms; ++g) {
-      int32_t rr[3];
+      int32_t rr[3], m[3] = {-1, -1, -1};
       const int nd = ranges_of(g, rr);
-      for (int u = 0; u < nd; ++u) rterm[size_t(fill[size_t(rr[u])]++)] = uint32_t(g) | (u == 0 ? 0x80000000u : 0u);
+      members(g, m);
+      for (int u = 0; u < nd; ++u)
+        rterm[size_t(fill[size_t(rr[u])]++)] = make_int4(int32_t(uint32_t(g) | (u == 0 ? 0x80000000u : 0u)), m[0], m[1], m[2]);
     }
     B.rstart.upload(rstart, s);
     B.roff.upload(roff, s);
@@ -770,7 +791,7 @@ void bonded_and_sum(capsgpu_ctx& c, bool want, bool nb, bool rec, bool bond, dou
     a.epart = B.epart.p;
   }
   kt_begin(c, "k_bonded_sum");
-  const dim3 grid(unsigned(B.nranges)), block(256);
+  const dim3 grid(unsigned(B.nranges)), block(kRangeThreads);
   if (a.C.ortho) {
     if (ke) hipLaunchKernelGGL((k_bonded_sum<true, 1>), grid, block, 0, c.stream, a);
     else hipLaunchKernelGGL(k_bonded_sum_ortho, grid, block, 0, c.stream, a);
@@ -903,7 +924,7 @@ struct StepRep {
   double* slot;
   const float* frec;
   const int32_t *rstart, *roff;
-  const uint32_t* rterm;
+  const int4* rterm;
   int bonded;
   AtomSplines sp;   // the replica's per-atom PME bookkeeping, done by the opening launch (sp.wq null: none)
 };
@@ -938,7 +959,7 @@ __global__ void __launch_bounds__(256) kg_sum_forces(const StepRep* __restrict__
   b_sum_forces(int(blockIdx.x) - off[r], off[r + 1] - off[r], a.n, nullptr, nullptr, a.frec, a.aoff, a.slot, a.f);
 }
 
-__global__ void __launch_bounds__(256) kg_bonded_sum(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
+__global__ void __launch_bounds__(kRangeThreads) kg_bonded_sum(const StepRep* __restrict__ R, const int* __restrict__ off, int nr) {
   const int r = find_rep(off, nr, blockIdx.x);
   const StepRep& a = R[r];
   BondSumArgs b{};
@@ -1014,7 +1035,7 @@ void step_group_open(StepGroup& g, hipStream_t s, uint64_t step_add) {
 void step_group_close(StepGroup& g, hipStream_t s) {
   const StepRep* R = reinterpret_cast<const StepRep*>(g.reps.p);
   if (g.ranged) {
-    hipLaunchKernelGGL(kg_bonded_sum, dim3(unsigned(g.h_ranges)), dim3(256), 0, s, R, g.off_ranges.p, g.nr);
+    hipLaunchKernelGGL(kg_bonded_sum, dim3(unsigned(g.h_ranges)), dim3(kRangeThreads), 0, s, R, g.off_ranges.p, g.nr);
     LAUNCHCHECK();
   } else {
     hipLaunchKernelGGL(kg_bonded, dim3(unsigned(g.h_terms)), dim3(256), 0, s, R, g.off_terms.p, g.nr);
