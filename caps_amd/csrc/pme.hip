@@ -454,38 +454,38 @@ __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const i
     }
     __syncthreads();
     const int c1 = min(m, c0 + kBlkStage);
-    // per tile column dx and home row (iy, iz): the home tiles dx … dx + NTP − 1 along a, consecutive in the staged
-    // order, each atom's θa on this column staged for it
+    // per home row (iy, iz): its NA home tiles along a, consecutive in the staged order, for both of the wave's tiles at
+    // once — each candidate's θa staged for either tile column, zero for the column it does not reach (the home tiles
+    // dx … dx + NTP − 1 reach column dx)
 #pragma unroll
-    for (int dx = 0; dx < 2; ++dx) {
+    for (int iz = dz; iz < dz + NTP; ++iz) {
 #pragma unroll
-      for (int iz = dz; iz < dz + NTP; ++iz) {
-#pragma unroll
-        for (int iy = dy; iy < dy + NTP; ++iy) {
-          const int h = dx + NA * (iy + NA * iz);
-          const int a = __builtin_amdgcn_readfirstlane(max(s_ex[h], c0));
-          const int n = __builtin_amdgcn_readfirstlane(min(s_ex[h + NTP], c1)) - a;
-          if (n <= 0) continue;
-          const int oy = dy + NTP - 1 - iy, oz = dz + NTP - 1 - iz;
-          const float* pb = sWb + (4 * oy + gy) * kBlkPitch + (a - c0) + eighth;
-          const float* pc = sWc + (4 * oz + gz) * kBlkPitch + (a - c0) + eighth;
-          const float4* pa = sWa + dx * kBlkStage + (a - c0) + eighth;
-          auto add = [&](int g, float z) {
-            const float c = pc[g] * z, v0 = pb[g] * c, v1 = pb[g + 2 * kBlkPitch] * c;
-            const float4 wa = pa[g];
-            acc[dx][0][0] += v0 * wa.x, acc[dx][0][1] += v0 * wa.y, acc[dx][0][2] += v0 * wa.z, acc[dx][0][3] += v0 * wa.w;
-            acc[dx][1][0] += v1 * wa.x, acc[dx][1][1] += v1 * wa.y, acc[dx][1][2] += v1 * wa.z, acc[dx][1][3] += v1 * wa.w;
-          };
-          // rounds of eight candidates, one per eighth: every lane has one in each round but the last
-          const int T = (n + 7) >> 3;
-          int k = 0;
-          for (; k + 2 < T; k += 2) add(8 * k, 1.0f), add(8 * k + 8, 1.0f);
-          if (k + 1 < T) add(8 * k, 1.0f), ++k;
-          // the last round: a lane past the run takes the run's last candidate with a zero weight (finite products of
-          // zero, so its sums are unchanged)
-          const bool in = 8 * k + eighth < n;
-          add(in ? 8 * k : n - 1 - eighth, in ? 1.0f : 0.0f);
-        }
+      for (int iy = dy; iy < dy + NTP; ++iy) {
+        const int h = NA * (iy + NA * iz);
+        const int a = __builtin_amdgcn_readfirstlane(max(s_ex[h], c0));
+        const int n = __builtin_amdgcn_readfirstlane(min(s_ex[h + NA], c1)) - a;
+        if (n <= 0) continue;
+        const int oy = dy + NTP - 1 - iy, oz = dz + NTP - 1 - iz;
+        const float* pb = sWb + (4 * oy + gy) * kBlkPitch + (a - c0) + eighth;
+        const float* pc = sWc + (4 * oz + gz) * kBlkPitch + (a - c0) + eighth;
+        const float4* pa = sWa + (a - c0) + eighth;
+        auto add = [&](int g, float z) {
+          const float c = pc[g] * z, v0 = pb[g] * c, v1 = pb[g + 2 * kBlkPitch] * c;
+          const float4 w0 = pa[g], w1 = pa[kBlkStage + g];
+          acc[0][0][0] += v0 * w0.x, acc[0][0][1] += v0 * w0.y, acc[0][0][2] += v0 * w0.z, acc[0][0][3] += v0 * w0.w;
+          acc[0][1][0] += v1 * w0.x, acc[0][1][1] += v1 * w0.y, acc[0][1][2] += v1 * w0.z, acc[0][1][3] += v1 * w0.w;
+          acc[1][0][0] += v0 * w1.x, acc[1][0][1] += v0 * w1.y, acc[1][0][2] += v0 * w1.z, acc[1][0][3] += v0 * w1.w;
+          acc[1][1][0] += v1 * w1.x, acc[1][1][1] += v1 * w1.y, acc[1][1][2] += v1 * w1.z, acc[1][1][3] += v1 * w1.w;
+        };
+        // rounds of eight candidates, one per eighth: every lane has one in each round but the last
+        const int T = (n + 7) >> 3;
+        int k = 0;
+        for (; k + 2 < T; k += 2) add(8 * k, 1.0f), add(8 * k + 8, 1.0f);
+        if (k + 1 < T) add(8 * k, 1.0f), ++k;
+        // the last round: a lane past the run takes the run's last candidate with a zero weight (finite products of
+        // zero, so its sums are unchanged)
+        const bool in = 8 * k + eighth < n;
+        add(in ? 8 * k : n - 1 - eighth, in ? 1.0f : 0.0f);
       }
     }
     __syncthreads();
