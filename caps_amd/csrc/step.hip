@@ -415,16 +415,20 @@ __device__ inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
   }
 }
 
-// three standard normals for (atom, step) by Box–Muller on Philox uniforms
+// three standard normals for (atom, step) by Box–Muller on Philox uniforms, in FP32 with the hardware's log2, sine and
+// cosine (argument in turns) and square root: u = (top 23 bits + ½) / 2^23 is exact in FP32 and lies in (0, 1), so every
+// log is finite; the normals are good to about 1e-6, far below what a thermostat's noise needs
 __device__ inline void normals3(uint64_t seed, uint64_t step, uint32_t atom, double* g) {
   uint32_t c[4] = {atom, uint32_t(step), uint32_t(step >> 32), 0x43415053u};
   philox(c, uint32_t(seed), uint32_t(seed >> 32));
-  double u[4];
-  for (int k = 0; k < 4; ++k) u[k] = (double(c[k]) + 0.5) * 2.3283064365386963e-10;
-  const double r0 = sqrt(-2 * log(u[0])), r1 = sqrt(-2 * log(u[2]));
-  g[0] = r0 * cos(2 * kPi * u[1]);
-  g[1] = r0 * sin(2 * kPi * u[1]);
-  g[2] = r1 * cos(2 * kPi * u[3]);
+  float u[4];
+  for (int k = 0; k < 4; ++k) u[k] = (float(c[k] >> 9) + 0.5f) * 1.1920928955078125e-7f;
+  const float ln2x2 = -1.3862943611198906f;   // −2 ln 2: −2 ln u = −2 ln 2 · log2 u
+  const float r0 = __builtin_amdgcn_sqrtf(ln2x2 * __builtin_amdgcn_logf(u[0]));
+  const float r1 = __builtin_amdgcn_sqrtf(ln2x2 * __builtin_amdgcn_logf(u[2]));
+  g[0] = double(r0 * __builtin_amdgcn_cosf(u[1]));
+  g[1] = double(r0 * __builtin_amdgcn_sinf(u[1]));
+  g[2] = double(r1 * __builtin_amdgcn_cosf(u[3]));
 }
 
 // the workgroup's sums of v into its row of part (kAdd: added to prior, what the row held, read by the caller when it
