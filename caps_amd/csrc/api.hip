@@ -538,7 +538,7 @@ int32_t capsgpu_compute_noalloc(capsgpu_ctx* c, const double* x, const double ce
     if (pme_on(*c)) {
       int k[3];
       pme_grid_dims(*c, cell, k);
-      const size_t k3 = size_t(k[0]) * k[1] * k[2], nh = size_t(k[0] / 2 + 1) * k[1] * k[2];
+      const size_t k3 = size_t(k[0]) * k[1] * k[2], nh = size_t(half_stride(k[0])) * k[1] * k[2];
       const size_t tiles = size_t((k[0] + 3) / 4) * ((k[1] + 3) / 4) * ((k[2] + 3) / 4);
       need += 4 * k3 + 12 * nh + 16 * size_t(k[0] + k[1] + k[2]) + (20 + 4 * size_t(tile_capacity(int64_t(n), int64_t(tiles)))) * tiles +
               60 * std::max({k[0], k[1], k[2]}) + 88 * n;

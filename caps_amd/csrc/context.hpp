@@ -14,6 +14,10 @@
 #include "scan.hpp"
 
 namespace capsgpu {
+
+// the PME half spectrum's rows along a: K1/2 + 1 elements at a stride rounded up to 16 (128 bytes), so runs of
+// consecutive a-frequencies start on cache lines; the padding is never read or written
+__host__ __device__ constexpr int half_stride(int K1) { return (K1 / 2 + 1 + 15) & ~15; }
 struct Learned;
 
 // Geometry of one cluster-list build, fixed on the host the way core/src/nbnxm.cpp:40-47 and 146-164 fix it
@@ -85,7 +89,7 @@ struct DevList {
 };
 
 struct Pme {
-  int k[3] = {0, 0, 0}, order = 5, h1 = 0;
+  int k[3] = {0, 0, 0}, order = 5, h1 = 0, hs = 0;   // hs: the half spectrum's row stride (half_stride)
   double beta = 0, cell[12] = {}, hi[3][3] = {}, vol = 0;
   bool green_ok = false;
   DevBuf<float> grid;                   // real K1·K2·K3

@@ -62,7 +62,7 @@ __device__ inline void bspline(float w, float* th, float* dth) {
 }
 
 struct GridArgs {
-  int k[3], nt[3], p, h1, ntm;   // ntm: the largest of nt (the home table's stride)
+  int k[3], nt[3], p, h1, hs, ntm;   // hs: half_stride(K1); ntm: the largest of nt (the home table's stride)
   int lx;                        // the transforms' tile shape: pairs of real lines per workgroup along a,
   int ly, ly_log2;               // and lines per workgroup along b and c (powers of two)
   double hi[3][3];
@@ -780,7 +780,7 @@ constexpr int kFftThreads = 256, kFftBatch = 8;
 __device__ __forceinline__ void b_fft_x_r2c(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float* __restrict__ grid,
                                                    float2* __restrict__ half) {
   extern __shared__ float2 lds[];
-  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
+  const int K1 = G.k[0], h1 = G.h1, hs = G.hs, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
   const int LP = G.lx;
   float2* A = lds;
   float2* B = lds + LP * ls;
@@ -814,10 +814,10 @@ __device__ __forceinline__ void b_fft_x_r2c(int bx, int gx, GridArgs G, FftPlan 
     const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
     const float2 zm = Z[pl * ls + lds_at(m)], zc0 = Z[pl * ls + lds_at((K1 - m) % K1)];
     const float2 zc = make_float2(zc0.x, -zc0.y);
-    half[size_t(la) * h1 + m] = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
+    half[size_t(la) * hs + m] = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y + zc.y));
     if (lb != la) {
       const float2 d = make_float2(zm.x - zc.x, zm.y - zc.y);
-      half[size_t(lb) * h1 + m] = make_float2(0.5f * d.y, -0.5f * d.x);
+      half[size_t(lb) * hs + m] = make_float2(0.5f * d.y, -0.5f * d.x);
     }
   }
 }
@@ -830,7 +830,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(GridArgs G, FftPlan FP, const
 __device__ __forceinline__ void b_fft_x_c2r(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, const float2* __restrict__ half,
                                                    float* __restrict__ grid) {
   extern __shared__ float2 lds[];
-  const int K1 = G.k[0], h1 = G.h1, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
+  const int K1 = G.k[0], h1 = G.h1, hs = G.hs, nl = G.k[1] * G.k[2], npairs = (nl + 1) / 2, ls = line_stride(K1);
   const int LP = G.lx;
   float2* A = lds;
   float2* B = lds + LP * ls;
@@ -846,8 +846,8 @@ __device__ __forceinline__ void b_fft_x_c2r(int bx, int gx, GridArgs G, FftPlan 
       if (t < LP * K1 && pr < npairs) {
         const int la = 2 * pr, lb = min(2 * pr + 1, nl - 1);
         const int mh = m > K1 / 2 ? K1 - m : m;
-        ra[u] = half[size_t(la) * h1 + mh];
-        if (lb != la) rb[u] = half[size_t(lb) * h1 + mh];
+        ra[u] = half[size_t(la) * hs + mh];
+        if (lb != la) rb[u] = half[size_t(lb) * hs + mh];
       }
     }
 #pragma unroll
@@ -877,7 +877,7 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(GridArgs G, FftPlan FP, const
   b_fft_x_c2r(blockIdx.x, gridDim.x, G, FP, tw, half, grid);
 }
 
-// lines along b of the half spectrum: element (m, j, k) at m + h1 (j + K2 k)
+// lines along b of the half spectrum: element (m, j, k) at m + hs (j + K2 k)
 template <int Sign>
 __device__ __forceinline__ void b_fft_y(int bx, int gx, GridArgs G, FftPlan FP, const float2* __restrict__ tw, float2* __restrict__ half) {
   extern __shared__ float2 lds[];
@@ -895,7 +895,7 @@ __device__ __forceinline__ void b_fft_y(int bx, int gx, GridArgs G, FftPlan FP, 
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, j = t >> lg, li = t & (LY - 1);
-      r[u] = t < LY * K2 && li < nl ? half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] : make_float2(0.0f, 0.0f);
+      r[u] = t < LY * K2 && li < nl ? half[size_t(m0 + li) + size_t(G.hs) * (j + size_t(K2) * kz)] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
@@ -908,7 +908,7 @@ __device__ __forceinline__ void b_fft_y(int bx, int gx, GridArgs G, FftPlan FP, 
   const float2* Z = fft_lines<Sign>(A, B, nl, FP.ax[1], twl);
   for (int t = threadIdx.x; t < LY * K2; t += blockDim.x) {
     const int j = t >> lg, li = t & (LY - 1);
-    if (li < nl) half[size_t(m0 + li) + size_t(h1) * (j + size_t(K2) * kz)] = Z[li * ls + lds_at(j)];
+    if (li < nl) half[size_t(m0 + li) + size_t(G.hs) * (j + size_t(K2) * kz)] = Z[li * ls + lds_at(j)];
   }
 }
 template <int Sign>
@@ -941,7 +941,8 @@ template <int KA, int KB>
 __device__ __forceinline__ void b_fft_xy_fwd(int kz, int part, int parts, GridArgs G, FftPlan FP, const float2* __restrict__ tw,
                                              const float* __restrict__ grid, float2* __restrict__ half) {
   extern __shared__ float2 lds[];
-  const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, ls1 = line_stride(K1), ls2 = line_stride(K2), npair = K2 / 2;
+  const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, hs = half_stride(K1), ls1 = line_stride(K1), ls2 = line_stride(K2),
+            npair = K2 / 2;
   const int S = plane_region(K1, K2);
   float2* A = lds;
   float2* B = lds + S;
@@ -984,10 +985,10 @@ __device__ __forceinline__ void b_fft_xy_fwd(int kz, int part, int parts, GridAr
   }
   __syncthreads();
   const float2* W = fft_lines<+1>(Y + m0 * ls2, Z + m0 * ls2, nm, FP.ax[1], twb);   // line m at W + (m − m0) ls2
-  float2* dst = half + size_t(kz) * K2 * h1;
+  float2* dst = half + size_t(kz) * K2 * hs;
   for (int t = threadIdx.x; t < K2 * nm; t += blockDim.x) {
     const int j = t / nm, m = t - j * nm;
-    dst[size_t(j) * h1 + m0 + m] = W[m * ls2 + lds_at(j)];
+    dst[size_t(j) * hs + m0 + m] = W[m * ls2 + lds_at(j)];
   }
 }
 
@@ -995,20 +996,21 @@ template <int KA, int KB>
 __device__ __forceinline__ void b_fft_xy_inv(int kz, int part, int parts, GridArgs G, FftPlan FP, const float2* __restrict__ tw,
                                              const float2* __restrict__ half, float* __restrict__ grid) {
   extern __shared__ float2 lds[];
-  const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, ls1 = line_stride(K1), ls2 = line_stride(K2), npair = K2 / 2;
+  const int K1 = KA ? KA : G.k[0], K2 = KB ? KB : G.k[1], h1 = K1 / 2 + 1, hs = half_stride(K1), ls1 = line_stride(K1), ls2 = line_stride(K2),
+            npair = K2 / 2;
   const int S = plane_region(K1, K2);
   float2* A = lds;
   float2* B = lds + S;
   float2* twa = lds + 2 * S;
   float2* twb = twa + K1;
-  const float2* src = half + size_t(kz) * K2 * h1;
+  const float2* src = half + size_t(kz) * K2 * hs;
   const int total = K2 * h1;
   for (int t0 = 0; t0 < total; t0 += kPlaneBatch * kPlaneThreads) {
     float2 r[kPlaneBatch];
 #pragma unroll
     for (int u = 0; u < kPlaneBatch; ++u) {
-      const int t = t0 + u * kPlaneThreads + threadIdx.x;
-      r[u] = t < total ? src[t] : make_float2(0.0f, 0.0f);
+      const int t = t0 + u * kPlaneThreads + threadIdx.x, j = t / h1, m = t - j * h1;
+      r[u] = t < total ? src[size_t(j) * hs + m] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < kPlaneBatch; ++u) {
@@ -1117,7 +1119,7 @@ __global__ void k_green(GridArgs G, double fac, double kcoul, double vol, const 
   const int a = int(t % G.h1), b = int((t / G.h1) % G.k[1]), c = int(t / (size_t(G.h1) * G.k[1]));
   double g = 0;
   if (a || b || c) green_at(G, a, b, c, fac, kcoul, vol, bm, &g, nullptr);
-  gtab[t] = float(g);
+  gtab[size_t(a) + size_t(G.hs) * (b + size_t(G.k[1]) * c)] = float(g);
 }
 
 // lines along c: forward transform, × G (energy and virial when kE), inverse transform
@@ -1137,13 +1139,13 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
   float2* B = lds + zline_cap(LY, K3) * ls;
   float2* twl = B + zline_cap(LY, K3) * ls;
   const TwFetch twf = fetch_twiddles(tw, FP.ax[2]);
-  const size_t zs = size_t(h1) * K2;
+  const size_t zs = size_t(G.hs) * K2;
   for (int t0 = 0; t0 < LY * K3; t0 += kFftBatch * kFftThreads) {
     float2 r[kFftBatch];
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-      r[u] = t < LY * K3 && li < nmain ? half[size_t(m0 + li) + size_t(h1) * jy + zs * k] : make_float2(0.0f, 0.0f);
+      r[u] = t < LY * K3 && li < nmain ? half[size_t(m0 + li) + size_t(G.hs) * jy + zs * k] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
@@ -1153,7 +1155,7 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
   }
   for (int t = threadIdx.x; t < nx * K3; t += blockDim.x) {
     const int k = t / nx, li = LY + (t - k * nx);
-    A[li * ls + lds_at(k)] = half[size_t(m0 + li) + size_t(h1) * jy + zs * k];
+    A[li * ls + lds_at(k)] = half[size_t(m0 + li) + size_t(G.hs) * jy + zs * k];
   }
   // the influence function, element by element as the lines were read (consecutive lanes, consecutive a-frequencies:
   // whole cache lines), the first batch requested now so its latency passes under the transform
@@ -1161,7 +1163,7 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
 #pragma unroll
   for (int u = 0; u < kFftBatch; ++u) {
     const int t = u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-    gpre[u] = t < LY * K3 && li < nmain ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+    gpre[u] = t < LY * K3 && li < nmain ? gtab[size_t(m0 + li) + size_t(G.hs) * jy + zs * k] : 0.0f;
   }
   store_twiddles(twl, twf, FP.ax[2]);
   __syncthreads();
@@ -1187,7 +1189,7 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
       const int t = t0 + u * kFftThreads + threadIdx.x, k = t >> lg, li = t & (LY - 1);
-      gb[u] = t0 == 0 ? gpre[u] : t < LY * K3 && li < nmain ? gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k] : 0.0f;
+      gb[u] = t0 == 0 ? gpre[u] : t < LY * K3 && li < nmain ? gtab[size_t(m0 + li) + size_t(G.hs) * jy + zs * k] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < kFftBatch; ++u) {
@@ -1198,18 +1200,18 @@ __device__ __forceinline__ void b_fft_z_conv(int bx, int gx, GridArgs G, FftPlan
   }
   for (int t = threadIdx.x; t < nx * K3; t += blockDim.x) {
     const int k = t / nx, li = LY + (t - k * nx);
-    convolve(li, k, gtab[size_t(m0 + li) + size_t(h1) * jy + zs * k]);
+    convolve(li, k, gtab[size_t(m0 + li) + size_t(G.hs) * jy + zs * k]);
   }
   __syncthreads();
   float2* other = Z == A ? B : A;
   const float2* R = fft_lines<-1>(Z, other, nl, FP.ax[2], twl);
   for (int t = threadIdx.x; t < LY * K3; t += blockDim.x) {
     const int k = t >> lg, li = t & (LY - 1);
-    if (li < nmain) half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
+    if (li < nmain) half[size_t(m0 + li) + size_t(G.hs) * jy + zs * k] = R[li * ls + lds_at(k)];
   }
   for (int t = threadIdx.x; t < nx * K3; t += blockDim.x) {
     const int k = t / nx, li = LY + (t - k * nx);
-    half[size_t(m0 + li) + size_t(h1) * jy + zs * k] = R[li * ls + lds_at(k)];
+    half[size_t(m0 + li) + size_t(G.hs) * jy + zs * k] = R[li * ls + lds_at(k)];
   }
   if (kE) {
     // the workgroup's partial sums in a fixed order
@@ -1473,6 +1475,7 @@ GridArgs grid_args(const Pme& P) {
   G.ntm = std::max(G.nt[0], std::max(G.nt[1], G.nt[2]));
   G.p = P.order;
   G.h1 = P.h1;
+  G.hs = P.hs;
   G.lx = P.lines_x > 0 ? P.lines_x : P.plane_ok ? 0 : 8;   // 0: the plane transform
   G.ly = P.lines_yz;
   G.ly_log2 = 0;
@@ -1609,6 +1612,7 @@ void pme_setup(capsgpu_ctx& c) {
   P.order = c.opt.pme_order;
   if (P.order < 3 || P.order > kMaxOrder) throw Error(-1, "PME order must be 3 … 8");
   P.h1 = P.k[0] / 2 + 1;
+  P.hs = half_stride(P.k[0]);
   hipStream_t s = c.stream;
   std::vector<double> bm(size_t(P.k[0] + P.k[1] + P.k[2]));
   moduli(P.k[0], P.order, bm.data());
@@ -1622,12 +1626,13 @@ void pme_setup(capsgpu_ctx& c) {
       tw.push_back(make_float2(float(std::cos(ang)), float(std::sin(ang))));
     }
   P.tw.upload(tw, s);
-  const size_t nh = size_t(P.h1) * P.k[1] * P.k[2];
+  const size_t nh = size_t(P.hs) * P.k[1] * P.k[2];
   P.gtab.resize(nh);
   const GridArgs G = grid_args(P);
   const double fac = kPi * kPi / (P.beta * P.beta);
   kt_begin(c, "k_green");
-  hipLaunchKernelGGL(k_green, dim3(blocks_for(nh, 256)), dim3(256), 0, s, G, fac, kCoulomb, P.vol, P.bmod.p, P.gtab.p);
+  hipLaunchKernelGGL(k_green, dim3(blocks_for(size_t(P.h1) * P.k[1] * P.k[2], 256)), dim3(256), 0, s, G, fac, kCoulomb, P.vol, P.bmod.p,
+                     P.gtab.p);
   LAUNCHCHECK();
   kt_end(c);
   // per axis and tile T: the distinct home tiles of the first points that reach points 4T … 4T + 3 (count, then up to
