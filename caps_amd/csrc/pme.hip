@@ -110,6 +110,10 @@ __device__ __forceinline__ void b_tile_rank(int bx, int gx, int n, int ntiles, c
                                             float4* __restrict__ wsort, int4* __restrict__ bsort, uint32_t* __restrict__ tcount,
                                             const int64_t* __restrict__ sums = nullptr, int nsums = 0, int64_t* __restrict__ tout = nullptr) {
   const int i = bx * blockDim.x + threadIdx.x;
+  // the atom's data requested first, so its latency passes under the block offsets' sums
+  int4 bi = make_int4(0, 0, 0, 0);
+  float4 wi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i < n) bi = base[i], wi = wq[i];
   __shared__ int64_t boff[64];
   if (sums) {
     block_offsets(sums, nsums, boff);
@@ -118,16 +122,26 @@ __device__ __forceinline__ void b_tile_rank(int bx, int gx, int n, int ntiles, c
   for (int t = i; t < ntiles; t += gx * blockDim.x) tcount[t] = 0u;
   const uint32_t novf = cnt[0];
   if (i < n) {
-    const int4 bi = base[i];
-    const float4 wi = wq[i];
     const int t = bi.w;
+    const uint32_t* bk = bucket + size_t(t) * cap;
+    // the bucket's first 16 slots requested with the tile's places, before its count is known (a bucket holds at least
+    // 32 slots, a multiple of 8, so these loads stay inside it and are 16-byte aligned): most tiles hold fewer atoms
+    uint4 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = reinterpret_cast<const uint4*>(bk)[u];
     int64_t b = tstart[t], e = tstart[t + 1];
     if (sums) b += boff[t / kScanTile], e += boff[(t + 1) / kScanTile];
     const int64_t m = e - b;
     const int nb = int(m < cap ? m : cap);
-    const uint32_t* bk = bucket + size_t(t) * cap;
     int64_t r = 0;
-    int k = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      r += (4 * u + 0 < nb && q[u].x < uint32_t(i)) ? 1 : 0;
+      r += (4 * u + 1 < nb && q[u].y < uint32_t(i)) ? 1 : 0;
+      r += (4 * u + 2 < nb && q[u].z < uint32_t(i)) ? 1 : 0;
+      r += (4 * u + 3 < nb && q[u].w < uint32_t(i)) ? 1 : 0;
+    }
+    int k = 16;
     for (; k + 8 <= nb; k += 8) {   // eight loads in flight
       uint32_t t8[8];
 #pragma unroll
