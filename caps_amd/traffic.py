@@ -5,7 +5,8 @@ calibrated against a plain device-to-device copy of known size in the same run.
     # kernel shapes the tuner picks
     rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o t -- python3 -m caps_amd.traffic run
     rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o t -- python3 -m caps_amd.traffic run
-    # the per-launch bytes of a kernel, written as JSON (and printed)
+    # the per-launch bytes of a kernel over the last five evaluations (after the tuner's candidates), written as JSON
+    # (and printed); a kernel launched twice per evaluation (k_fft_xy: forward and inverse) takes a sixth argument, 2
     python3 -m caps_amd.traffic report OUT/fetch/t_counter_collection.csv OUT/write/t_counter_collection.csv k_spread OUT.json
 
 The copy reads and writes COPY_BYTES; each counter's scale is COPY_BYTES over what it reads for that copy, so the
@@ -16,6 +17,7 @@ import json
 import sys
 
 COPY_BYTES = 256 << 20
+EVALS = 5   # the evaluations after tuning whose launches a report averages
 
 
 def run():
@@ -31,30 +33,30 @@ def run():
     ev = ca.Evaluator(s.ff, op)
     ev.compute(s.x, s.cell)
     ev.tune()   # the kernel shapes a run takes: the gather's workgroups and the transforms' tiles, picked by timing
-    for _ in range(5):
+    for _ in range(EVALS):
         ev.compute(s.x, s.cell)
     torch.cuda.synchronize()
     print("traffic run done", flush=True)
 
 
 def _per_dispatch(path, counter):
-    """{dispatch id: (kernel name, counter value)} of one counter's CSV"""
+    """{dispatch id: (kernel name, counter value)} of one counter's CSV, in dispatch order"""
     out = {}
     for r in csv.DictReader(open(path)):
         if r["Counter_Name"] != counter:
             continue
-        d = r["Dispatch_Id"]
+        d = int(r["Dispatch_Id"])
         name, v = out.get(d, (r["Kernel_Name"], 0.0))
         out[d] = (name, v + float(r["Counter_Value"]))
-    return out
+    return dict(sorted(out.items()))
 
 
-def report(fetch_csv, write_csv, kernel, out_json):
-    res = {"kernel": kernel, "copy_bytes": COPY_BYTES}
+def report(fetch_csv, write_csv, kernel, out_json, per_eval="1"):
+    res = {"kernel": kernel, "copy_bytes": COPY_BYTES, "launches_averaged": "the last %d evaluations' (after tuning)" % EVALS}
     for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
         d = _per_dispatch(path, counter)
         copies = [v for n, v in d.values() if "copyBuffer" in n]
-        launches = [v for n, v in d.values() if kernel in n]
+        launches = [v for n, v in d.values() if kernel in n][-EVALS * int(per_eval):]
         if not copies or not launches:
             raise SystemExit("no %s readings for the copy or for %s in %s" % (counter, kernel, path))
         scale = COPY_BYTES / max(copies)
@@ -70,7 +72,7 @@ def report(fetch_csv, write_csv, kernel, out_json):
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "run":
         run()
-    elif len(sys.argv) == 6 and sys.argv[1] == "report":
+    elif len(sys.argv) in (6, 7) and sys.argv[1] == "report":
         report(*sys.argv[2:])
     else:
         print(__doc__)
