@@ -108,7 +108,7 @@ struct Pme {
   DevBuf<float> wsort;                  // theta in tatoms's order (float4 per atom)
   DevBuf<int32_t> bsort;                // base in tatoms's order (int4 per atom)
   DevBuf<int64_t> tstart;
-  DevBuf<int64_t> tlocal;               // the tile counts' scan within each scan block (k_tile_fill adds the block offsets)
+  DevBuf<int64_t> tlocal;               // the tile counts' scan within each scan block (k_tile_rank adds the block offsets)
   DevBuf<float> fpme;                   // 3 per atom
   DevBuf<float2> tw;                    // twiddles of the three axes
   DevBuf<double> epart;                 // energy and virial partials

@@ -2159,8 +2159,10 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
 
 void pme_group_destroy(PmeGroup* g) { delete g; }
 
-// the reciprocal forces of every replica into its pme.fpme: eight launches for the whole group when every replica is on
-// the plane transform, ten when none is
+// the reciprocal forces of every replica into its pme.fpme: for the whole group seven launches when every replica is on
+// the plane transform and shares one spread and one gather shape; one more for each of those two kernels whose
+// replicas take both shapes; the line transforms' four when some replicas take them (replacing the plane kernel's two
+// when all do); one more for the splines when the integrator's launch has not placed the atoms
 AtomSplines pme_atom_splines(capsgpu_ctx& c) {
   Pme& P = c.pme;
   return atom_splines(grid_args(P), c.d_charge.p, reinterpret_cast<float4*>(P.theta.p), reinterpret_cast<int4*>(P.base.p), P.tcount.p,
