@@ -4,7 +4,7 @@
 //
 //  k_splines      per atom: fractional grid coordinate, first grid point, B-spline weights and derivatives; home tile
 //  k_tile_*       atoms binned by the 4x4x4 grid tile of their first point (sorted by atom in each tile)
-//  k_spread_blk   one 256-thread workgroup per block of 2×2×2 tiles (every K a multiple of 8): each atom of the home
+//  k_spread_blk   one 256-thread workgroup per block of 2×2×2 tiles (every K a multiple of 4): each atom of the home
 //                 tiles that reach the block staged once (splines, weight windows) in LDS, then each wave sums two
 //                 tiles, a lane taking a row of 4 points over a quarter of the atoms (every point written once, no
 //                 atomics, a fixed order)
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(64) k_spread(GridArgs G, const int* __restrict
   b_spread<P>(tile, G, home, tstart, wsort, bsort, grid);
 }
 
-// ---- the spread by blocks of 2×2×2 tiles (every K a multiple of 8) ---------------------------------------------------
+// ---- the spread by blocks of 2×2×2 tiles (every K a multiple of 4) ---------------------------------------------------
 // k_spread stages each atom once per tile it reaches (eight tiles at order 5), its splines recomputed each time; here a
 // workgroup of four waves takes a block of eight tiles and stages each atom of the home tiles that reach the block
 // once: its splines (the B-spline pieces as polynomials in w, by Horner), and per axis the NTP windows of four weights
@@ -370,6 +370,11 @@ __device__ __forceinline__ void bspline_horner(float w, float* th) {
   }
 }
 
+// the block spread's workgroups: blocks of 2x2x2 tiles, the last along an axis with an odd tile count one tile deep
+__host__ __device__ inline int spread_block_count(const GridArgs& G) {
+  return ((G.nt[0] + 1) / 2) * ((G.nt[1] + 1) / 2) * ((G.nt[2] + 1) / 2);
+}
+
 template <int P>
 __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const int64_t* __restrict__ tstart, const float4* __restrict__ wsort,
                                              const int4* __restrict__ bsort, float* __restrict__ grid) {
@@ -387,7 +392,9 @@ __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const i
   static_assert(sizeof(sW) >= 8 * 8 * 64 * sizeof(float), "the tiles' partial sums reuse the rows' space");
   float(*s_part)[8][64] = reinterpret_cast<float(*)[8][64]>(sW);   // per tile: the eighths' sums by point (after the chunks)
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  const int nbx = G.nt[0] / 2, nby = G.nt[1] / 2;
+  // blocks of 2x2x2 tiles; with an odd tile count along an axis the last block along it has one tile (its second
+  // column's sums are not written)
+  const int nbx = (G.nt[0] + 1) / 2, nby = (G.nt[1] + 1) / 2;
   const int bx = blk % nbx, by = (blk / nbx) % nby, bz = blk / (nbx * nby);
   const int tx0 = 2 * bx, ty0 = 2 * by, tz0 = 2 * bz;
   // home tile h = ix + NA (iy + NA iz): tiles t0 − NTP + 1 + i along each axis (periodic)
@@ -522,14 +529,14 @@ __device__ __forceinline__ void b_spread_blk(int blk, const GridArgs& G, const i
 #pragma unroll
     for (int e = 1; e < 8; ++e) sum += s_part[t][e][lane];
     const int gx = 4 * (tx0 + dx) + (lane & 3), gy_ = 4 * (ty0 + dy) + ((lane >> 2) & 3), gz_ = 4 * (tz0 + dz) + (lane >> 4);
-    grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
+    if (tx0 + dx < G.nt[0] && ty0 + dy < G.nt[1] && tz0 + dz < G.nt[2]) grid[(size_t(gz_) * G.k[1] + gy_) * G.k[0] + gx] = sum;
   }
 }
 template <int P>
 __global__ void __launch_bounds__(kBlkThreads) k_spread_blk(GridArgs G, const int64_t* __restrict__ tstart, const float4* __restrict__ wsort,
                                                             const int4* __restrict__ bsort, float* __restrict__ grid, uint32_t* __restrict__ ovfc) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *ovfc = 0u;
-  const int nblk = (G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2);
+  const int nblk = spread_block_count(G);
   b_spread_blk<P>(int(xcd_block(blockIdx.x, nblk)), G, tstart, wsort, bsort, grid);
 }
 
@@ -1474,12 +1481,12 @@ __global__ void __launch_bounds__(64) k_gather(GridArgs G, const int64_t* __rest
   b_gather<P>(tile, G, tstart, tatoms, wsort, bsort, grid, f);
 }
 
-// the spread by blocks: every K a multiple of 8 (whole blocks of 2×2×2 tiles) and more tiles per axis than reach a
-// block's tile (so its home tiles are distinct)
+// the spread by blocks: every K a multiple of 4 (whole tiles; with an odd tile count the last block along that axis is
+// one tile deep) and more tiles per axis than reach a block's tile (so its home tiles are distinct)
 bool spread_blocks(const GridArgs& G) {
   const int ntp = (G.p + 6) / 4;
   for (int d = 0; d < 3; ++d)
-    if (G.k[d] % 8 != 0 || G.nt[d] < ntp + 1) return false;
+    if (G.k[d] % 4 != 0 || G.nt[d] < ntp + 1) return false;
   return true;
 }
 
@@ -1929,7 +1936,7 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   kt_end(c);
   kt_begin(c, "k_spread");
   const bool blocks = spread_blocks(G);
-  const unsigned nblk = unsigned((G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2));
+  const unsigned nblk = unsigned(spread_block_count(G));
   switch (G.p) {
 #define CAPSGPU_SPREAD(Q)                                                                                                          \
   case Q:                                                                                                                          \
@@ -2125,7 +2132,7 @@ PmeGroup* pme_group_create(const std::vector<capsgpu_ctx*>& cs, hipStream_t s) {
     // the spread each replica takes alone: by blocks of 2×2×2 tiles or by tiles
     const bool blocks = spread_blocks(G);
     osp.push_back(osp.back() + (blocks ? 0 : a.ntiles));
-    oblk.push_back(oblk.back() + (blocks ? (G.nt[0] / 2) * (G.nt[1] / 2) * (G.nt[2] / 2) : 0));
+    oblk.push_back(oblk.back() + (blocks ? spread_block_count(G) : 0));
     // the gather each replica takes alone: by runs of 64 sorted atoms or by tiles (the same forces)
     const bool runs = gather_by_runs(P, G);
     ogs.push_back(ogs.back() + (runs ? (c.n + 63) / 64 : 0));

@@ -30,9 +30,11 @@ def _oracle_recip(s, rc, rtol, spacing, order):
 
 
 # the grids: 27 (radix 3), 24 (8 · 3); spacing 0.7 on the triclinic cell gives 35 × 32 × 35 (radices 5, 7, 8, 4) and
-# 0.45 on the melt 56 (8 · 7)
+# 0.45 on the melt 56 (8 · 7); on the melt 0.70 gives 36 and 1.26 gives 20 (9 and 5 tiles a side: the block spread's
+# last block along each axis one tile deep)
 CASES = [("melt order 4", "melt", 4, 1.0), ("melt order 5", "melt", 5, 1.0), ("triclinic order 5", "tri", 5, 1.0),
-         ("triclinic order 6", "tri", 6, 1.0), ("triclinic order 5 radix 5 and 7", "tri", 5, 0.7), ("melt order 8 radix 7", "melt", 8, 0.45)]
+         ("triclinic order 6", "tri", 6, 1.0), ("triclinic order 5 radix 5 and 7", "tri", 5, 0.7), ("melt order 8 radix 7", "melt", 8, 0.45),
+         ("melt order 5 odd tile count 36", "melt", 5, 0.70), ("melt order 4 odd tile count 20", "melt", 4, 1.26)]
 
 
 @pytest.mark.parametrize("name,which,order,spacing", CASES, ids=[c[0] for c in CASES])
