@@ -1920,8 +1920,10 @@ void pme_forces(capsgpu_ctx& c, bool want) {
   P.tstart.resize(size_t(ntiles) + 1);
   // the count scan: its blocks' offsets added by k_tile_rank when there are few blocks (one launch less), else whole
   P.tlocal.resize(size_t(ntiles) + 1);
+  kt_begin(c, "k_scan_tiles");
   const int nsums = c.scan.blocks_only(P.tcount.p, P.tlocal.p, size_t(ntiles), s);
   if (!nsums) c.scan.exclusive(P.tcount.p, P.tstart.p, size_t(ntiles), s);
+  kt_end(c);
   P.tatoms.resize(size_t(n));
   P.wsort.resize(4 * size_t(n));
   P.bsort.resize(4 * size_t(n));
