@@ -1380,6 +1380,7 @@ __device__ __forceinline__ void b_gather_seq(int wg, int n, GridArgs G, const ui
   constexpr int NY = P + 3;                               // box points along b and c
   constexpr int NX4 = kSeqTiles + 2, RX = 4 * NX4;        // a box row: float4s, floats
   constexpr int PS = NY * RX + 4;                         // a box plane along c, floats
+  static_assert(NX4 <= 16, "a box row's float4s are loaded by the 16 lanes of a row of four");
   float* box = reinterpret_cast<float*>(lds);             // [c][b][a]: planes of PS floats, rows of RX
   float4* box4 = reinterpret_cast<float4*>(lds);
   const int lane = threadIdx.x, c4 = lane & 15, r4 = lane >> 4;
